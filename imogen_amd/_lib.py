@@ -1,0 +1,137 @@
+"""ctypes binding of libimogen_hip.so (the C ABI declared in include/imogen_hip.h).
+
+The shared library is built in-tree by ``imogen_amd/build.py``.  There is no fallback:
+if it is missing or a call fails, an exception is raised.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libimogen_hip.so")
+
+IMH_OK = 0
+STATUS = {-1: "IMH_EINVAL", -2: "IMH_EHIP", -3: "IMH_ENOMEM", -4: "IMH_ECFL",
+          -5: "IMH_ESTATE", -6: "IMH_ENOTSUP"}
+FP64, FP32_BULK = 0, 1
+
+FIELD_IDS = {"mass": 0, "momx": 1, "momy": 2, "momz": 3, "ener": 4, "magx": 5, "magy": 6,
+             "magz": 7, "cmagx": 8, "cmagy": 9, "cmagz": 10}
+
+
+class ImogenError(RuntimeError):
+    """A C-ABI call returned a non-zero status."""
+
+    def __init__(self, status, message):
+        super().__init__("%s: %s" % (STATUS.get(status, str(status)), message))
+        self.status = status
+
+
+class CFLError(ImogenError):
+    """TimeManager.update found no signal speed above 1e-2 (TimeManager.m:82-85)."""
+
+
+class ImhConfig(ctypes.Structure):
+    _fields_ = [
+        ("grid", ctypes.c_int * 3),
+        ("dgrid", ctypes.c_double * 3),
+        ("gamma", ctypes.c_double),
+        ("minmass", ctypes.c_double),
+        ("cfl", ctypes.c_double),
+        ("bc_infinity", ctypes.c_int),
+        ("bc", (ctypes.c_int * 3) * 2),
+        ("limiter", ctypes.c_int * 3),
+        ("fluid_active", ctypes.c_int),
+        ("magnet_active", ctypes.c_int),
+        ("update_mode", ctypes.c_int),
+        ("iter_max", ctypes.c_int),
+        ("time_max", ctypes.c_double),
+        ("perms_order", ctypes.c_int),
+        ("precision", ctypes.c_int),
+        ("device", ctypes.c_int),
+    ]
+
+
+_lib = None
+
+# Every entry point of include/imogen_hip.h with its argument types.
+_P = ctypes.c_void_p
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.c_int
+SIGNATURES = {
+    "imh_last_error": (ctypes.c_char_p, []),
+    "imh_version": (ctypes.c_char_p, []),
+    "imh_create": (_I, [ctypes.POINTER(ImhConfig), ctypes.POINTER(_P)]),
+    "imh_destroy": (_I, [_P]),
+    "imh_upload_state": (_I, [_P, _D, _D * 3, _D, _D * 3, ctypes.c_size_t]),
+    "imh_upload_field": (_I, [_P, _I, _D, ctypes.c_size_t]),
+    "imh_download_field": (_I, [_P, _I, _D, ctypes.c_size_t]),
+    "imh_fluid_sweep": (_I, [_P, _I]),
+    "imh_magnet_sweep": (_I, [_P, _I, _I * 2]),
+    "imh_time_update": (_I, [_P, _I]),
+    "imh_flux": (_I, [_P, _I]),
+    "imh_iteration": (_I, [_P]),
+    "imh_run": (_I, [_P, _I, ctypes.POINTER(_I)]),
+    "imh_set_dtime": (_I, [_P, ctypes.c_double]),
+    "imh_get_time": (_I, [_P, _D, _D, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+    "imh_set_iteration": (_I, [_P, _I]),
+    "imh_synchronize": (_I, [_P]),
+    "imh_conservation_sums": (_I, [_P, _D]),
+    "imh_memory_bytes": (_I, [_P, ctypes.POINTER(ctypes.c_size_t)]),
+    "imh_batch_create": (_I, [ctypes.POINTER(ImhConfig), _I, ctypes.POINTER(_P)]),
+    "imh_batch_destroy": (_I, [_P]),
+    "imh_batch_upload_state": (_I, [_P, _I, _D, _D * 3, _D, _D * 3, ctypes.c_size_t]),
+    "imh_batch_download_field": (_I, [_P, _I, _I, _D, ctypes.c_size_t]),
+    "imh_batch_step": (_I, [_P, _I]),
+    "imh_batch_get_time": (_I, [_P, _I, _D, _D, ctypes.POINTER(_I)]),
+    "imh_batch_synchronize": (_I, [_P]),
+    "imh_batch_memory_bytes": (_I, [_P, ctypes.POINTER(ctypes.c_size_t)]),
+}
+
+# Library-internal tuning hooks (not part of the drop-in boundary).
+EXTRA = {
+    "imh_set_fluid_tile": (_I, [_P, _I, _I, _I, _I]),
+    "imh_get_fluid_tile": (_I, [_P, _I, ctypes.POINTER(_I), ctypes.POINTER(_I),
+                                ctypes.POINTER(_I)]),
+    "imh_set_graphs": (_I, [_P, _I]),
+}
+
+
+def lib():
+    """Load the in-tree libimogen_hip.so; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("imogen_amd: %s is missing; build it with "
+                              "`python -m imogen_amd.build`" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in list(SIGNATURES.items()) + list(EXTRA.items()):
+            fn = getattr(L, name, None)
+            if fn is None:
+                continue
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != IMH_OK:
+        msg = lib().imh_last_error().decode(errors="replace")
+        if rc == -4:
+            raise CFLError(rc, msg)
+        raise ImogenError(rc, msg)
+    return rc
+
+
+def dptr(a):
+    return a.ctypes.data_as(_D)
+
+
+def as_field(a, ncell):
+    """A host array in the reference layout (x fastest) as contiguous float64."""
+    arr = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+    if arr.size != ncell:
+        raise ValueError("array has %d elements, the grid has %d cells" % (arr.size, ncell))
+    return arr

@@ -1,0 +1,228 @@
+// cfl.hip -- CFL timestep reduction (time/TimeManager.m:55-99), device-side time
+// bookkeeping, and the compensated conservation totals.
+//
+// The CFL maximum of |p_i/rho| + c_s for i = 1..3 is reduced per block with wavefront
+// shuffles and one LDS pass, then merged with one 64-bit atomicMax per block and
+// direction (non-negative doubles order like their bit patterns).  dt is computed on the
+// device, so an iteration needs no host round trip and can be replayed as a hipGraph.
+#include "imh_common.h"
+#include "imh_kernels.h"
+
+namespace imh {
+
+namespace {
+
+__device__ inline double wave_max(double v) {
+    for (int off = 32; off >= 1; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+__device__ inline void atomic_max_nonneg(unsigned long long* slot, double v) {
+    if (!(v >= 0.0)) return;  // NaN: MATLAB max ignores it
+    atomicMax(slot, (unsigned long long)__double_as_longlong(v));
+}
+
+}  // namespace
+
+template <typename T>
+__global__ void __launch_bounds__(256) cfl_kernel(const CflArgs<T> a) {
+    if (!a.sim->active) return;
+    __shared__ double s_red[3][4];
+    double m[3] = {NAN, NAN, NAN};
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < a.ncell; c += stride) {
+        double rho = double(a.q[0][c]);
+        double px = double(a.q[1][c]), py = double(a.q[2][c]), pz = double(a.q[3][c]);
+        double bx = a.bc[0] ? double(a.bc[0][c]) : 0.0;
+        double by = a.bc[1] ? double(a.bc[1][c]) : 0.0;
+        double bz = a.bc[2] ? double(a.bc[2][c]) : 0.0;
+        double cs = pressure_sound(rho, px, py, pz, double(a.q[4][c]), bx, by, bz, a.use, a.gamma);
+        m[0] = fmax(m[0], fabs(px / rho) + cs);
+        m[1] = fmax(m[1], fabs(py / rho) + cs);
+        m[2] = fmax(m[2], fabs(pz / rho) + cs);
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int k = 0; k < 3; ++k) {
+        double v = wave_max(m[k]);
+        if (lane == 0) s_red[k][wave] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        int k = threadIdx.x;
+        double v = fmax(fmax(s_red[k][0], s_red[k][1]), fmax(s_red[k][2], s_red[k][3]));
+        atomic_max_nonneg(&a.sim->cmax_bits[k], v);
+    }
+}
+
+template <typename T>
+hipError_t launch_cfl(const CflArgs<T>& a, hipStream_t st) {
+    long long nb = (a.ncell + 255) / 256;
+    if (nb > 2048) nb = 2048;
+    if (nb < 1) nb = 1;
+    hipLaunchKernelGGL((cfl_kernel<T>), dim3(nb), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+template hipError_t launch_cfl<double>(const CflArgs<double>&, hipStream_t);
+template hipError_t launch_cfl<float>(const CflArgs<float>&, hipStream_t);
+
+// ---- time bookkeeping (one thread per simulation) ------------------------------------
+
+// TimeManager.running (TimeManager.m:36-39), evaluated at the top of the loop body.
+__global__ void begin_iteration_kernel(SimDev* sim, const SimParams* p, int nsims) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsims) return;
+    SimDev& d = sim[s];
+    const SimParams& q = p[s];
+    int running = (d.iteration <= q.iter_max) && (d.time < q.time_max) && d.error == 0;
+    d.active = (q.unconditional || running) ? 1 : 0;
+}
+
+__global__ void cfl_reset_kernel(SimDev* sim, int nsims) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsims) return;
+    if (!sim[s].active) return;
+    for (int k = 0; k < 3; ++k) sim[s].cmax_bits[k] = 0ull;
+}
+
+// dt from the reduced speeds (TimeManager.m:65-98).
+__global__ void time_finalize_kernel(SimDev* sim, const SimParams* p, int nsims, int step) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsims) return;
+    SimDev& d = sim[s];
+    const SimParams& q = p[s];
+    if (!d.active) return;
+    if (step > 1 && q.update_mode != 0) return;
+    double cmax = 1e-2;
+    int gi = 0;
+    for (int k = 0; k < 3; ++k) {
+        double c = __longlong_as_double((long long)d.cmax_bits[k]);
+        if (c > cmax) {
+            cmax = c;
+            gi = k + 1;
+        }
+    }
+    if (gi < 1) {
+        d.error = IMH_ECFL;
+        d.active = 0;
+        return;
+    }
+    d.grid_index = gi;
+    double upd = double(q.update_mode + 1);
+    d.dt = q.cfl * q.dgrid[gi - 1] / cmax;
+    double newtime = d.time + upd * d.dt;
+    if (newtime > q.time_max) {
+        d.dt = (q.time_max - d.time) / upd;
+        newtime = q.time_max;
+    }
+    d.time = newtime;
+}
+
+// TimeManager.step: iteration = iteration + 1.
+__global__ void end_iteration_kernel(SimDev* sim, int nsims) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsims) return;
+    if (sim[s].active) sim[s].iteration += 1;
+}
+
+static inline int nblk(int n) { return (n + 255) / 256; }
+
+hipError_t launch_begin_iteration(SimDev* sim, const SimParams* p, int nsims, hipStream_t st) {
+    hipLaunchKernelGGL(begin_iteration_kernel, dim3(nblk(nsims)), dim3(256), 0, st, sim, p, nsims);
+    return hipGetLastError();
+}
+hipError_t launch_cfl_reset(SimDev* sim, int nsims, hipStream_t st) {
+    hipLaunchKernelGGL(cfl_reset_kernel, dim3(nblk(nsims)), dim3(256), 0, st, sim, nsims);
+    return hipGetLastError();
+}
+hipError_t launch_time_finalize(SimDev* sim, const SimParams* p, int nsims, int step,
+                                hipStream_t st) {
+    hipLaunchKernelGGL(time_finalize_kernel, dim3(nblk(nsims)), dim3(256), 0, st, sim, p, nsims,
+                       step);
+    return hipGetLastError();
+}
+hipError_t launch_end_iteration(SimDev* sim, int nsims, hipStream_t st) {
+    hipLaunchKernelGGL(end_iteration_kernel, dim3(nblk(nsims)), dim3(256), 0, st, sim, nsims);
+    return hipGetLastError();
+}
+
+// ---- conservation totals -------------------------------------------------------------
+// Neumaier-compensated sums; every block owns a fixed contiguous range and combines its
+// 256 partials in a fixed binary tree, the block partials are combined the same way, so
+// the result is independent of timing and of the device.
+
+struct CSum {
+    double s, c;
+};
+
+__device__ inline CSum csum_add(CSum a, double x) {
+    double t = a.s + x;
+    double e = (fabs(a.s) >= fabs(x)) ? (a.s - t) + x : (x - t) + a.s;
+    return {t, a.c + e};
+}
+
+__device__ inline CSum csum_merge(CSum a, CSum b) {
+    CSum r = csum_add(a, b.s);
+    r.c += b.c;
+    return r;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) csum_blocks_kernel(const T* x, long long n, long long chunk,
+                                                          double* partials) {
+    __shared__ double ss[256], sc[256];
+    const long long b0 = (long long)blockIdx.x * chunk;
+    long long b1 = b0 + chunk;
+    if (b1 > n) b1 = n;
+    CSum acc = {0.0, 0.0};
+    const long long per = (chunk + 255) / 256;
+    long long s0 = b0 + (long long)threadIdx.x * per;
+    long long s1 = s0 + per;
+    if (s1 > b1) s1 = b1;
+    for (long long i = s0; i < s1; ++i) acc = csum_add(acc, double(x[i]));
+    ss[threadIdx.x] = acc.s;
+    sc[threadIdx.x] = acc.c;
+    __syncthreads();
+    for (int w = 128; w >= 1; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            CSum r = csum_merge({ss[threadIdx.x], sc[threadIdx.x]},
+                                {ss[threadIdx.x + w], sc[threadIdx.x + w]});
+            ss[threadIdx.x] = r.s;
+            sc[threadIdx.x] = r.c;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partials[2 * blockIdx.x] = ss[0];
+        partials[2 * blockIdx.x + 1] = sc[0];
+    }
+}
+
+__global__ void csum_final_kernel(const double* partials, int nb, double* out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    CSum acc = {0.0, 0.0};
+    for (int b = 0; b < nb; ++b) acc = csum_merge(acc, {partials[2 * b], partials[2 * b + 1]});
+    *out = acc.s + acc.c;
+}
+
+template <typename T>
+hipError_t launch_conservation(const T* const* fields, int nfields, long long ncell,
+                               double* partials, int nblocks, double* out, hipStream_t st) {
+    long long chunk = (ncell + nblocks - 1) / nblocks;
+    if (chunk < 1) chunk = 1;
+    int nb = int((ncell + chunk - 1) / chunk);
+    for (int f = 0; f < nfields; ++f) {
+        hipLaunchKernelGGL((csum_blocks_kernel<T>), dim3(nb), dim3(256), 0, st, fields[f], ncell,
+                           chunk, partials + (size_t)f * 2 * nblocks);
+        hipLaunchKernelGGL(csum_final_kernel, dim3(1), dim3(1), 0, st,
+                           partials + (size_t)f * 2 * nblocks, nb, out + f);
+    }
+    return hipGetLastError();
+}
+
+template hipError_t launch_conservation<double>(const double* const*, int, long long, double*, int,
+                                                double*, hipStream_t);
+template hipError_t launch_conservation<float>(const float* const*, int, long long, double*, int,
+                                               double*, hipStream_t);
+
+}  // namespace imh

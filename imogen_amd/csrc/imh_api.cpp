@@ -1,0 +1,831 @@
+// imh_api.cpp -- host side of the C ABI (include/imogen_hip.h) for one simulation
+// context: device memory, the boundary-condition edge-tracker store, the sweep order of
+// flux.m, the time loop of imogen.m and the hipGraph replay of an iteration.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "imh_common.h"
+#include "imh_host.h"
+#include "imh_kernels.h"
+
+using namespace imh;
+
+namespace imh {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+    return fail(IMH_EHIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+// perms([1 2 3]) rows (flux.m:19).  Order 0: the recursive construction of MATLAB
+// R2009b-R2010b perms.m; order 1: reverse lexicographic (later releases).
+static const int PERMS_R2010[6][3] = {{3, 2, 1}, {3, 1, 2}, {2, 3, 1},
+                                      {2, 1, 3}, {1, 2, 3}, {1, 3, 2}};
+static const int PERMS_LEXREV[6][3] = {{3, 2, 1}, {3, 1, 2}, {2, 3, 1},
+                                       {2, 1, 3}, {1, 3, 2}, {1, 2, 3}};
+
+void sweep_order(int iteration, int order, int perms_order, int dirs[3], int mi[3][2]) {
+    const int(*P)[3] = perms_order ? PERMS_LEXREV : PERMS_R2010;
+    int row = ((iteration - 1) % 6 + 6) % 6;
+    static const int FWD[3][2] = {{2, 3}, {1, 3}, {1, 2}};
+    static const int BWD[3][2] = {{3, 2}, {3, 1}, {2, 1}};
+    for (int t = 0; t < 3; ++t) {
+        if (order > 0) {
+            dirs[t] = P[row][t];
+            mi[t][0] = FWD[dirs[t] - 1][0];
+            mi[t][1] = FWD[dirs[t] - 1][1];
+        } else {
+            dirs[t] = 4 - P[row][t];  // perms([3 2 1]) = V(perms(1:3)) with V = [3 2 1]
+            mi[t][0] = BWD[dirs[t] - 1][0];
+            mi[t][1] = BWD[dirs[t] - 1][1];
+        }
+    }
+}
+
+int validate_config(const imh_config* c) {
+    for (int d = 0; d < 3; ++d)
+        if (c->grid[d] < 1) return fail(IMH_EINVAL, "grid[%d] must be >= 1", d);
+    if (c->grid[2] > 1 && (c->grid[0] == 1 || c->grid[1] == 1))
+        return fail(IMH_ENOTSUP,
+                    "a grid with nz > 1 and a singleton x or y is re-ordered by squeeze() "
+                    "in the reference (FluidArray.m:174)");
+    for (int d = 0; d < 3; ++d)
+        if (c->dgrid[d] <= 0) return fail(IMH_EINVAL, "dgrid[%d] must be > 0", d);
+    if (c->precision != IMH_FP64 && c->precision != IMH_FP32_BULK)
+        return fail(IMH_EINVAL, "unknown precision %d", c->precision);
+    const int ndims = c->grid[2] > 1 ? 3 : 2;
+    for (int s = 0; s < 2; ++s)
+        for (int d = 0; d < 3; ++d) {
+            int m = c->bc[s][d];
+            if (m < IMH_BC_CIRC || m > IMH_BC_ZERO)
+                return fail(IMH_EINVAL, "Imogen:UnknownType: Unknown BC mode %d", m);
+            if (d >= ndims) continue;  // every dim <= ndims is shifted (MagnetArray.m:131)
+            if (m == IMH_BC_FADE &&
+                (c->bc_infinity < 4 || c->bc_infinity > MAX_FADE || c->grid[d] < c->bc_infinity))
+                return fail(IMH_ENOTSUP,
+                            "fade BC needs 4 <= bcInfinity <= %d and >= bcInfinity cells "
+                            "along dim %d (fade_shift.m:36)", MAX_FADE, d + 1);
+            if (m == IMH_BC_TRANS && c->bc_infinity <= 3)
+                return fail(IMH_EINVAL, "transparent BC needs bcInfinity > 3 (pchip nodes)");
+            if (m == IMH_BC_MIRROR && c->grid[d] < 2)
+                return fail(IMH_ENOTSUP, "mirror BC needs >= 2 cells along dim %d", d + 1);
+        }
+    for (int d = 0; d < 3; ++d)
+        if (c->limiter[d] < 0 || c->limiter[d] > 2)
+            return fail(IMH_EINVAL, "Imogen:UnknownType: Unknown flux limiter type %d",
+                        c->limiter[d]);
+    if (c->update_mode != 0 && c->update_mode != 1)
+        return fail(IMH_EINVAL, "timeUpdateMode must be 0 or 1");
+    if (c->cfl <= 0) return fail(IMH_EINVAL, "cfl must be > 0");
+    return IMH_OK;
+}
+
+static int next_pow2(int v) {
+    int p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+// Tile of the fused fluid sweep for pencils of length N (fluid_sweep_impl.h).  Compiled
+// variants: FP64 (bs, kmax) in {(256,1), (256,2), (512,2), (512,4)}; FP32 in {(256,2),
+// (256,4), (512,4), (512,8)}.  Pencils along x are contiguous (threads of a pencil are
+// adjacent lanes); along y/z several adjacent pencils share a block so that each row
+// access is a contiguous run.
+bool fluid_variant_exists(int esz, int bs, int kmax) {
+    if (esz == 8)
+        return (bs == 256 && (kmax == 1 || kmax == 2)) || (bs == 512 && (kmax == 2 || kmax == 4));
+    return (bs == 256 && (kmax == 2 || kmax == 4)) || (bs == 512 && (kmax == 4 || kmax == 8));
+}
+
+FluidTile choose_fluid_tile(int N, int d, int esz, const TileOverride* ov) {
+    FluidTile t;
+    t.xmajor = (d == 0);
+    if (ov && ov->valid) {
+        t.bs = ov->bs;
+        t.kmax = ov->kmax;
+        t.PX = ov->px;
+    } else {
+        const int kpref = esz == 8 ? 2 : 4;
+        int tl = next_pow2((N + kpref - 1) / kpref);
+        if (tl < 8) tl = 8;
+        if (tl <= 256 && t.xmajor) {
+            t.bs = 256;
+            t.kmax = kpref;
+        } else if (tl <= 512) {
+            t.bs = 512;
+            t.kmax = kpref;
+        } else {
+            t.bs = 512;
+            t.kmax = 2 * kpref;
+            tl = next_pow2((N + t.kmax - 1) / t.kmax);
+            if (tl < 8) tl = 8;
+        }
+        t.PX = t.bs / (tl < t.bs ? tl : t.bs);
+    }
+    if (t.PX > 32) t.PX = 32;
+    if (t.PX < 1) t.PX = 1;
+    t.TL = t.bs / t.PX;
+    t.K = (N + t.TL - 1) / t.TL;
+    if (t.K < 1) t.K = 1;
+    return t;
+}
+
+MagTile choose_mag_tile(int N) {
+    MagTile t;
+    int tl = next_pow2((N + 3) / 4);
+    if (tl > 256) tl = 256;
+    if (tl < 16) tl = 16;
+    t.TL = tl;
+    t.PX = 256 / tl;
+    return t;
+}
+
+}  // namespace imh
+
+// ======================================================================================
+// context
+// ======================================================================================
+
+static size_t esize(const imh_ctx* c) { return c->cfg.precision == IMH_FP64 ? 8 : 4; }
+
+static void ctx_free(imh_ctx* c) {
+    if (!c) return;
+    for (int i = 0; i < 6; ++i)
+        if (c->graph[i]) hipGraphExecDestroy(c->graph[i]);
+    for (int i = 0; i < IMH_NFIELDS; ++i)
+        if (c->f[i]) hipFree(c->f[i]);
+    if (c->scratch) hipFree(c->scratch);
+    if (c->edges) hipFree(c->edges);
+    if (c->eoff_dev) hipFree(c->eoff_dev);
+    if (c->sim) hipFree(c->sim);
+    if (c->par) hipFree(c->par);
+    if (c->par_uncond) hipFree(c->par_uncond);
+    if (c->fade_dev) hipFree(c->fade_dev);
+    if (c->csum_part) hipFree(c->csum_part);
+    if (c->csum_out) hipFree(c->csum_out);
+    if (c->st) hipStreamDestroy(c->st);
+    delete c;
+}
+
+// Offsets of the edge stores (objects x sides x dims) for trans/wall sides.
+static void layout_edges(imh_ctx* c) {
+    long long off = 0;
+    for (int o = 0; o < NOBJ; ++o)
+        for (int s = 0; s < 2; ++s)
+            for (int d = 0; d < 3; ++d) c->eoff[o][s][d] = 0;
+    auto exists = [&](int o) {
+        if (o < OBJ_MAG + 3) return true;
+        if (!c->cfg.magnet_active) return false;
+        int k = (o - OBJ_VG) % 9;
+        return (k / 3) != (k % 3);  // (I, X) pairs with I != X
+    };
+    for (int o = 0; o < NOBJ; ++o) {
+        if (o >= OBJ_MFLUX + 9) break;
+        if (!exists(o)) continue;
+        for (int s = 0; s < 2; ++s)
+            for (int d = 0; d < c->ndims; ++d) {
+                int m = c->cfg.bc[s][d];
+                if (m != IMH_BC_TRANS && m != IMH_BC_WALL) continue;
+                c->eoff[o][s][d] = off;
+                off += c->nface[d];
+            }
+    }
+    c->edges_len = off > 0 ? off : 1;
+}
+
+extern "C" {
+
+const char* imh_last_error(void) { return imh::g_last_error.c_str(); }
+
+const char* imh_version(void) { return "imogen-hip 0.1 (gfx950)"; }
+
+int imh_create(const imh_config* cfg, imh_ctx** out) {
+    if (!cfg || !out) return fail(IMH_EINVAL, "null argument");
+    *out = nullptr;
+    int rc = validate_config(cfg);
+    if (rc) return rc;
+    hipError_t e = hipSetDevice(cfg->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    imh_ctx* c = new imh_ctx();
+    c->cfg = *cfg;
+    for (int d = 0; d < 3; ++d) c->n[d] = cfg->grid[d];
+    c->ndims = cfg->grid[2] > 1 ? 3 : 2;
+    c->ncell = (long long)c->n[0] * c->n[1] * c->n[2];
+    for (int d = 0; d < 3; ++d) c->nface[d] = c->ncell / c->n[d];
+    for (int d = 0; d < 3; ++d) {
+        int L = d;  // Edges.ACTIVE(D) with linear index D = d+1 into the 2x3 array
+        int row = L % 2, col = L / 2;
+        int m = cfg->bc[row][col];
+        c->active_lin[d] = (col < c->ndims) &&
+                           (m == IMH_BC_TRANS || m == IMH_BC_FADE || m == IMH_BC_WALL);
+    }
+    if (cfg->bc_infinity >= 4 && cfg->bc_infinity <= MAX_FADE)
+        fade_coefficients(cfg->bc_infinity, c->fade);
+    const size_t es = esize(c);
+    size_t bytes = (size_t)c->ncell * es;
+#define CHK(x, what)                                    \
+    do {                                                \
+        hipError_t _e = (x);                            \
+        if (_e != hipSuccess) {                         \
+            int _r = (_e == hipErrorOutOfMemory) ? fail(IMH_ENOMEM, "%s: out of device memory", what) \
+                                                 : hip_fail(_e, what); \
+            ctx_free(c);                                \
+            return _r;                                  \
+        }                                               \
+    } while (0)
+    CHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking), "hipStreamCreate");
+    for (int i = 0; i < IMH_NFIELDS; ++i) CHK(hipMalloc(&c->f[i], bytes), "hipMalloc(field)");
+    if (cfg->magnet_active) CHK(hipMalloc(&c->scratch, bytes), "hipMalloc(scratch)");
+    layout_edges(c);
+    CHK(hipMalloc(&c->edges, c->edges_len * sizeof(double)), "hipMalloc(edges)");
+    CHK(hipMemset(c->edges, 0, c->edges_len * sizeof(double)), "hipMemset(edges)");
+    CHK(hipMalloc(&c->eoff_dev, sizeof(c->eoff)), "hipMalloc(eoff)");
+    CHK(hipMemcpy(c->eoff_dev, c->eoff, sizeof(c->eoff), hipMemcpyHostToDevice), "eoff upload");
+    CHK(hipMalloc(&c->fade_dev, sizeof(c->fade)), "hipMalloc(fade)");
+    CHK(hipMemcpy(c->fade_dev, c->fade, sizeof(c->fade), hipMemcpyHostToDevice), "fade upload");
+    CHK(hipMalloc(&c->sim, sizeof(SimDev)), "hipMalloc(sim)");
+    CHK(hipMalloc(&c->par, sizeof(SimParams)), "hipMalloc(par)");
+    CHK(hipMalloc(&c->csum_part, sizeof(double) * 2 * 1024 * 8), "hipMalloc(csum)");
+    CHK(hipMalloc(&c->csum_out, sizeof(double) * 8), "hipMalloc(csum_out)");
+    SimDev sd;
+    memset(&sd, 0, sizeof(sd));
+    sd.iteration = 1;  // imogen.m:50
+    sd.active = 1;
+    CHK(hipMemcpy(c->sim, &sd, sizeof(sd), hipMemcpyHostToDevice), "sim upload");
+    c->host_par.cfl = cfg->cfl;
+    for (int d = 0; d < 3; ++d) c->host_par.dgrid[d] = cfg->dgrid[d];
+    c->host_par.time_max = cfg->time_max;
+    c->host_par.iter_max = cfg->iter_max;
+    c->host_par.update_mode = cfg->update_mode;
+    c->host_par.unconditional = 0;
+    CHK(hipMemcpy(c->par, &c->host_par, sizeof(SimParams), hipMemcpyHostToDevice), "par upload");
+    for (int d = 0; d < 3; ++d) {
+        c->ftile[d] = choose_fluid_tile(c->n[d], d, (int)es, nullptr);
+        c->mtile[d] = choose_mag_tile(c->n[d]);
+        if (c->n[d] >= 3 && c->ftile[d].K > c->ftile[d].kmax) {
+            int maxn = c->ftile[d].kmax * 512;
+            ctx_free(c);
+            return fail(IMH_ENOTSUP, "pencils of %d cells along dim %d exceed the longest "
+                        "compiled sweep tile (%d cells at this precision)", cfg->grid[d], d + 1, maxn);
+        }
+    }
+    c->host_iteration = 1;
+#undef CHK
+    *out = c;
+    return IMH_OK;
+}
+
+int imh_destroy(imh_ctx* c) {
+    if (!c) return IMH_OK;
+    hipStreamSynchronize(c->st);
+    ctx_free(c);
+    return IMH_OK;
+}
+
+}  // extern "C"
+
+// ---- state --------------------------------------------------------------------------
+
+static int upload_array(imh_ctx* c, void* dst, const double* src) {
+    hipError_t e;
+    if (c->cfg.precision == IMH_FP64) {
+        e = hipMemcpyAsync(dst, src, sizeof(double) * c->ncell, hipMemcpyHostToDevice, c->st);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    } else {
+        std::vector<float> tmp(c->ncell);
+        for (long long i = 0; i < c->ncell; ++i) tmp[i] = (float)src[i];
+        e = hipMemcpyAsync(dst, tmp.data(), sizeof(float) * c->ncell, hipMemcpyHostToDevice, c->st);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    }
+    return e == hipSuccess ? IMH_OK : hip_fail(e, "upload");
+}
+
+// sumND (utils/sumND.m): sum along the last dim first, down to dim 1.
+static double sum_nd(const imh_ctx* c, const double* a) {
+    long long nx = c->n[0], ny = c->n[1], nz = c->n[2];
+    std::vector<double> plane(nx * ny), col(nx);
+    for (long long ij = 0; ij < nx * ny; ++ij) {
+        double t = 0.0;
+        for (long long k = 0; k < nz; ++k) t += a[ij + nx * ny * k];
+        plane[ij] = t;
+    }
+    for (long long i = 0; i < nx; ++i) {
+        double t = 0.0;
+        for (long long j = 0; j < ny; ++j) t += plane[i + nx * j];
+        col[i] = t;
+    }
+    double tot = 0.0;
+    for (long long i = 0; i < nx; ++i) tot += col[i];
+    return tot;
+}
+
+// Edge stores of an object built from an array (Edges.m:37-108): face slices of the
+// trans/wall sides.  `h` is the host copy of the edges buffer.
+static void host_edges_from(imh_ctx* c, std::vector<double>& h, int obj, const double* a) {
+    for (int s = 0; s < 2; ++s)
+        for (int d = 0; d < c->ndims; ++d) {
+            int m = c->cfg.bc[s][d];
+            if (m != IMH_BC_TRANS && m != IMH_BC_WALL) continue;
+            long long off = c->eoff[obj][s][d];
+            long long N = c->n[d];
+            for (long long fi = 0; fi < c->nface[d]; ++fi) {
+                long long base, stride;
+                pencil_base(c->n, d, fi, base, stride);
+                h[off + fi] = a[base + (s ? (N - 1) : 0) * stride];
+            }
+        }
+}
+
+template <typename T>
+static int launch_cellmag(imh_ctx* c, int k, int force) {
+    CellMagArgs<T> a;
+    memset(&a, 0, sizeof(a));
+    a.b = (const T*)c->f[IMH_F_MAGX + k];
+    a.cb = (T*)c->f[IMH_F_CMAGX + k];
+    a.edges = c->edges;
+    a.eoff = c->eoff_dev;
+    a.sim = c->sim;
+    a.n0 = c->n[0];
+    a.n1 = c->n[1];
+    a.n2 = c->n[2];
+    a.k = k;
+    a.ndims = c->ndims;
+    a.modes = pack_modes(c->cfg.bc);
+    a.active_lin = c->active_lin[k];
+    a.bc_inf = c->cfg.bc_infinity;
+    a.force = force;
+    a.fade = c->fade_dev;
+    hipError_t e = launch_cell_centered(a, c->st);
+    return e == hipSuccess ? IMH_OK : hip_fail(e, "cell_centered launch");
+}
+
+static int cellmag(imh_ctx* c, int k, int force) {
+    return c->cfg.precision == IMH_FP64 ? launch_cellmag<double>(c, k, force)
+                                        : launch_cellmag<float>(c, k, force);
+}
+
+extern "C" int imh_upload_state(imh_ctx* c, const double* mass, const double* const mom[3],
+                                const double* ener, const double* const mag[3], size_t n) {
+    if (!c || !mass || !mom || !ener || !mag) return fail(IMH_EINVAL, "null argument");
+    if ((long long)n != c->ncell)
+        return fail(IMH_EINVAL, "array length %zu does not match the grid (%lld cells)", n,
+                    c->ncell);
+    for (int k = 0; k < 3; ++k)
+        if (!mom[k] || !mag[k]) return fail(IMH_EINVAL, "null component array");
+    const double* src[8] = {mass, mom[0], mom[1], mom[2], ener, mag[0], mag[1], mag[2]};
+    for (int i = 0; i < 8; ++i) {
+        int rc = upload_array(c, c->f[i], src[i]);
+        if (rc) return rc;
+    }
+    // edge stores of the FluidArray and MagnetArray objects come from the IC
+    std::vector<double> h(c->edges_len, 0.0);
+    for (int i = 0; i < 5; ++i) host_edges_from(c, h, OBJ_PRIM + i, src[i]);
+    for (int k = 0; k < 3; ++k) host_edges_from(c, h, OBJ_MAG + k, src[5 + k]);
+    hipError_t e = hipMemcpy(c->edges, h.data(), sizeof(double) * c->edges_len,
+                             hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, "edges upload");
+    // MagnetArray.isZero = (sumND(array) == 0) && ~run.magnet.ACTIVE
+    bool anyB = false;
+    for (int k = 0; k < 3; ++k) {
+        c->mag_is_zero[k] = (sum_nd(c, src[5 + k]) == 0.0) && !c->cfg.magnet_active;
+        for (long long i = 0; i < c->ncell && !anyB; ++i) anyB = src[5 + k][i] != 0.0;
+    }
+    c->mhd_kernel = c->cfg.magnet_active || anyB;
+    // time state of a new run (imogen.m:50)
+    SimDev sd;
+    memset(&sd, 0, sizeof(sd));
+    sd.iteration = 1;
+    sd.active = 1;
+    e = hipMemcpy(c->sim, &sd, sizeof(sd), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, "sim reset");
+    c->host_iteration = 1;
+    c->fluid_init = false;
+    for (int o = 0; o < NOBJ; ++o) c->obj_init[o] = false;
+    for (int i = 0; i < 6; ++i) {
+        if (c->graph[i]) hipGraphExecDestroy(c->graph[i]);
+        c->graph[i] = nullptr;
+    }
+    // cellMag from updateCellCentered in the MagnetArray constructor (MagnetArray.m:197)
+    for (int k = 0; k < 3; ++k) {
+        int rc = cellmag(c, k, 1);
+        if (rc) return rc;
+    }
+    e = hipStreamSynchronize(c->st);
+    if (e != hipSuccess) return hip_fail(e, "cellMag");
+    c->have_state = true;
+    return IMH_OK;
+}
+
+extern "C" int imh_upload_field(imh_ctx* c, int field, const double* data, size_t n) {
+    if (!c || !data) return fail(IMH_EINVAL, "null argument");
+    if (field < 0 || field >= IMH_NFIELDS) return fail(IMH_EINVAL, "bad field id %d", field);
+    if ((long long)n != c->ncell) return fail(IMH_EINVAL, "array length mismatch");
+    return upload_array(c, c->f[field], data);
+}
+
+extern "C" int imh_download_field(imh_ctx* c, int field, double* out, size_t n) {
+    if (!c || !out) return fail(IMH_EINVAL, "null argument");
+    if (field < 0 || field >= IMH_NFIELDS) return fail(IMH_EINVAL, "bad field id %d", field);
+    if ((long long)n != c->ncell) return fail(IMH_EINVAL, "array length mismatch");
+    hipError_t e = hipStreamSynchronize(c->st);
+    if (e != hipSuccess) return hip_fail(e, "stream sync");
+    if (c->cfg.precision == IMH_FP64) {
+        e = hipMemcpy(out, c->f[field], sizeof(double) * n, hipMemcpyDeviceToHost);
+    } else {
+        std::vector<float> tmp(n);
+        e = hipMemcpy(tmp.data(), c->f[field], sizeof(float) * n, hipMemcpyDeviceToHost);
+        for (size_t i = 0; i < n; ++i) out[i] = tmp[i];
+    }
+    return e == hipSuccess ? IMH_OK : hip_fail(e, "download");
+}
+
+// ---- sweeps -------------------------------------------------------------------------
+
+template <typename T>
+static int fluid_sweep_t(imh_ctx* c, int d, int force) {
+    FluidArgs<T> a;
+    memset(&a, 0, sizeof(a));
+    for (int v = 0; v < 5; ++v) a.q[v] = (T*)c->f[v];
+    for (int k = 0; k < 3; ++k) a.bc[k] = (const T*)c->f[IMH_F_CMAGX + k];
+    a.edges = c->edges;
+    a.eoff = c->eoff_dev;
+    a.sim = c->sim;
+    a.n0 = c->n[0];
+    a.n1 = c->n[1];
+    a.n2 = c->n[2];
+    a.d = d;
+    a.ndims = c->ndims;
+    a.modes = pack_modes(c->cfg.bc);
+    a.active_lin = c->active_lin[d];
+    a.limiter = c->cfg.limiter[d];
+    a.use = {!c->mag_is_zero[0], !c->mag_is_zero[1], !c->mag_is_zero[2]};
+    a.gamma = c->cfg.gamma;
+    a.minmass = c->cfg.minmass;
+    a.dx = c->cfg.dgrid[d];
+    a.init_edges = c->fluid_init ? 0 : 1;
+    const FluidTile& t = c->ftile[d];
+    a.PX = t.PX;
+    a.TL = t.TL;
+    a.K = t.K;
+    a.xmajor = t.xmajor;
+    a.npencil = c->nface[d];
+    a.bc_inf = c->cfg.bc_infinity;
+    a.xcd_remap = 1;
+    a.force = force;
+    a.fade = c->fade_dev;
+    hipError_t e = launch_fluid_sweep(a, t.bs, t.kmax, c->mhd_kernel, c->st);
+    if (e != hipSuccess) return hip_fail(e, "fluid sweep launch");
+    c->fluid_init = true;
+    return IMH_OK;
+}
+
+int imh_fluid_sweep_internal(imh_ctx* c, int X, int force) {
+    if (!c->have_state) return fail(IMH_ESTATE, "state not uploaded");
+    if (X < 1 || X > 3) return fail(IMH_EINVAL, "X must be 1, 2 or 3");
+    if (c->n[X - 1] < 3)
+        return fail(IMH_ENOTSUP, "sweep along a dimension with < 3 cells (flux.m:39 skips it)");
+    return c->cfg.precision == IMH_FP64 ? fluid_sweep_t<double>(c, X - 1, force)
+                                        : fluid_sweep_t<float>(c, X - 1, force);
+}
+
+template <typename T>
+static int magnet_sweep_t(imh_ctx* c, int dX, const int I[2], int force) {
+    int lastI = I[1] - 1;
+    for (int t = 0; t < 2; ++t) {
+        int dI = I[t] - 1;
+        MagnetArgs<T> a;
+        memset(&a, 0, sizeof(a));
+        a.rho = (const T*)c->f[IMH_F_MASS];
+        a.mom = (const T*)c->f[IMH_F_MOMX + dX];
+        a.bI = (T*)c->f[IMH_F_MAGX + dI];
+        a.bX = (T*)c->f[IMH_F_MAGX + dX];
+        a.flux = (T*)c->scratch;
+        a.edges = c->edges;
+        a.eoff = c->eoff_dev;
+        a.sim = c->sim;
+        a.n0 = c->n[0];
+        a.n1 = c->n[1];
+        a.n2 = c->n[2];
+        a.dX = dX;
+        a.dI = dI;
+        a.ndims = c->ndims;
+        a.modes = pack_modes(c->cfg.bc);
+        a.active_linX = c->active_lin[dX];
+        a.active_linI = c->active_lin[dI];
+        a.limiter = c->cfg.limiter[dX];
+        a.dxX = c->cfg.dgrid[dX];
+        a.dxI = c->cfg.dgrid[dI];
+        int pair = 3 * dI + dX;
+        a.init_vg = !c->obj_init[OBJ_VG + pair];
+        a.init_store = !c->obj_init[OBJ_MSFR + pair];
+        a.init_wmag = !c->obj_init[OBJ_WMAG + pair];
+        a.init_flux = !c->obj_init[OBJ_MFLUX + pair];
+        a.PX = c->mtile[dX].PX;
+        a.TL = c->mtile[dX].TL;
+        a.npencil = c->nface[dX];
+        a.bc_inf = c->cfg.bc_infinity;
+        a.force = force;
+        a.fade = c->fade_dev;
+        hipError_t e = launch_magnet_advect(a, 256, c->st);
+        if (e != hipSuccess) return hip_fail(e, "magnet advect launch");
+        e = launch_magnet_ct(a, c->st);
+        if (e != hipSuccess) return hip_fail(e, "magnet ct launch");
+        c->obj_init[OBJ_VG + pair] = c->obj_init[OBJ_MSFR + pair] = true;
+        c->obj_init[OBJ_WMAG + pair] = c->obj_init[OBJ_MFLUX + pair] = true;
+    }
+    // mag(I).updateCellCentered() for the last I only, then mag(X) (magnetFlux.m:30-31)
+    int rc = c->cfg.precision == IMH_FP64 ? launch_cellmag<double>(c, lastI, force)
+                                          : launch_cellmag<float>(c, lastI, force);
+    if (rc) return rc;
+    return c->cfg.precision == IMH_FP64 ? launch_cellmag<double>(c, dX, force)
+                                        : launch_cellmag<float>(c, dX, force);
+}
+
+int imh_magnet_sweep_internal(imh_ctx* c, int X, const int I[2], int force) {
+    if (!c->have_state) return fail(IMH_ESTATE, "state not uploaded");
+    if (!c->cfg.magnet_active || !c->scratch)
+        return fail(IMH_ESTATE, "magnet sweep on a context created with magnet_active = 0");
+    if (X < 1 || X > 3 || !I) return fail(IMH_EINVAL, "bad direction");
+    for (int t = 0; t < 2; ++t)
+        if (I[t] < 1 || I[t] > 3 || I[t] == X) return fail(IMH_EINVAL, "bad magnetic index");
+    if (c->n[X - 1] < 3) return fail(IMH_ENOTSUP, "magnet sweep along a dimension with < 3 cells");
+    return c->cfg.precision == IMH_FP64 ? magnet_sweep_t<double>(c, X - 1, I, force)
+                                        : magnet_sweep_t<float>(c, X - 1, I, force);
+}
+
+template <typename T>
+static int cfl_t(imh_ctx* c) {
+    CflArgs<T> a;
+    for (int v = 0; v < 5; ++v) a.q[v] = (const T*)c->f[v];
+    for (int k = 0; k < 3; ++k) a.bc[k] = (const T*)c->f[IMH_F_CMAGX + k];
+    a.sim = c->sim;
+    a.ncell = c->ncell;
+    a.use = {!c->mag_is_zero[0], !c->mag_is_zero[1], !c->mag_is_zero[2]};
+    a.gamma = c->cfg.gamma;
+    hipError_t e = launch_cfl(a, c->st);
+    return e == hipSuccess ? IMH_OK : hip_fail(e, "cfl launch");
+}
+
+int imh_time_update_internal(imh_ctx* c, int step) {
+    if (step > 1 && c->cfg.update_mode != 0) return IMH_OK;  // TimeManager.m:61-63
+    hipError_t e = launch_cfl_reset(c->sim, 1, c->st);
+    if (e != hipSuccess) return hip_fail(e, "cfl reset");
+    int rc = c->cfg.precision == IMH_FP64 ? cfl_t<double>(c) : cfl_t<float>(c);
+    if (rc) return rc;
+    e = launch_time_finalize(c->sim, c->par, 1, step, c->st);
+    return e == hipSuccess ? IMH_OK : hip_fail(e, "time finalize");
+}
+
+int imh_flux_internal(imh_ctx* c, int order, int force) {
+    if (order != 1 && order != -1) return fail(IMH_EINVAL, "%d is not a recognized direction", order);
+    int dirs[3], mi[3][2];
+    sweep_order(c->host_iteration, order, c->cfg.perms_order, dirs, mi);
+    for (int t = 0; t < 3; ++t) {
+        int X = dirs[t];
+        if (c->n[X - 1] < 3) continue;  // flux.m:39,53
+        int rc;
+        if (order > 0) {
+            if (c->cfg.fluid_active && (rc = imh_fluid_sweep_internal(c, X, force))) return rc;
+            if (c->cfg.magnet_active && (rc = imh_magnet_sweep_internal(c, X, mi[t], force)))
+                return rc;
+        } else {
+            if (c->cfg.magnet_active && (rc = imh_magnet_sweep_internal(c, X, mi[t], force)))
+                return rc;
+            if (c->cfg.fluid_active && (rc = imh_fluid_sweep_internal(c, X, force))) return rc;
+        }
+    }
+    return IMH_OK;
+}
+
+// One imogen.m:61-71 loop body; `conditional` = 0 runs it unconditionally (imh_iteration).
+static int iteration_body(imh_ctx* c, int conditional) {
+    int rc;
+    hipError_t e = launch_begin_iteration(c->sim, conditional ? c->par : c->par_uncond, 1, c->st);
+    if (e != hipSuccess) return hip_fail(e, "begin iteration");
+    for (int i = 1; i <= 2; ++i) {
+        if ((rc = imh_time_update_internal(c, i))) return rc;
+        if ((rc = imh_flux_internal(c, i == 1 ? 1 : -1, 0))) return rc;
+        // treadmill, gravity: inactive; source(): ener - dt*0 leaves every value unchanged
+    }
+    e = launch_end_iteration(c->sim, 1, c->st);
+    if (e != hipSuccess) return hip_fail(e, "end iteration");
+    return IMH_OK;
+}
+
+static int ensure_uncond(imh_ctx* c) {
+    if (c->par_uncond) return IMH_OK;
+    SimParams p = c->host_par;
+    p.unconditional = 1;
+    hipError_t e = hipMalloc(&c->par_uncond, sizeof(SimParams));
+    if (e == hipSuccess) e = hipMemcpy(c->par_uncond, &p, sizeof(p), hipMemcpyHostToDevice);
+    return e == hipSuccess ? IMH_OK : hip_fail(e, "params");
+}
+
+extern "C" {
+
+int imh_fluid_sweep(imh_ctx* c, int X) {
+    if (!c) return fail(IMH_EINVAL, "null context");
+    return imh_fluid_sweep_internal(c, X, 1);
+}
+
+int imh_magnet_sweep(imh_ctx* c, int X, const int I[2]) {
+    if (!c) return fail(IMH_EINVAL, "null context");
+    return imh_magnet_sweep_internal(c, X, I, 1);
+}
+
+int imh_time_update(imh_ctx* c, int step) {
+    if (!c) return fail(IMH_EINVAL, "null context");
+    if (!c->have_state) return fail(IMH_ESTATE, "state not uploaded");
+    // a direct call behaves as inside an unconditional loop body
+    int rc = ensure_uncond(c);
+    if (rc) return rc;
+    hipError_t e = launch_begin_iteration(c->sim, c->par_uncond, 1, c->st);
+    if (e != hipSuccess) return hip_fail(e, "begin");
+    return imh_time_update_internal(c, step);
+}
+
+int imh_flux(imh_ctx* c, int order) {
+    if (!c) return fail(IMH_EINVAL, "null context");
+    if (!c->have_state) return fail(IMH_ESTATE, "state not uploaded");
+    return imh_flux_internal(c, order, 1);
+}
+
+int imh_iteration(imh_ctx* c) {
+    if (!c) return fail(IMH_EINVAL, "null context");
+    if (!c->have_state) return fail(IMH_ESTATE, "state not uploaded");
+    int rc = ensure_uncond(c);
+    if (rc) return rc;
+    rc = iteration_body(c, 0);
+    if (rc) return rc;
+    c->host_iteration += 1;
+    return IMH_OK;
+}
+
+int imh_run(imh_ctx* c, int max_iterations, int* done) {
+    if (!c) return fail(IMH_EINVAL, "null context");
+    if (!c->have_state) return fail(IMH_ESTATE, "state not uploaded");
+    int rc = ensure_uncond(c);
+    if (rc) return rc;
+    SimDev sd;
+    hipError_t e = hipStreamSynchronize(c->st);
+    if (e == hipSuccess) e = hipMemcpy(&sd, c->sim, sizeof(sd), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(e, "time state");
+    c->host_iteration = sd.iteration;
+    const int start = sd.iteration;
+    int todo = std::min(max_iterations, c->cfg.iter_max - start + 1);
+    for (int it = 0; it < todo; ++it) {
+        bool inited = c->fluid_init || !c->cfg.fluid_active;
+        if (!inited || c->no_graphs) {
+            if ((rc = iteration_body(c, 1))) return rc;
+        } else {
+            int row = ((c->host_iteration - 1) % 6 + 6) % 6;
+            if (!c->graph[row]) {
+                hipGraph_t g;
+                e = hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal);
+                if (e != hipSuccess) return hip_fail(e, "begin capture");
+                rc = iteration_body(c, 1);
+                e = hipStreamEndCapture(c->st, &g);
+                if (rc) return rc;
+                if (e != hipSuccess) return hip_fail(e, "end capture");
+                e = hipGraphInstantiate(&c->graph[row], g, nullptr, nullptr, 0);
+                hipGraphDestroy(g);
+                if (e != hipSuccess) return hip_fail(e, "graph instantiate");
+            }
+            e = hipGraphLaunch(c->graph[row], c->st);
+            if (e != hipSuccess) return hip_fail(e, "graph launch");
+        }
+        c->host_iteration += 1;
+        if ((it & 63) == 63) {
+            // stop early once the run has ended on the device
+            e = hipMemcpyAsync(&sd, c->sim, sizeof(sd), hipMemcpyDeviceToHost, c->st);
+            if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+            if (e != hipSuccess) return hip_fail(e, "time state");
+            if (sd.error || sd.iteration > c->cfg.iter_max || sd.time >= c->cfg.time_max) break;
+        }
+    }
+    e = hipStreamSynchronize(c->st);
+    if (e == hipSuccess) e = hipMemcpy(&sd, c->sim, sizeof(sd), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(e, "time state");
+    c->host_iteration = sd.iteration;
+    if (done) *done = sd.iteration - start;
+    if (sd.error == IMH_ECFL)
+        return fail(IMH_ECFL, "CFL error: no signal speed above 1e-2 (TimeManager.m:82-85)");
+    return IMH_OK;
+}
+
+int imh_set_dtime(imh_ctx* c, double dt) {
+    if (!c) return fail(IMH_EINVAL, "null context");
+    hipError_t e = hipStreamSynchronize(c->st);
+    if (e == hipSuccess)
+        e = hipMemcpy((char*)c->sim + offsetof(SimDev, dt), &dt, sizeof(double), hipMemcpyHostToDevice);
+    return e == hipSuccess ? IMH_OK : hip_fail(e, "set dt");
+}
+
+int imh_set_iteration(imh_ctx* c, int iteration) {
+    if (!c) return fail(IMH_EINVAL, "null context");
+    hipError_t e = hipStreamSynchronize(c->st);
+    if (e == hipSuccess)
+        e = hipMemcpy((char*)c->sim + offsetof(SimDev, iteration), &iteration, sizeof(int),
+                      hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, "set iteration");
+    c->host_iteration = iteration;
+    return IMH_OK;
+}
+
+int imh_get_time(imh_ctx* c, double* time, double* dtime, int* iteration, int* grid_index) {
+    if (!c) return fail(IMH_EINVAL, "null context");
+    SimDev sd;
+    hipError_t e = hipStreamSynchronize(c->st);
+    if (e == hipSuccess) e = hipMemcpy(&sd, c->sim, sizeof(sd), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(e, "time state");
+    if (time) *time = sd.time;
+    if (dtime) *dtime = sd.dt;
+    if (iteration) *iteration = sd.iteration;
+    if (grid_index) *grid_index = sd.grid_index;
+    if (sd.error == IMH_ECFL)
+        return fail(IMH_ECFL, "CFL error: no signal speed above 1e-2 (TimeManager.m:82-85)");
+    return IMH_OK;
+}
+
+int imh_synchronize(imh_ctx* c) {
+    if (!c) return fail(IMH_EINVAL, "null context");
+    hipError_t e = hipStreamSynchronize(c->st);
+    return e == hipSuccess ? IMH_OK : hip_fail(e, "synchronize");
+}
+
+int imh_conservation_sums(imh_ctx* c, double out[8]) {
+    if (!c || !out) return fail(IMH_EINVAL, "null argument");
+    hipError_t e;
+    if (c->cfg.precision == IMH_FP64) {
+        const double* f[8];
+        for (int i = 0; i < 8; ++i) f[i] = (const double*)c->f[i];
+        e = launch_conservation<double>(f, 8, c->ncell, c->csum_part, 1024, c->csum_out, c->st);
+    } else {
+        const float* f[8];
+        for (int i = 0; i < 8; ++i) f[i] = (const float*)c->f[i];
+        e = launch_conservation<float>(f, 8, c->ncell, c->csum_part, 1024, c->csum_out, c->st);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    if (e == hipSuccess) e = hipMemcpy(out, c->csum_out, sizeof(double) * 8, hipMemcpyDeviceToHost);
+    return e == hipSuccess ? IMH_OK : hip_fail(e, "conservation sums");
+}
+
+int imh_memory_bytes(imh_ctx* c, size_t* bytes) {
+    if (!c || !bytes) return fail(IMH_EINVAL, "null argument");
+    size_t b = (size_t)c->ncell * esize(c) * (IMH_NFIELDS + (c->scratch ? 1 : 0));
+    b += c->edges_len * sizeof(double) + sizeof(c->eoff) + 2 * sizeof(SimDev);
+    *bytes = b;
+    return IMH_OK;
+}
+
+// Tuning hook (used by the autotuner): override the fluid tile of one sweep dim.
+int imh_set_fluid_tile(imh_ctx* c, int X, int bs, int kmax, int px) {
+    if (!c || X < 1 || X > 3) return fail(IMH_EINVAL, "bad argument");
+    TileOverride ov = {1, bs, kmax, px};
+    FluidTile t = choose_fluid_tile(c->n[X - 1], X - 1, (int)esize(c), &ov);
+    if (t.K > t.kmax || !fluid_variant_exists((int)esize(c), t.bs, t.kmax) || (t.bs % t.PX) != 0 ||
+        t.PX > 32 || (t.PX & (t.PX - 1)) != 0)
+        return fail(IMH_EINVAL, "tile (bs=%d kmax=%d px=%d) does not fit pencils of %d cells",
+                    bs, kmax, px, c->n[X - 1]);
+    c->ftile[X - 1] = t;
+    for (int i = 0; i < 6; ++i) {
+        if (c->graph[i]) hipGraphExecDestroy(c->graph[i]);
+        c->graph[i] = nullptr;
+    }
+    return IMH_OK;
+}
+
+int imh_get_fluid_tile(imh_ctx* c, int X, int* bs, int* kmax, int* px) {
+    if (!c || X < 1 || X > 3) return fail(IMH_EINVAL, "bad argument");
+    *bs = c->ftile[X - 1].bs;
+    *kmax = c->ftile[X - 1].kmax;
+    *px = c->ftile[X - 1].PX;
+    return IMH_OK;
+}
+
+int imh_set_graphs(imh_ctx* c, int enable) {
+    if (!c) return fail(IMH_EINVAL, "null context");
+    c->no_graphs = !enable;
+    return IMH_OK;
+}
+
+hipStream_t imh_stream(imh_ctx* c) { return c ? c->st : nullptr; }
+
+}  // extern "C"

@@ -1,0 +1,133 @@
+// imh_kernels.h -- kernel argument blocks and launcher declarations.
+#pragma once
+
+#include "imh_common.h"
+
+namespace imh {
+
+constexpr int MAX_FADE = 64;
+
+// BC modes [side][dim] packed 4 bits each (kernel arguments are never indexed at run
+// time: that would copy the whole argument block to scratch memory).
+__host__ __device__ inline int mode_at(int packed, int side, int dim) {
+    return (packed >> (4 * (3 * side + dim))) & 15;
+}
+inline int pack_modes(const int m[2][3]) {
+    int p = 0;
+    for (int s = 0; s < 2; ++s)
+        for (int d = 0; d < 3; ++d) p |= (m[s][d] & 15) << (4 * (3 * s + d));
+    return p;
+}
+__host__ __device__ inline int sel3i(int i, int a, int b, int c) {
+    return i == 0 ? a : (i == 1 ? b : c);
+}
+
+// One fluid sweep of one simulation (fluid_sweep.hip).
+template <typename T>
+struct FluidArgs {
+    T* q[5];                 // mass, mom(1..3), ener -- updated in place
+    const T* bc[3];          // cell-centred B (MagnetArray.cellMag)
+    double* edges;           // edge-tracker storage
+    const long long* eoff;   // [NOBJ][2][3] offsets into edges
+    const SimDev* sim;       // device time state (dt, active)
+    int n0, n1, n2;
+    int d;                   // sweep dimension 0..2
+    int ndims;
+    int modes;               // BC modes [side][dim], packed (mode_at)
+    int active_lin;          // Edges.ACTIVE(d+1) read as a linear index
+    int limiter;
+    MagUse use;              // !MagnetArray.isZero per component
+    double gamma, minmass, dx;
+    int init_edges;          // first assignment of the flux objects (first fluid sweep)
+    int PX, TL, K, xmajor;   // tile: PX pencils x TL threads, K cells per thread
+    long long npencil;
+    int bc_inf;
+    int xcd_remap;
+    int force;               // run even when the loop's running flag is off (direct calls)
+    const double* fade;      // fade_shift.m coefficients (device)
+};
+
+template <typename T>
+hipError_t launch_fluid_sweep(const FluidArgs<T>& a, int bs, int kmax, bool mhd,
+                              hipStream_t st);
+
+// One magnetFlux sub-step (component I advected along X) of one simulation
+// (magnet_sweep.hip).
+template <typename T>
+struct MagnetArgs {
+    const T* rho;            // mass
+    const T* mom;            // mom(X)
+    T* bI;                   // mag(I), updated in place
+    T* bX;                   // mag(X), updated by the CT step
+    T* flux;                 // scratch: mag(I).flux(X) after the limiter
+    double* edges;
+    const long long* eoff;
+    const SimDev* sim;
+    int n0, n1, n2;
+    int dX, dI;              // 0-based sweep and component dimensions
+    int ndims;
+    int modes;               // packed BC modes
+    int active_linX, active_linI;  // Edges.ACTIVE(D) (linear index) for D = X, I
+    int limiter;
+    double dxX, dxI;
+    int init_vg, init_store, init_wmag, init_flux;  // first assignments
+    int PX, TL;              // tile for the pencil kernel
+    long long npencil;
+    int bc_inf;
+    int force;
+    const double* fade;
+};
+
+template <typename T>
+hipError_t launch_magnet_advect(const MagnetArgs<T>& a, int bs, hipStream_t st);
+template <typename T>
+hipError_t launch_magnet_ct(const MagnetArgs<T>& a, hipStream_t st);
+
+// updateCellCentered for component k: cmag_k = 0.5*(B_k + shift_k(B_k, +1)).
+template <typename T>
+struct CellMagArgs {
+    const T* b;
+    T* cb;
+    double* edges;
+    const long long* eoff;
+    const SimDev* sim;
+    int n0, n1, n2;
+    int k;                   // component (0-based) = shift dim
+    int ndims;
+    int modes;
+    int active_lin;
+    int bc_inf;
+    int force;               // run even if the simulation is not active (constructor)
+    const double* fade;
+};
+
+template <typename T>
+hipError_t launch_cell_centered(const CellMagArgs<T>& a, hipStream_t st);
+
+// CFL reduction (TimeManager.update) over one simulation.
+template <typename T>
+struct CflArgs {
+    const T* q[5];
+    const T* bc[3];
+    SimDev* sim;
+    long long ncell;
+    MagUse use;
+    double gamma;
+};
+
+template <typename T>
+hipError_t launch_cfl(const CflArgs<T>& a, hipStream_t st);
+
+// Device-side bookkeeping kernels (time_kernels in cfl.hip).
+hipError_t launch_begin_iteration(SimDev* sim, const SimParams* p, int nsims, hipStream_t st);
+hipError_t launch_cfl_reset(SimDev* sim, int nsims, hipStream_t st);
+hipError_t launch_time_finalize(SimDev* sim, const SimParams* p, int nsims, int step,
+                                hipStream_t st);
+hipError_t launch_end_iteration(SimDev* sim, int nsims, hipStream_t st);
+
+// Conservation totals (Neumaier-compensated FP64, fixed-order tree).
+template <typename T>
+hipError_t launch_conservation(const T* const* fields, int nfields, long long ncell,
+                               double* partials, int nblocks, double* out, hipStream_t st);
+
+}  // namespace imh

@@ -1,0 +1,623 @@
+// magnet_sweep.hip -- constrained-transport magnetic sweep (magnet/magnetFlux.m,
+// magnet/relaxingMagnet.m, MagnetArray.updateCellCentered).
+//
+// One magnetFlux(X, [I1 I2]) call is, per component I:
+//   magnet_advect_kernel  (pencils along X): grid-aligned velocity (shift of mass and
+//                         mom(X) along I, smoothing along X), upwind predictor, TVD
+//                         corrector with the limiter, B_I update; the limited flux is kept
+//                         in a scratch array for the CT step;
+//   magnet_ct_kernel      (pencils along X): G = F - shift_I(F,+1), B_X -= dt/dx_I *
+//                         shift_X(G,-1);
+// followed by cell_centered_kernel for the last I and for X.
+//
+// Each pencil lives in LDS with two ghost slots per side; every shift of the .m text is
+// one exchange of the whole pencil (write, barrier, ghost cells from the BC rule with the
+// object's edge tracker, barrier, read).  The magnet sweeps are a smaller share of the
+// MHD step than the fluid sweep, so this kernel favours generality (all BC modes, +-2
+// shifts, fade windows) over the register-chunk layout of fluid_sweep.hip.
+#include "imh_common.h"
+#include "imh_kernels.h"
+
+namespace imh {
+
+namespace {
+
+constexpr int MAG_BS = 256;
+constexpr int MAG_PXMAX = 16;
+constexpr int MAG_NMAX = 4096;   // LDS holds 2 pencils of up to this many cells in total
+
+// Index helpers for a pencil of length N at LDS row `row`: slot 2+i holds cell i, slots
+// 0..1 hold ghost cells -2..-1 and N+2..N+3 ghost cells N..N+1.
+template <typename T>
+struct Pencil {
+    T* row;
+    int N;
+    __device__ T& at(int i) { return row[2 + i]; }
+};
+
+// Ends (a[0..2], a[N-3..N-1], clamped) of an LDS pencil, in FP64.
+template <typename T>
+__device__ inline Ends lds_ends(const T* row, int N) {
+    auto lo = [&](int k) __attribute__((always_inline)) { return double(row[2 + (k < N - 1 ? k : N - 1)]); };
+    auto hi = [&](int k) __attribute__((always_inline)) { int j = N - 3 + k; return double(row[2 + (j > 0 ? j : 0)]); };
+    Ends e = {lo(0), lo(1), lo(2), hi(0), hi(1), hi(2)};
+    return e;
+}
+
+// shift(D, m) (m = +-1, +-2) of an LDS pencil at cell i, with ghost values g (the m
+// values ghost_values() returned for this call) and the fade window when mode is fade.
+template <typename T>
+__device__ inline T lds_shift(const T* row, int N, int i, int m, const double* g, int mode,
+                              const double* fade, int f) {
+    int up = m > 0;
+    int am = up ? m : -m;
+    int idx = i + m;
+    T r;
+    if (idx >= 0 && idx <= N - 1) r = row[2 + idx];
+    else r = T(up ? (idx - N == 0 ? g[0] : g[1]) : (i == 0 ? g[0] : g[1]));
+    if (mode == IMH_BC_FADE) {
+        T v1 = up ? row[2 + (am < N - 1 ? am : N - 1)] : row[2];
+        int jn = N - 1 - am;
+        T vN = up ? row[2 + N - 1] : row[2 + (jn > 0 ? jn : 0)];
+        r = fade_blend(r, i, N, v1, vN, fade, f);
+    }
+    return r;
+}
+
+// Transverse shift along dimension dI of global array `arr` at the cell whose
+// coordinate along dI is jI: shift(I,-1) (upper=0) or shift(I,+1) (upper=1), one cell.
+// tracker is this cell's stored edge (used when the cell lies on that face).
+template <typename T>
+__device__ inline T transverse_shift(const T* arr, long long cell, int jI, int NI,
+                                     long long sI, bool enabled, const GhostSpec& s,
+                                     double* tracker, const double* fade, int f) {
+    if (!enabled) return arr[cell];
+    const long long row0 = cell - (long long)jI * sI;
+    T r;
+    bool ghost = s.upper ? (jI >= NI - 1) : (jI <= 0);
+    if (!ghost) {
+        r = arr[cell + (s.upper ? sI : -sI)];
+    } else {
+        auto lo = [&](int k) __attribute__((always_inline)) { return double(arr[row0 + (long long)(k < NI - 1 ? k : NI - 1) * sI]); };
+        auto hi = [&](int k) __attribute__((always_inline)) { int j = NI - 3 + k; return double(arr[row0 + (long long)(j > 0 ? j : 0) * sI]); };
+        Ends e = {lo(0), lo(1), lo(2), hi(0), hi(1), hi(2)};
+        double g0, g1;
+        ghost_values(s, e, NI, 1, tracker, g0, g1);
+        r = T(g0);
+    }
+    if (s.mode == IMH_BC_FADE) {
+        int i1 = 1 < NI - 1 ? 1 : NI - 1;
+        int jn = NI - 2 > 0 ? NI - 2 : 0;
+        T v1 = s.upper ? arr[row0 + i1 * sI] : arr[row0];
+        T vN = s.upper ? arr[row0 + (long long)(NI - 1) * sI] : arr[row0 + jn * sI];
+        r = fade_blend(r, jI, NI, v1, vN, fade, f);
+    }
+    return r;
+}
+
+__device__ inline long long coord_of(long long cell, const int n[3], int d) {
+    const long long c0 = cell % n[0];
+    const long long c1 = (cell / n[0]) % n[1];
+    const long long c2 = cell / ((long long)n[0] * n[1]);
+    return d == 0 ? c0 : (d == 1 ? c1 : c2);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------
+// Advection sub-step for component I along X (one pencil per PX group of threads).
+// ------------------------------------------------------------------------------------
+template <typename T, int KMAX>
+__global__ void __launch_bounds__(MAG_BS) magnet_advect_kernel(const MagnetArgs<T> a) {
+    if (!a.force && !a.sim->active) return;
+    extern __shared__ unsigned char smem_raw[];
+    const int dX = a.dX, dI = a.dI;
+    const int nn[3] = {a.n0, a.n1, a.n2};
+    const int N = sel3i(dX, a.n0, a.n1, a.n2);
+    const int rowlen = N + 4;
+    T* lds = reinterpret_cast<T*>(smem_raw);            // [2][PX][rowlen]
+    __shared__ double s_g[MAG_PXMAX][8];                 // ghost values per call
+
+    const int tid = threadIdx.x;
+    const int px = tid / a.TL;
+    const int ty = tid % a.TL;
+    const long long pen = (long long)blockIdx.x * a.PX + px;
+    const bool pen_ok = pen < a.npencil && px < a.PX;
+    const int K = (N + a.TL - 1) / a.TL;
+    const int i0 = ty * K;
+    int kc = N - i0;
+    kc = kc < 0 ? 0 : (kc > K ? K : kc);
+    if (!pen_ok) kc = 0;
+    const bool ghost_thread = pen_ok && ty == 0;
+    const int f = a.bc_inf;
+
+    long long base = 0, stride = 1;
+    if (pen_ok) pencil_base(nn, dX, pen, base, stride);
+    const double dt = a.sim->dt;
+    const double ff = 0.5 * dt / a.dxX;       // relaxingMagnet.m:47
+    const double ff2 = 2.0 * ff;              // relaxingMagnet.m:64
+
+    const int mlo = mode_at(a.modes, 0, dX), mhi = mode_at(a.modes, 1, dX);
+    const int oVG = OBJ_VG + 3 * dI + dX, oSFR = OBJ_MSFR + 3 * dI + dX;
+    const int oWM = OBJ_WMAG + 3 * dI + dX, oFL = OBJ_MFLUX + 3 * dI + dX;
+    const int Xc = dX + 1;                    // field index of mom(X)
+    const bool trk_lo = (mlo == IMH_BC_TRANS || mlo == IMH_BC_WALL);
+    const bool trk_hi = (mhi == IMH_BC_TRANS || mhi == IMH_BC_WALL);
+    const bool I_enabled = dI < a.ndims;
+    const long long sI = dI == 0 ? 1 : (dI == 1 ? (long long)a.n0 : (long long)a.n0 * a.n1);
+    const int NI = sel3i(dI, a.n0, a.n1, a.n2);
+
+    T* row0 = lds + (size_t)(0 * a.PX + (pen_ok ? px : 0)) * rowlen;
+    T* row1 = lds + (size_t)(1 * a.PX + (pen_ok ? px : 0)) * rowlen;
+
+    // edge trackers of this pencil along X (ghost thread only)
+    auto eidx = [&](int obj, int side) __attribute__((always_inline)) -> long long {
+        return a.eoff[(obj * 2 + side) * 3 + dX] + pen;
+    };
+    double evg_lo = 0, evg_hi = 0, esfr_lo = 0, esfr_hi = 0, ewm_lo = 0, ewm_hi = 0, efl_lo = 0;
+    if (ghost_thread) {
+        if (trk_lo) {
+            if (!a.init_vg) evg_lo = a.edges[eidx(oVG, 0)];
+            if (!a.init_store) esfr_lo = a.edges[eidx(oSFR, 0)];
+            if (!a.init_wmag) ewm_lo = a.edges[eidx(oWM, 0)];
+            if (!a.init_flux) efl_lo = a.edges[eidx(oFL, 0)];
+        }
+        if (trk_hi) {
+            if (!a.init_vg) evg_hi = a.edges[eidx(oVG, 1)];
+            if (!a.init_store) esfr_hi = a.edges[eidx(oSFR, 1)];
+            if (!a.init_wmag) ewm_hi = a.edges[eidx(oWM, 1)];
+        }
+    }
+    auto spec = [&](int upper, int comp_is_dir, int zero_ghost) __attribute__((always_inline)) {
+        GhostSpec s;
+        s.mode = upper ? mhi : mlo;
+        s.upper = upper;
+        s.flip_sign = comp_is_dir;
+        s.zero_ghost = zero_ghost;
+        s.active_linear = a.active_linX;
+        s.bc_infinity = f;
+        return s;
+    };
+    // record the edges of a freshly assigned object on all faces (InitializedArray.m:43-47)
+    auto init_faces = [&](int obj, const T (&vals)[KMAX]) __attribute__((always_inline)) {
+        for (int dd = 0; dd < a.ndims; ++dd) {
+            if (dd == dX) continue;
+            for (int s = 0; s < 2; ++s) {
+                int mode = mode_at(a.modes, s, dd);
+                if (mode != IMH_BC_TRANS && mode != IMH_BC_WALL) continue;
+                long long off = a.eoff[(obj * 2 + s) * 3 + dd];
+#pragma unroll
+                for (int k = 0; k < KMAX; ++k) {
+                    if (k >= kc) continue;
+                    long long cell = base + (long long)(i0 + k) * stride;
+                    const int c0 = int(coord_of(cell, nn, 0));
+                    const int c1 = int(coord_of(cell, nn, 1));
+                    const int c2 = int(coord_of(cell, nn, 2));
+                    if (sel3i(dd, c0, c1, c2) != (s ? sel3i(dd, a.n0, a.n1, a.n2) - 1 : 0)) continue;
+                    a.edges[off + face_index(nn, dd, c0, c1, c2)] = double(vals[k]);
+                }
+            }
+        }
+    };
+    auto put = [&](T* row, const T (&v)[KMAX]) __attribute__((always_inline)) {
+#pragma unroll
+        for (int k = 0; k < KMAX; ++k)
+            if (k < kc) row[2 + i0 + k] = v[k];
+    };
+
+    // ---- grid-aligned velocity: (shift_I(p_X,-1) + p_X) ./ (shift_I(rho,-1) + rho) ----
+    T vg[KMAX], bI[KMAX];
+    {
+        const int mIlo = mode_at(a.modes, 0, dI);
+        GhostSpec sm = {mIlo, 0, 0, 0, a.active_linI, f};  // FluidArray: no flips
+        GhostSpec sr = sm;
+        sm.flip_sign = (Xc == dI + 1);  // mom(X).component == I ?  (X != I, so never)
+#pragma unroll
+        for (int k = 0; k < KMAX; ++k) {
+            if (k < kc) {
+                long long cell = base + (long long)(i0 + k) * stride;
+                int jI = int(coord_of(cell, nn, dI));
+                double* tm = nullptr;
+                double* tr = nullptr;
+                double em = 0, er = 0;
+                long long fI = 0;
+                bool trkI = (mIlo == IMH_BC_TRANS || mIlo == IMH_BC_WALL) &&
+                            jI == 0 && I_enabled;
+                if (trkI) {
+                    fI = face_index(nn, dI, int(coord_of(cell, nn, 0)), int(coord_of(cell, nn, 1)),
+                                    int(coord_of(cell, nn, 2)));
+                    em = a.edges[a.eoff[((OBJ_PRIM + Xc) * 2 + 0) * 3 + dI] + fI];
+                    er = a.edges[a.eoff[((OBJ_PRIM + 0) * 2 + 0) * 3 + dI] + fI];
+                    tm = &em;
+                    tr = &er;
+                }
+                T sm_v = transverse_shift(a.mom, cell, jI, NI, sI, I_enabled, sm, tm, a.fade, f);
+                T sr_v = transverse_shift(a.rho, cell, jI, NI, sI, I_enabled, sr, tr, a.fade, f);
+                if (trkI) {
+                    a.edges[a.eoff[((OBJ_PRIM + Xc) * 2 + 0) * 3 + dI] + fI] = em;
+                    a.edges[a.eoff[((OBJ_PRIM + 0) * 2 + 0) * 3 + dI] + fI] = er;
+                }
+                vg[k] = (sm_v + a.mom[cell]) / (sr_v + a.rho[cell]);
+                bI[k] = a.bI[cell];
+            } else {
+                vg[k] = T(0);
+                bI[k] = T(0);
+            }
+        }
+    }
+    if (a.init_vg) init_faces(oVG, vg);
+
+    // ---- smoothing: vg = 0.25*(vg.shift(X,-1) + vg.shift(X,1) + 2*vg) -----------------
+    put(row0, vg);
+    __syncthreads();
+    if (ghost_thread) {
+        double g0, g1;
+        const Ends e = lds_ends(row0, N);
+        if (a.init_vg) { evg_lo = e.a0; evg_hi = e.b2; }
+        ghost_values(spec(0, 0, 0), e, N, 1, &evg_lo, g0, g1);
+        s_g[px][0] = g0;
+        ghost_values(spec(1, 0, 0), e, N, 1, &evg_hi, g0, g1);
+        s_g[px][1] = g0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+        if (k < kc) {
+            int i = i0 + k;
+            T s1 = lds_shift(row0, N, i, -1, &s_g[px][0], mlo, a.fade, f);
+            T s2 = lds_shift(row0, N, i, 1, &s_g[px][1], mhi, a.fade, f);
+            vg[k] = T(0.25) * (s1 + s2 + T(2) * vg[k]);
+        }
+    }
+
+    // ---- velocityFlow = (vg + vg.shift(X,1)) < 0 -------------------------------------
+    T vf[KMAX];
+    put(row1, vg);
+    __syncthreads();
+    if (ghost_thread) {
+        double g0, g1;
+        const Ends e = lds_ends(row1, N);
+        ghost_values(spec(1, 0, 0), e, N, 1, &evg_hi, g0, g1);
+        s_g[px][2] = g0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+        if (k < kc) {
+            T s = lds_shift(row1, N, i0 + k, 1, &s_g[px][2], mhi, a.fade, f);
+            vf[k] = (vg[k] + s < T(0)) ? T(1) : T(0);
+        } else {
+            vf[k] = T(0);
+        }
+    }
+
+    // ---- predictor: store.fluxR = B.*vg; upwind select; B_half ----------------------
+    T w[KMAX];
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) w[k] = bI[k] * vg[k];
+    if (a.init_store) init_faces(oSFR, w);
+    put(row0, w);
+    __syncthreads();
+    if (ghost_thread) {
+        double g0, g1;
+        const Ends e = lds_ends(row0, N);
+        if (a.init_store) { esfr_lo = e.a0; esfr_hi = e.b2; }
+        ghost_values(spec(1, 0, 0), e, N, 1, &esfr_hi, g0, g1);  // FLUXR keeps the upper side
+        s_g[px][3] = g0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+        if (k < kc) {
+            T s = lds_shift(row0, N, i0 + k, 1, &s_g[px][3], mhi, a.fade, f);
+            w[k] = w[k] * (T(1) - vf[k]) + s * vf[k];
+        }
+    }
+    put(row1, w);
+    __syncthreads();
+    if (ghost_thread) {
+        double g0, g1;
+        const Ends e = lds_ends(row1, N);
+        ghost_values(spec(0, 0, 1), e, N, 1, &esfr_lo, g0, g1);  // FLUXR: zero-mode lower ghost 0
+        s_g[px][4] = g0;
+    }
+    __syncthreads();
+    T st[KMAX];
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+        if (k < kc) {
+            T s = lds_shift(row1, N, i0 + k, -1, &s_g[px][4], mlo, a.fade, f);
+            st[k] = bI[k] - T(ff) * (w[k] - s);
+        } else {
+            st[k] = T(0);
+        }
+    }
+
+    // ---- corrector: wMag = store.*vg; flux; dFluxR, dFluxL; limiter ------------------
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) w[k] = st[k] * vg[k];
+    if (a.init_wmag) init_faces(oWM, w);
+    put(row0, w);
+    __syncthreads();
+    if (ghost_thread) {
+        double g0, g1;
+        const Ends e = lds_ends(row0, N);
+        if (a.init_wmag) { ewm_lo = e.a0; ewm_hi = e.b2; }
+        // wMag is an InitializedArray (not a FluxArray): zero mode keeps constant ghosts
+        ghost_values(spec(1, 0, 0), e, N, 1, &ewm_hi, g0, g1);      // flux = ... shift(X,1)
+        s_g[px][0] = g0;
+        ghost_values(spec(1, 0, 0), e, N, 1, &ewm_hi, g0, g1);      // dFluxR: shift(X,1)
+        s_g[px][1] = g0;
+        ghost_values(spec(1, 0, 0), e, N, 2, &ewm_hi, g0, g1);      // dFluxR: shift(X,2)
+        s_g[px][2] = g0;
+        s_g[px][3] = g1;
+        ghost_values(spec(1, 0, 0), e, N, 1, &ewm_hi, g0, g1);      // dFluxL: shift(X,1)
+        s_g[px][4] = g0;
+        ghost_values(spec(0, 0, 0), e, N, 1, &ewm_lo, g0, g1);      // dFluxL: shift(X,-1)
+        s_g[px][5] = g0;
+    }
+    __syncthreads();
+    T F[KMAX];
+    const int lim = a.limiter;
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+        if (k < kc) {
+            int i = i0 + k;
+            T a1 = lds_shift(row0, N, i, 1, &s_g[px][0], mhi, a.fade, f);
+            F[k] = w[k] * (T(1) - vf[k]) + a1 * vf[k];
+        } else {
+            F[k] = T(0);
+        }
+    }
+    // flux(X) is first assigned here (pre-limiter): its edges come from this array.
+    if (a.init_flux) {
+        init_faces(oFL, F);
+        if (ghost_thread) efl_lo = double(F[0]);
+    }
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+        if (k < kc) {
+            int i = i0 + k;
+            T b1 = lds_shift(row0, N, i, 1, &s_g[px][1], mhi, a.fade, f);
+            T b2 = lds_shift(row0, N, i, 2, &s_g[px][2], mhi, a.fade, f);
+            T dR = b1 - (vf[k] * b2 + (T(1) - vf[k]) * w[k]);
+            T c1 = lds_shift(row0, N, i, 1, &s_g[px][4], mhi, a.fade, f);
+            T c2 = lds_shift(row0, N, i, -1, &s_g[px][5], mlo, a.fade, f);
+            T dL = w[k] - (vf[k] * c1 + (T(1) - vf[k]) * c2);
+            F[k] = F[k] + lim_delta(lim, dL, dR);
+        }
+    }
+
+    // ---- B_I = B_I - 2*fluxFactor .* (flux - flux.shift(X,-1)) -------------------------
+    put(row1, F);
+    __syncthreads();
+    if (ghost_thread) {
+        double g0, g1;
+        const Ends e = lds_ends(row1, N);
+        ghost_values(spec(0, 0, 1), e, N, 1, &efl_lo, g0, g1);  // 'flux' type: zero ghost 0
+        s_g[px][6] = g0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+        if (k < kc) {
+            long long cell = base + (long long)(i0 + k) * stride;
+            T s = lds_shift(row1, N, i0 + k, -1, &s_g[px][6], mlo, a.fade, f);
+            a.bI[cell] = bI[k] - T(ff2) * (F[k] - s);
+            a.flux[cell] = F[k];
+        }
+    }
+
+    if (ghost_thread) {
+        if (trk_lo) {
+            a.edges[eidx(oVG, 0)] = evg_lo;
+            a.edges[eidx(oSFR, 0)] = esfr_lo;
+            a.edges[eidx(oWM, 0)] = ewm_lo;
+            a.edges[eidx(oFL, 0)] = efl_lo;
+        }
+        if (trk_hi) {
+            a.edges[eidx(oVG, 1)] = evg_hi;
+            a.edges[eidx(oSFR, 1)] = esfr_hi;
+            a.edges[eidx(oWM, 1)] = ewm_hi;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// Constrained transport: flux = flux - flux.shift(I,1); flux = flux.shift(X,-1);
+// mag(X) = mag(X) - dt/DGRID{I} .* flux   (relaxingMagnet.m:88-92).
+// ------------------------------------------------------------------------------------
+template <typename T, int KMAX>
+__global__ void __launch_bounds__(MAG_BS) magnet_ct_kernel(const MagnetArgs<T> a) {
+    if (!a.force && !a.sim->active) return;
+    extern __shared__ unsigned char smem_raw[];
+    const int dX = a.dX, dI = a.dI;
+    const int nn[3] = {a.n0, a.n1, a.n2};
+    const int N = sel3i(dX, a.n0, a.n1, a.n2);
+    const int rowlen = N + 4;
+    T* lds = reinterpret_cast<T*>(smem_raw);
+    __shared__ double s_g[MAG_PXMAX][2];
+
+    const int tid = threadIdx.x;
+    const int px = tid / a.TL;
+    const int ty = tid % a.TL;
+    const long long pen = (long long)blockIdx.x * a.PX + px;
+    const bool pen_ok = pen < a.npencil && px < a.PX;
+    const int K = (N + a.TL - 1) / a.TL;
+    const int i0 = ty * K;
+    int kc = N - i0;
+    kc = kc < 0 ? 0 : (kc > K ? K : kc);
+    if (!pen_ok) kc = 0;
+    const bool ghost_thread = pen_ok && ty == 0;
+    const int f = a.bc_inf;
+    long long base = 0, stride = 1;
+    if (pen_ok) pencil_base(nn, dX, pen, base, stride);
+    const double ffI = a.sim->dt / a.dxI;
+    const int oFL = OBJ_MFLUX + 3 * dI + dX;
+    const int mlo = mode_at(a.modes, 0, dX);
+    const bool I_enabled = dI < a.ndims;
+    const long long sI = dI == 0 ? 1 : (dI == 1 ? (long long)a.n0 : (long long)a.n0 * a.n1);
+    const int NI = sel3i(dI, a.n0, a.n1, a.n2);
+    T* row = lds + (size_t)(pen_ok ? px : 0) * rowlen;
+
+    // G = F - shift_I(F, +1): flux object, component I == DIRECT (flips for mirror/flip),
+    // type 'flux' (zero mode: ghost 0)
+    const int mIhi = mode_at(a.modes, 1, dI);
+    GhostSpec sI_spec = {mIhi, 1, 1, 1, a.active_linI, f};
+    const bool trkI = (mIhi == IMH_BC_TRANS || mIhi == IMH_BC_WALL) && I_enabled;
+    T G[KMAX];
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+        if (k < kc) {
+            long long cell = base + (long long)(i0 + k) * stride;
+            int jI = int(coord_of(cell, nn, dI));
+            double et = 0;
+            double* tp = nullptr;
+            long long fI = 0;
+            bool onface = trkI && jI == NI - 1;
+            if (onface) {
+                fI = face_index(nn, dI, int(coord_of(cell, nn, 0)), int(coord_of(cell, nn, 1)),
+                                int(coord_of(cell, nn, 2)));
+                et = a.edges[a.eoff[(oFL * 2 + 1) * 3 + dI] + fI];
+                tp = &et;
+            }
+            T s = transverse_shift((const T*)a.flux, cell, jI, NI, sI, I_enabled, sI_spec, tp,
+                                   a.fade, f);
+            if (onface) a.edges[a.eoff[(oFL * 2 + 1) * 3 + dI] + fI] = et;
+            G[k] = a.flux[cell] - s;
+        } else {
+            G[k] = T(0);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k)
+        if (k < kc) row[2 + i0 + k] = G[k];
+    __syncthreads();
+    if (ghost_thread) {
+        double g0, g1;
+        const Ends e = lds_ends(row, N);
+        double tr = 0;
+        bool trk = (mlo == IMH_BC_TRANS || mlo == IMH_BC_WALL);
+        long long ei = a.eoff[(oFL * 2 + 0) * 3 + dX] + pen;
+        if (trk) tr = a.edges[ei];
+        GhostSpec s = {mlo, 0, 0, 1, a.active_linX, f};
+        ghost_values(s, e, N, 1, &tr, g0, g1);
+        if (trk) a.edges[ei] = tr;
+        s_g[px][0] = g0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+        if (k < kc) {
+            long long cell = base + (long long)(i0 + k) * stride;
+            T h = lds_shift(row, N, i0 + k, -1, &s_g[px][0], mlo, a.fade, f);
+            a.bX[cell] = a.bX[cell] - T(ffI) * h;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// cellMag_k = 0.5*(B_k + B_k.shift(k, 1))  (MagnetArray.m:36-38), element-wise with the
+// mag(k) edge tracker on the upper k face.
+// ------------------------------------------------------------------------------------
+template <typename T>
+__global__ void cell_centered_kernel(const CellMagArgs<T> a, long long ncell) {
+    if (!a.force && !a.sim->active) return;
+    long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= ncell) return;
+    const int d = a.k;
+    const int nn[3] = {a.n0, a.n1, a.n2};
+    const bool enabled = d < a.ndims;
+    const long long sd = d == 0 ? 1 : (d == 1 ? (long long)a.n0 : (long long)a.n0 * a.n1);
+    const int Nd = sel3i(d, a.n0, a.n1, a.n2);
+    int jd = int(coord_of(cell, nn, d));
+    GhostSpec s = {mode_at(a.modes, 1, d), 1, 1, 0, a.active_lin, a.bc_inf};  // comp k == DIRECT
+    bool trk = enabled && (s.mode == IMH_BC_TRANS || s.mode == IMH_BC_WALL) && jd == Nd - 1;
+    double et = 0;
+    long long fi = 0;
+    if (trk) {
+        fi = face_index(nn, d, int(coord_of(cell, nn, 0)), int(coord_of(cell, nn, 1)),
+                        int(coord_of(cell, nn, 2)));
+        et = a.edges[a.eoff[((OBJ_MAG + d) * 2 + 1) * 3 + d] + fi];
+    }
+    T sh = transverse_shift(a.b, cell, jd, Nd, sd, enabled, s, trk ? &et : nullptr, a.fade,
+                            a.bc_inf);
+    if (trk) a.edges[a.eoff[((OBJ_MAG + d) * 2 + 1) * 3 + d] + fi] = et;
+    a.cb[cell] = T(0.5) * (a.b[cell] + sh);
+}
+
+// ------------------------------------------------------------------------------------
+// Launchers
+// ------------------------------------------------------------------------------------
+template <typename T, int KMAX>
+static hipError_t launch_adv_k(const MagnetArgs<T>& a, size_t smem, long long nb,
+                               hipStream_t st) {
+    static bool attr_set = false;  // LDS above 64 KiB needs the opt-in attribute
+    if (smem > 64 * 1024 && !attr_set) {
+        hipError_t e = hipFuncSetAttribute((const void*)magnet_advect_kernel<T, KMAX>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           150 * 1024);
+        if (e != hipSuccess) return e;
+        attr_set = true;
+    }
+    hipLaunchKernelGGL((magnet_advect_kernel<T, KMAX>), dim3(nb), dim3(MAG_BS), smem, st, a);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_magnet_advect(const MagnetArgs<T>& a, int bs, hipStream_t st) {
+    (void)bs;
+    const int N = sel3i(a.dX, a.n0, a.n1, a.n2);
+    if (a.PX < 1 || a.PX > MAG_PXMAX || a.PX * a.TL != MAG_BS) return hipErrorInvalidValue;
+    const int K = (N + a.TL - 1) / a.TL;
+    size_t smem = sizeof(T) * 2 * a.PX * (size_t)(N + 4);
+    if (smem > 150 * 1024) return hipErrorInvalidValue;
+    long long nb = (a.npencil + a.PX - 1) / a.PX;
+    if (nb == 0) return hipSuccess;
+    if (K <= 1) return launch_adv_k<T, 1>(a, smem, nb, st);
+    if (K <= 2) return launch_adv_k<T, 2>(a, smem, nb, st);
+    if (K <= 4) return launch_adv_k<T, 4>(a, smem, nb, st);
+    if (K <= 8) return launch_adv_k<T, 8>(a, smem, nb, st);
+    if (K <= 16) return launch_adv_k<T, 16>(a, smem, nb, st);
+    return hipErrorInvalidValue;
+}
+
+template <typename T, int KMAX>
+static hipError_t launch_ct_k(const MagnetArgs<T>& a, size_t smem, long long nb,
+                              hipStream_t st) {
+    hipLaunchKernelGGL((magnet_ct_kernel<T, KMAX>), dim3(nb), dim3(MAG_BS), smem, st, a);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_magnet_ct(const MagnetArgs<T>& a, hipStream_t st) {
+    const int N = sel3i(a.dX, a.n0, a.n1, a.n2);
+    if (a.PX < 1 || a.PX > MAG_PXMAX || a.PX * a.TL != MAG_BS) return hipErrorInvalidValue;
+    const int K = (N + a.TL - 1) / a.TL;
+    size_t smem = sizeof(T) * a.PX * (size_t)(N + 4);
+    long long nb = (a.npencil + a.PX - 1) / a.PX;
+    if (nb == 0) return hipSuccess;
+    if (K <= 1) return launch_ct_k<T, 1>(a, smem, nb, st);
+    if (K <= 2) return launch_ct_k<T, 2>(a, smem, nb, st);
+    if (K <= 4) return launch_ct_k<T, 4>(a, smem, nb, st);
+    if (K <= 8) return launch_ct_k<T, 8>(a, smem, nb, st);
+    if (K <= 16) return launch_ct_k<T, 16>(a, smem, nb, st);
+    return hipErrorInvalidValue;
+}
+
+template <typename T>
+hipError_t launch_cell_centered(const CellMagArgs<T>& a, hipStream_t st) {
+    long long ncell = (long long)a.n0 * a.n1 * a.n2;
+    long long nb = (ncell + 255) / 256;
+    hipLaunchKernelGGL((cell_centered_kernel<T>), dim3(nb), dim3(256), 0, st, a, ncell);
+    return hipGetLastError();
+}
+
+template hipError_t launch_magnet_advect<double>(const MagnetArgs<double>&, int, hipStream_t);
+template hipError_t launch_magnet_advect<float>(const MagnetArgs<float>&, int, hipStream_t);
+template hipError_t launch_magnet_ct<double>(const MagnetArgs<double>&, hipStream_t);
+template hipError_t launch_magnet_ct<float>(const MagnetArgs<float>&, hipStream_t);
+template hipError_t launch_cell_centered<double>(const CellMagArgs<double>&, hipStream_t);
+template hipError_t launch_cell_centered<float>(const CellMagArgs<float>&, hipStream_t);
+
+}  // namespace imh

@@ -1,0 +1,174 @@
+/*
+ * imogen_hip.h -- C ABI of the MI355X (gfx950) relaxing-TVD fluid/magnet update.
+ *
+ * This is the drop-in boundary: the MATLAB functions of the reference keep their
+ * signatures and their bodies become calls of these entry points through the MEX
+ * gateway (imogen_amd/mex/imogen_mex.cpp), exactly as the reference's own MEX modules
+ * are called (gravity/utils/cudaCommon.cu:20-56 marshal GPUmat arrays; outputs are
+ * allocated by the library or updated in place).  Plain C types only: pointers, sizes,
+ * ints and doubles.
+ *
+ * Arrays crossing the boundary are column-major [nx ny nz] (x fastest), FP64, the
+ * layout of the reference's FluidArray/MagnetArray data (imogen.m:34-42).  The library
+ * owns all device memory and the boundary-condition edge trackers; host buffers are
+ * borrowed for the duration of a call only.  A context is used by one thread at a time
+ * and owns one HIP stream; separate contexts are independent.
+ *
+ * Every call returns IMH_OK (0) or a negative IMH_E* status; imh_last_error() returns
+ * the message of the last failure on the calling thread.
+ */
+#ifndef IMOGEN_HIP_H
+#define IMOGEN_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------------------- */
+#define IMH_OK 0
+#define IMH_EINVAL -1     /* bad argument or configuration */
+#define IMH_EHIP -2       /* a HIP runtime call failed */
+#define IMH_ENOMEM -3     /* device memory exhausted or HBM budget exceeded */
+#define IMH_ECFL -4       /* no signal speed above 1e-2 (TimeManager.m:82-85) */
+#define IMH_ESTATE -5     /* call out of order (e.g. a sweep before the state upload) */
+#define IMH_ENOTSUP -6    /* configuration the reference itself rejects or errors on */
+
+/* ---- boundary-condition modes (enums/ENUM.m:23-32) --------------------------------- */
+#define IMH_BC_CIRC 0
+#define IMH_BC_CONST 1
+#define IMH_BC_TRANS 2
+#define IMH_BC_FADE 3
+#define IMH_BC_MIRROR 4
+#define IMH_BC_FLIP 5
+#define IMH_BC_WALL 6
+#define IMH_BC_ZERO 7
+
+/* ---- flux limiters (enums/FluxLimiterEnum.m) ---------------------------------------- */
+#define IMH_LIM_VANLEER 0
+#define IMH_LIM_SUPERBEE 1
+#define IMH_LIM_MINMOD 2
+
+/* ---- precision --------------------------------------------------------------------- */
+#define IMH_FP64 0        /* plain path: every operation in FP64 */
+#define IMH_FP32_BULK 1   /* FP32 storage and flux arithmetic; FP64 pressure floor, dt and
+                             compensated conservation sums */
+
+/* ---- field ids ---------------------------------------------------------------------- */
+#define IMH_F_MASS 0
+#define IMH_F_MOMX 1
+#define IMH_F_MOMY 2
+#define IMH_F_MOMZ 3
+#define IMH_F_ENER 4
+#define IMH_F_MAGX 5
+#define IMH_F_MAGY 6
+#define IMH_F_MAGZ 7
+#define IMH_F_CMAGX 8     /* cell-centred B (MagnetArray.cellMag) */
+#define IMH_F_CMAGY 9
+#define IMH_F_CMAGZ 10
+#define IMH_NFIELDS 11
+
+/* Run settings: the fields of the reference's managers that reach this path
+ * (ImogenManager.DGRID/GAMMA, FluidManager.MINMASS/limiter/ACTIVE,
+ * MagnetManager.ACTIVE, TimeManager.CFL/updateMode/ITERMAX/TIMEMAX,
+ * BCManager.modes/infinity). */
+typedef struct {
+    int grid[3];          /* ini.grid */
+    double dgrid[3];      /* run.DGRID (uniform spacing) */
+    double gamma;         /* run.GAMMA */
+    double minmass;       /* run.fluid.MINMASS */
+    double cfl;           /* run.time.CFL */
+    int bc_infinity;      /* run.bc.infinity */
+    int bc[2][3];         /* run.bc.modes: [side 0 lower / 1 upper][dim] */
+    int limiter[3];       /* run.fluid.limiter{dim}; run.magnet.limiter is the same */
+    int fluid_active;     /* run.fluid.ACTIVE */
+    int magnet_active;    /* run.magnet.ACTIVE */
+    int update_mode;      /* run.time.updateMode: 0 per step, 1 per iteration */
+    int iter_max;         /* run.time.ITERMAX */
+    double time_max;      /* run.time.TIMEMAX */
+    int perms_order;      /* 0: perms() row order of MATLAB R2009b-R2010b; 1: reverse lexicographic */
+    int precision;        /* IMH_FP64 or IMH_FP32_BULK */
+    int device;           /* HIP device ordinal */
+} imh_config;
+
+typedef struct imh_ctx imh_ctx;
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *imh_last_error(void);
+
+/* Library version string. */
+const char *imh_version(void);
+
+/* ---- lifecycle --------------------------------------------------------------------- */
+/* Replaces the manager set-up of initialize.m:24-190 for this path. */
+int imh_create(const imh_config *cfg, imh_ctx **out);
+int imh_destroy(imh_ctx *ctx);
+
+/* ---- state ------------------------------------------------------------------------- */
+/* Initial state; builds what the FluidArray/MagnetArray constructors build
+ * (imogen.m:34-42, FluidArray.m:170-194, MagnetArray.m:135-143): the data, the edge
+ * stores taken from the IC, MagnetArray.isZero and the initial cell-centred B.
+ * n = grid[0]*grid[1]*grid[2] elements per array. */
+int imh_upload_state(imh_ctx *ctx, const double *mass, const double *const mom[3],
+                     const double *ener, const double *const mag[3], size_t n);
+/* Overwrite one field's data (no constructor side effects). */
+int imh_upload_field(imh_ctx *ctx, int field, const double *colmajor, size_t n);
+/* Copy one field to the host (converted to FP64). */
+int imh_download_field(imh_ctx *ctx, int field, double *out, size_t n);
+
+/* ---- hot path ---------------------------------------------------------------------- */
+/* relaxingFluid(run,mass,mom,ener,mag,grav,X) -- fluid/relaxingFluid.m:1-76.
+ * X in 1..3; uses the context's current dTime (imh_set_dtime or imh_time_update). */
+int imh_fluid_sweep(imh_ctx *ctx, int X);
+/* magnetFlux(run,mass,mom,mag,X,magneticIndices) -- magnet/magnetFlux.m:1-32. */
+int imh_magnet_sweep(imh_ctx *ctx, int X, const int I[2]);
+/* run.time.update(mass,mom,ener,mag,i) -- time/TimeManager.m:55-99.  dt stays on the
+ * device; nothing is copied back. */
+int imh_time_update(imh_ctx *ctx, int step_in_iteration);
+/* flux(run,...,order) -- flux/flux.m:1-64 (order +1 / -1). */
+int imh_flux(imh_ctx *ctx, int order);
+/* One pass of the imogen.m:61-71 loop body: two timesteps with device-side dt and the
+ * iteration counter advanced (TimeManager.step). */
+int imh_iteration(imh_ctx *ctx);
+/* Run iterations while TimeManager.running holds (iteration <= ITERMAX and
+ * time < TIMEMAX), at most max_iterations; *done gets the number run.  The per-iteration
+ * launch sequence is captured once into a hipGraph and replayed. */
+int imh_run(imh_ctx *ctx, int max_iterations, int *done);
+
+/* ---- time state -------------------------------------------------------------------- */
+int imh_set_dtime(imh_ctx *ctx, double dt);
+/* Blocks until the stream is idle, then reads the device time state. */
+int imh_get_time(imh_ctx *ctx, double *time, double *dtime, int *iteration, int *grid_index);
+int imh_set_iteration(imh_ctx *ctx, int iteration);
+int imh_synchronize(imh_ctx *ctx);
+
+/* Totals of mass, momentum (3), energy and B (3) in FP64 with Neumaier compensation,
+ * reduced in a fixed order (the result does not depend on launch shape). */
+int imh_conservation_sums(imh_ctx *ctx, double out[8]);
+
+/* Device bytes held by the context. */
+int imh_memory_bytes(imh_ctx *ctx, size_t *bytes);
+
+/* ---- batched ensemble (many independent simulations, one launch per sweep phase) -- */
+typedef struct imh_batch imh_batch;
+
+/* All simulations share precision and device; grids, spacings, gamma, BCs, limiters,
+ * CFL and time limits are per simulation. */
+int imh_batch_create(const imh_config *cfgs, int nsims, imh_batch **out);
+int imh_batch_destroy(imh_batch *b);
+int imh_batch_upload_state(imh_batch *b, int sim, const double *mass,
+                           const double *const mom[3], const double *ener,
+                           const double *const mag[3], size_t n);
+int imh_batch_download_field(imh_batch *b, int sim, int field, double *out, size_t n);
+/* Advance every simulation that is still running by up to `iterations` iterations. */
+int imh_batch_step(imh_batch *b, int iterations);
+int imh_batch_get_time(imh_batch *b, int sim, double *time, double *dtime, int *iteration);
+int imh_batch_synchronize(imh_batch *b);
+int imh_batch_memory_bytes(imh_batch *b, size_t *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
