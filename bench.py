@@ -1,0 +1,246 @@
+#!/usr/bin/env python3
+"""Throughput of the relaxing-TVD fluid/magnet update on one MI355X.
+
+Workload (the batched ensemble): 512 one-dimensional MHD Riemann problems with N drawn
+uniformly from [256, 4096] plus 64 two-dimensional Kelvin-Helmholtz boxes with side
+drawn from {128, 192, 256, 384, 512}, seeded.  A step is one batched iteration (two
+timesteps of every simulation, as one pass of the imogen.m loop body), replayed from
+captured hipGraphs.  One cell-update = one cell advanced by one timestep, so a step is
+2 * sum(cells) cell-updates.
+
+Printed as one JSON line:
+  value            batched FP32-bulk Mcell-updates/s over the K timed steps
+  plain_fp64       the same ensemble through the plain FP64 path, one simulation after
+                   the other (imh_run), and the ratio batched / plain
+  error            L1 / L-inf of the FP32-bulk density against the FP64 path after the
+                   same number of iterations (per simulation, max and median), and
+                   against the CPU oracle on the cpu_baseline sample
+  roofline         the dominant kernel of the batched step, timed with HIP events
+                   around each launch on the batch stream (an eager pass of the same
+                   launch plan), algorithmic bytes = compulsory field reads + writes
+  plain_roofline   the same for the plain FP64 path on a 256^3 hydro grid
+  cpu_baseline     the CPU oracle (oracle/, C FP64 restatement), single thread, on a
+                   bounded sample of the ensemble
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+HBM_PEAK_GBS = 8000.0
+KH_SIDES = [128, 192, 256, 384, 512]
+BYTES_NOTE = ("fluid sweep: (5 reads + 5 writes + 3 cell-centred B reads if MHD) x cells x "
+              "element size per launch")
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=6)
+    ap.add_argument("--riemann", type=int, default=512, help="1-D MHD Riemann problems")
+    ap.add_argument("--kh", type=int, default=64, help="2-D Kelvin-Helmholtz boxes")
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 leg")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle leg")
+    ap.add_argument("--no-large", action="store_true", help="skip the 256^3 plain roofline")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--profile-steps", type=int, default=6)
+    return ap.parse_args()
+
+
+def ensemble(seed, n_riemann, n_kh):
+    """(ini, state) of every simulation of the batched ensemble."""
+    from imogen_amd import problems
+    rng = np.random.default_rng(seed)
+    sims = []
+    for i in range(n_riemann):
+        n = int(rng.integers(256, 4097))
+        sims.append(problems.mhd_riemann(n, seed=seed * 1000003 + i, iter_max=10 ** 8))
+    for j in range(n_kh):
+        side = int(rng.choice(KH_SIDES))
+        sims.append(problems.kelvin_helmholtz(side, seed=seed * 1000003 + 500000 + j,
+                                              iter_max=10 ** 8))
+    return sims
+
+
+def ncells(ini):
+    g = ini["grid"]
+    return int(g[0]) * int(g[1]) * int(g[2])
+
+
+def rel_l1(a, ref):
+    return float(np.sum(np.abs(a - ref)) / np.sum(np.abs(ref)))
+
+
+def rel_linf(a, ref):
+    return float(np.max(np.abs(a - ref)) / np.max(np.abs(ref)))
+
+
+def roofline(prof, label):
+    """Dominant kernel kind of a profile {kind: (ms, bytes, launches)}."""
+    kind = max((k for k in prof if k != "bookkeeping"), key=lambda k: prof[k][0])
+    ms, by, cnt = prof[kind]
+    avg_s = ms / 1e3 / max(cnt, 1)
+    achieved = by / max(cnt, 1) / avg_s / 1e9
+    total_ms = sum(v[0] for v in prof.values())
+    return {
+        "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+        "kernel": kind, "workload": label, "launches": cnt,
+        "avg_launch_us": round(avg_s * 1e6, 2),
+        "alg_bytes_per_launch": int(by / max(cnt, 1)),
+        "share_of_step_time": round(ms / total_ms, 3) if total_ms > 0 else None,
+        "by_kind_ms": {k: round(v[0], 3) for k, v in prof.items()},
+    }
+
+
+def batched_leg(sims, args, cells):
+    from imogen_amd.ensemble import Ensemble
+    ens = Ensemble(sims, "fp32")
+    ens.step(args.warmup)          # first iteration of each simulation + graph capture
+    ens.synchronize()
+    t0 = time.perf_counter()
+    ens.step(args.steps)
+    ens.synchronize()
+    dt = time.perf_counter() - t0
+    rate = 2.0 * cells.sum() * args.steps / dt / 1e6
+    return ens, rate, dt
+
+
+def plain_leg(sims, args):
+    """Every simulation through the plain FP64 path on its own (imh_run); returns the
+    rate over the timed K iterations and the final densities."""
+    import imogen_amd
+    total_t, total_cu, rho = 0.0, 0.0, []
+    for ini, st in sims:
+        run = imogen_amd.initialize(ini, "fp64")
+        imogen_amd.create_arrays(run, st)
+        run.run_iterations(args.warmup + 1)
+        run.synchronize()
+        t0 = time.perf_counter()
+        run.run_iterations(args.steps)
+        run.synchronize()
+        total_t += time.perf_counter() - t0
+        total_cu += 2.0 * ncells(ini) * args.steps
+        rho.append(run.download("mass"))
+        run.close()
+    return total_cu / total_t / 1e6, rho, total_t
+
+
+def cpu_leg(sims, args):
+    """The CPU oracle, single thread, on a bounded sample: the ensemble's simulations in
+    order (Riemann problems and every 8th KH box) until cpu_seconds of work is done, each
+    advanced warmup + 1 + steps iterations like the GPU legs."""
+    sys.path.insert(0, os.path.join(HERE, "tests"))
+    from helpers import make_oracle
+    n_iter = args.warmup + 1 + args.steps
+    order = [i for i in range(len(sims)) if sims[i][0]["grid"][1] == 1][:64]
+    order += [i for i in range(len(sims)) if sims[i][0]["grid"][1] > 1][::8]
+    done, t_used, cu = [], 0.0, 0.0
+    for i in order:
+        ini, st = sims[i]
+        orc = make_oracle(ini, st, threads=1)
+        t0 = time.perf_counter()
+        for _ in range(n_iter):
+            orc.iteration()
+        t_used += time.perf_counter() - t0
+        cu += 2.0 * ncells(ini) * n_iter
+        done.append((i, orc.get("mass")))
+        orc.close()
+        if t_used >= args.cpu_seconds:
+            break
+    return cu / t_used / 1e6, done, t_used, n_iter
+
+
+def large_plain_roofline(args):
+    import imogen_amd
+    from imogen_amd import problems
+    ini, st = problems.sedov(256)
+    run = imogen_amd.initialize(ini, "fp64")
+    imogen_amd.create_arrays(run, st)
+    del st
+    run.run_iterations(1)
+    prof = run.profile_iterations(2)
+    run.close()
+    return roofline(prof, "sedov 256^3 hydro FP64, plain path")
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (--gpus %d "
+              "requested); run with --gpus 1" % args.gpus, file=sys.stderr)
+        return 2
+    import imogen_amd._lib as L
+    L.lib()  # fails loudly if the HIP library is missing
+
+    sims = ensemble(args.seed, args.riemann, args.kh)
+    cells = np.array([ncells(ini) for ini, _ in sims], dtype=np.int64)
+    print("ensemble: %d simulations, %d cells" % (len(sims), cells.sum()), file=sys.stderr,
+          flush=True)
+
+    ens, rate, t_batched = batched_leg(sims, args, cells)
+    print("batched fp32: %.1f Mcell-updates/s (%.3f s)" % (rate, t_batched), file=sys.stderr,
+          flush=True)
+    rho32 = [ens.download(i, "mass") for i in range(len(sims))]
+
+    out = {
+        "metric": "Mcell-updates/s batched FP32 vs plain FP64 @L1<=1e-4; p50/p99 solve latency",
+        "value": round(rate, 2), "unit": "Mcell-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(t_batched / args.steps * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "batched ensemble: %d 1-D MHD Riemann N~U[256,4096] + %d 2-D "
+                               "KH side in %s, FP32 bulk (FP64 dt, pressure floor, sums), "
+                               "seed %d" % (args.riemann, args.kh, KH_SIDES, args.seed),
+                   "simulations": len(sims), "cells": int(cells.sum()),
+                   "cell_updates_per_step": int(2 * cells.sum()),
+                   "parallelism": "single GPU, one persistent launch per kernel kind"},
+    }
+
+    if not args.no_plain:
+        prate, rho64, t_plain = plain_leg(sims, args)
+        l1 = [rel_l1(a, b) for a, b in zip(rho32, rho64)]
+        li = [rel_linf(a, b) for a, b in zip(rho32, rho64)]
+        out["plain_fp64"] = {"value": round(prate, 2), "unit": "Mcell-updates/s",
+                             "seconds": round(t_plain, 3),
+                             "ratio_batched_over_plain": round(rate / prate, 2)}
+        out["error"] = {"density_l1_vs_fp64_max": float("%.3e" % max(l1)),
+                        "density_l1_vs_fp64_median": float("%.3e" % np.median(l1)),
+                        "density_linf_vs_fp64_max": float("%.3e" % max(li)),
+                        "iterations": args.warmup + 1 + args.steps}
+        print("plain fp64: %.1f Mcell-updates/s; max L1 %.2e" % (prate, max(l1)),
+              file=sys.stderr, flush=True)
+
+    prof = ens.profile_step(args.profile_steps)
+    out["roofline"] = roofline(prof, "batched ensemble FP32, persistent launches")
+    out["roofline"]["bytes_rule"] = BYTES_NOTE
+    ens.close()
+
+    if not args.no_large:
+        out["plain_roofline"] = large_plain_roofline(args)
+
+    if not args.no_cpu:
+        crate, done, t_cpu, n_iter = cpu_leg(sims, args)
+        out["cpu_baseline"] = {
+            "value": round(crate, 3), "unit": "Mcell-updates/s", "cores": 1, "kind": "port",
+            "sample": "%d of the ensemble's simulations (%d cells), %d iterations each, "
+                      "%.1f s on one host core" % (len(done), sum(ncells(sims[i][0]) for i, _ in done),
+                                                   n_iter, t_cpu)}
+        l1o = [rel_l1(rho32[i], r) for i, r in done]
+        out.setdefault("error", {})["density_l1_vs_oracle_max"] = float("%.3e" % max(l1o))
+        out["error"]["oracle_sample"] = len(done)
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

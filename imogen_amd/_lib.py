@@ -83,19 +83,34 @@ SIGNATURES = {
     "imh_batch_destroy": (_I, [_P]),
     "imh_batch_upload_state": (_I, [_P, _I, _D, _D * 3, _D, _D * 3, ctypes.c_size_t]),
     "imh_batch_download_field": (_I, [_P, _I, _I, _D, ctypes.c_size_t]),
+    "imh_batch_create_view": (_I, [ctypes.POINTER(_P), _I, ctypes.POINTER(_P)]),
     "imh_batch_step": (_I, [_P, _I]),
     "imh_batch_get_time": (_I, [_P, _I, _D, _D, ctypes.POINTER(_I)]),
     "imh_batch_synchronize": (_I, [_P]),
     "imh_batch_memory_bytes": (_I, [_P, ctypes.POINTER(ctypes.c_size_t)]),
 }
 
-# Library-internal tuning hooks (not part of the drop-in boundary).
+# Tuning and measurement entry points (include/imogen_hip_ext.h; not part of the drop-in
+# boundary).
+NKIND = 6
+KERNEL_KINDS = ["cfl", "fluid", "mag_adv", "mag_ct", "cellmag", "bookkeeping"]
 EXTRA = {
     "imh_set_fluid_tile": (_I, [_P, _I, _I, _I, _I]),
     "imh_get_fluid_tile": (_I, [_P, _I, ctypes.POINTER(_I), ctypes.POINTER(_I),
                                 ctypes.POINTER(_I)]),
     "imh_set_graphs": (_I, [_P, _I]),
+    "imh_profile_iterations": (_I, [_P, _I, _D, _D, ctypes.POINTER(_I)]),
+    "imh_batch_profile_step": (_I, [_P, _I, _D, _D, ctypes.POINTER(_I)]),
 }
+
+
+def run_profile(fn, handle, iterations):
+    """Call one of the profiling entry points; returns {kind: (ms, bytes, launches)}."""
+    ms = (ctypes.c_double * NKIND)()
+    by = (ctypes.c_double * NKIND)()
+    cnt = (ctypes.c_int * NKIND)()
+    check(fn(handle, int(iterations), ms, by, cnt))
+    return {KERNEL_KINDS[k]: (ms[k], by[k], cnt[k]) for k in range(NKIND)}
 
 
 def lib():

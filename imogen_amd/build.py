@@ -6,6 +6,7 @@ linked into one shared library next to this file.
 """
 import concurrent.futures as cf
 import os
+import re
 import subprocess
 import sys
 
@@ -23,28 +24,59 @@ COMMON = [
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
 
-SOURCES = ["fluid_sweep_f64.hip", "fluid_sweep_f32.hip", "magnet_sweep.hip", "cfl.hip", "imh_api.cpp"]
+SOURCES = ["magnet_sweep.hip", "cfl.hip", "fluid_dispatch.cpp", "imh_api.cpp", "batch.cpp"]
+VARIANT_SRC = "fluid_sweep_variant.hip"
 
 
-def _compile(src, verbose=False):
+def fluid_variants():
+    """(element size, block size, KMAX) of every fluid sweep variant, read from the
+    IMH_FLUID_VARIANTS list in imh_kernels.h so the build and the dispatcher agree."""
+    text = open(os.path.join(CSRC, "imh_kernels.h")).read()
+    m = re.search(r"#define IMH_FLUID_VARIANTS\(X\)((?:[^\n]*\\\n)*[^\n]*)", text)
+    return [tuple(int(v) for v in t) for t in re.findall(r"X\((\d+), (\d+), (\d+)\)", m.group(1))]
+
+
+def units():
+    """(source, object name, extra flags) of every translation unit."""
+    out = [(s, os.path.splitext(s)[0], []) for s in SOURCES]
+    for e, b, k in fluid_variants():
+        out.append((VARIANT_SRC, "fluid_sweep_v%d_%d_%d" % (e, b, k),
+                    ["-DIMH_FLUID_ESZ=%d" % e, "-DIMH_FLUID_BS=%d" % b, "-DIMH_FLUID_KMAX=%d" % k]))
+    return out
+
+
+def _deps_mtime():
+    names = [f for f in os.listdir(CSRC) if f.endswith((".h", ".hip", ".cpp"))]
+    inc = os.path.join(HERE, "..", "include")
+    paths = [os.path.join(CSRC, f) for f in names] + [os.path.join(inc, f) for f in os.listdir(inc)]
+    return max(os.path.getmtime(p) for p in paths)
+
+
+def _compile(unit, verbose=False, incremental=False):
+    src, name, flags = unit
     path = os.path.join(CSRC, src)
-    obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC, "-x", "hip", "--offload-arch=" + ARCH] + COMMON + ["-c", path, "-o", obj]
+    obj = os.path.join(BUILD, name + ".o")
+    if incremental and os.path.exists(obj) and os.path.getmtime(obj) >= _deps_mtime():
+        return obj, ""
+    cmd = [HIPCC, "-x", "hip", "--offload-arch=" + ARCH] + COMMON + flags + ["-c", path, "-o", obj]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
-        raise RuntimeError("hipcc failed for %s:\n%s\n%s" % (src, r.stdout, r.stderr))
+        raise RuntimeError("hipcc failed for %s:\n%s\n%s" % (name, r.stdout, r.stderr))
     return obj, r.stderr
 
 
-def build(verbose=False, jobs=None):
-    """Compile every source and link libimogen_hip.so.  Always recompiles."""
+def build(verbose=False, jobs=None, incremental=False):
+    """Compile every translation unit and link libimogen_hip.so.  Recompiles everything
+    unless `incremental` (then only objects older than some source or header)."""
     os.makedirs(BUILD, exist_ok=True)
-    srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
-    jobs = jobs or min(len(srcs), max(1, (os.cpu_count() or 2)))
+    us = units()
+    # the long fluid variants first, so they overlap with the short units
+    us.sort(key=lambda u: 0 if u[0] == VARIANT_SRC else 1)
+    jobs = jobs or min(len(us), max(1, (os.cpu_count() or 2)))
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        results = list(ex.map(lambda s: _compile(s, verbose), srcs))
+        results = list(ex.map(lambda u: _compile(u, verbose, incremental), us))
     objs = []
     for obj, warn in results:
         objs.append(obj)
@@ -60,4 +92,4 @@ def build(verbose=False, jobs=None):
 
 
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    print(build(verbose="-v" in sys.argv, incremental="--incremental" in sys.argv))

@@ -24,13 +24,15 @@ __device__ inline void atomic_max_nonneg(unsigned long long* slot, double v) {
 
 }  // namespace
 
+// Max of |p_i/rho| + c_s over cells [c0, c1) of one simulation (grid-strided by
+// `stride`), merged into the simulation's slots with one atomic per block and direction.
 template <typename T>
-__global__ void __launch_bounds__(256) cfl_kernel(const CflArgs<T> a) {
-    if (!a.sim->active) return;
+__device__ __forceinline__ void cfl_body(const CflArgs<T>& a, long long c0, long long c1,
+                                         long long stride) {
     __shared__ double s_red[3][4];
+    if (!a.sim->active) return;
     double m[3] = {NAN, NAN, NAN};
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < a.ncell; c += stride) {
+    for (long long c = c0 + threadIdx.x; c < c1; c += stride) {
         double rho = double(a.q[0][c]);
         double px = double(a.q[1][c]), py = double(a.q[2][c]), pz = double(a.q[3][c]);
         double bx = a.bc[0] ? double(a.bc[0][c]) : 0.0;
@@ -52,45 +54,67 @@ __global__ void __launch_bounds__(256) cfl_kernel(const CflArgs<T> a) {
         double v = fmax(fmax(s_red[k][0], s_red[k][1]), fmax(s_red[k][2], s_red[k][3]));
         atomic_max_nonneg(&a.sim->cmax_bits[k], v);
     }
+    __syncthreads();
+}
+
+constexpr long long CFL_CHUNK = 256 * 32;  // cells per tile of a batched launch
+
+template <typename T>
+__global__ void __launch_bounds__(256) cfl_batch_kernel(const CflArgs<T>* args,
+                                                        const BatchTile* tiles, int ntiles) {
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const BatchTile bt = tiles ? tiles[t] : BatchTile{0, 0, (long long)t};
+        const CflArgs<T>& a = args[bt.arg];
+        long long c0 = bt.tile * a.chunk;
+        long long c1 = c0 + a.chunk < a.ncell ? c0 + a.chunk : a.ncell;
+        cfl_body(a, c0, c1, 256);
+    }
 }
 
 template <typename T>
-hipError_t launch_cfl(const CflArgs<T>& a, hipStream_t st) {
-    long long nb = (a.ncell + 255) / 256;
-    if (nb > 2048) nb = 2048;
-    if (nb < 1) nb = 1;
-    hipLaunchKernelGGL((cfl_kernel<T>), dim3(nb), dim3(256), 0, st, a);
+hipError_t launch_cfl_batch(const CflArgs<T>* args, const BatchTile* tiles, int ntiles,
+                            int nblocks, hipStream_t st) {
+    if (ntiles == 0) return hipSuccess;
+    hipLaunchKernelGGL((cfl_batch_kernel<T>), dim3(nblocks), dim3(256), 0, st, args, tiles, ntiles);
     return hipGetLastError();
 }
 
-template hipError_t launch_cfl<double>(const CflArgs<double>&, hipStream_t);
-template hipError_t launch_cfl<float>(const CflArgs<float>&, hipStream_t);
+template hipError_t launch_cfl_batch<double>(const CflArgs<double>*, const BatchTile*, int, int,
+                                             hipStream_t);
+template hipError_t launch_cfl_batch<float>(const CflArgs<float>*, const BatchTile*, int, int,
+                                            hipStream_t);
 
-// ---- time bookkeeping (one thread per simulation) ------------------------------------
+long long cfl_chunk() { return CFL_CHUNK; }
+
+// The kernels below run one thread per simulation over arrays of per-simulation
+// pointers (a batch is a view over independently allocated simulations).
 
 // TimeManager.running (TimeManager.m:36-39), evaluated at the top of the loop body.
-__global__ void begin_iteration_kernel(SimDev* sim, const SimParams* p, int nsims) {
+__global__ void begin_iteration_kernel(SimDev* const* sim, const SimParams* const* p, int nsims,
+                                       int unconditional) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nsims) return;
-    SimDev& d = sim[s];
-    const SimParams& q = p[s];
+    SimDev& d = *sim[s];
+    const SimParams& q = *p[s];
     int running = (d.iteration <= q.iter_max) && (d.time < q.time_max) && d.error == 0;
-    d.active = (q.unconditional || running) ? 1 : 0;
+    d.active = (unconditional || running) ? 1 : 0;
 }
 
-__global__ void cfl_reset_kernel(SimDev* sim, int nsims) {
+__global__ void cfl_reset_kernel(SimDev* const* sim, int nsims) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nsims) return;
-    if (!sim[s].active) return;
-    for (int k = 0; k < 3; ++k) sim[s].cmax_bits[k] = 0ull;
+    SimDev& d = *sim[s];
+    if (!d.active) return;
+    for (int k = 0; k < 3; ++k) d.cmax_bits[k] = 0ull;
 }
 
 // dt from the reduced speeds (TimeManager.m:65-98).
-__global__ void time_finalize_kernel(SimDev* sim, const SimParams* p, int nsims, int step) {
+__global__ void time_finalize_kernel(SimDev* const* sim, const SimParams* const* p, int nsims,
+                                     int step) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nsims) return;
-    SimDev& d = sim[s];
-    const SimParams& q = p[s];
+    SimDev& d = *sim[s];
+    const SimParams& q = *p[s];
     if (!d.active) return;
     if (step > 1 && q.update_mode != 0) return;
     double cmax = 1e-2;
@@ -119,29 +143,31 @@ __global__ void time_finalize_kernel(SimDev* sim, const SimParams* p, int nsims,
 }
 
 // TimeManager.step: iteration = iteration + 1.
-__global__ void end_iteration_kernel(SimDev* sim, int nsims) {
+__global__ void end_iteration_kernel(SimDev* const* sim, int nsims) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nsims) return;
-    if (sim[s].active) sim[s].iteration += 1;
+    if (sim[s]->active) sim[s]->iteration += 1;
 }
 
 static inline int nblk(int n) { return (n + 255) / 256; }
 
-hipError_t launch_begin_iteration(SimDev* sim, const SimParams* p, int nsims, hipStream_t st) {
-    hipLaunchKernelGGL(begin_iteration_kernel, dim3(nblk(nsims)), dim3(256), 0, st, sim, p, nsims);
+hipError_t launch_begin_iteration(SimDev* const* sim, const SimParams* const* p, int nsims,
+                                  int unconditional, hipStream_t st) {
+    hipLaunchKernelGGL(begin_iteration_kernel, dim3(nblk(nsims)), dim3(256), 0, st, sim, p, nsims,
+                       unconditional);
     return hipGetLastError();
 }
-hipError_t launch_cfl_reset(SimDev* sim, int nsims, hipStream_t st) {
+hipError_t launch_cfl_reset(SimDev* const* sim, int nsims, hipStream_t st) {
     hipLaunchKernelGGL(cfl_reset_kernel, dim3(nblk(nsims)), dim3(256), 0, st, sim, nsims);
     return hipGetLastError();
 }
-hipError_t launch_time_finalize(SimDev* sim, const SimParams* p, int nsims, int step,
-                                hipStream_t st) {
+hipError_t launch_time_finalize(SimDev* const* sim, const SimParams* const* p, int nsims,
+                                int step, hipStream_t st) {
     hipLaunchKernelGGL(time_finalize_kernel, dim3(nblk(nsims)), dim3(256), 0, st, sim, p, nsims,
                        step);
     return hipGetLastError();
 }
-hipError_t launch_end_iteration(SimDev* sim, int nsims, hipStream_t st) {
+hipError_t launch_end_iteration(SimDev* const* sim, int nsims, hipStream_t st) {
     hipLaunchKernelGGL(end_iteration_kernel, dim3(nblk(nsims)), dim3(256), 0, st, sim, nsims);
     return hipGetLastError();
 }

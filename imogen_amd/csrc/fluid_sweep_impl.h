@@ -72,8 +72,11 @@ __device__ inline T pencil_reduce_max(T v, const FluidArgs<T>& a, int tid, int p
 
 }  // namespace
 
+// The sweep of one tile (PX pencils) of one simulation.  Called by the per-simulation
+// kernel (one block per tile) and by the batched persistent kernel (a block loops over
+// a tile list that mixes simulations of different sizes).
 template <typename T, int BS, int KMAX, bool MHD>
-__global__ void __launch_bounds__(BS) fluid_sweep_kernel(const FluidArgs<T> a) {
+__device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& a, long long tile) {
     if (!a.force && !a.sim->active) return;
     __shared__ T s_x[2][4][BS];
     __shared__ double s_ends[2][2][PXMAX][6];
@@ -86,7 +89,6 @@ __global__ void __launch_bounds__(BS) fluid_sweep_kernel(const FluidArgs<T> a) {
     const int px = a.xmajor ? tid / a.TL : tid % a.PX;
     const int ty = a.xmajor ? tid % a.TL : tid / a.PX;
     const int dstep = a.xmajor ? 1 : a.PX;  // thread distance between neighbouring chunks
-    const long long tile = a.xcd_remap ? xcd_tile(blockIdx.x, gridDim.x) : (long long)blockIdx.x;
     const long long pen0 = tile * a.PX;
     const long long pen = pen0 + px;
     const bool pen_ok = pen < a.npencil;
@@ -457,34 +459,49 @@ __global__ void __launch_bounds__(BS) fluid_sweep_kernel(const FluidArgs<T> a) {
     }
 }
 
-// ------------------------------------------------------------------------------------
-// Launchers.  The translation unit is compiled once per precision (IMH_FLUID_T = double
-// or float, see fluid_sweep_f64.hip / fluid_sweep_f32.hip) with the (BS, KMAX) variants
-// of that precision; the host picks one per sweep dimension (imh_api.cpp).
-// ------------------------------------------------------------------------------------
-template <typename T, int BS, int KMAX>
-static hipError_t launch_fluid_t(const FluidArgs<T>& a, bool mhd, long long nblocks,
-                                 hipStream_t st) {
-    if (mhd)
-        hipLaunchKernelGGL((fluid_sweep_kernel<T, BS, KMAX, true>), dim3(nblocks), dim3(BS), 0,
-                           st, a);
-    else
-        hipLaunchKernelGGL((fluid_sweep_kernel<T, BS, KMAX, false>), dim3(nblocks), dim3(BS), 0,
-                           st, a);
-    return hipGetLastError();
+// Persistent batched sweep: the grid is sized to the device and every block walks the
+// tile list (largest tiles first), so hundreds of uneven simulations share one launch.
+// With tiles == nullptr it is the sweep of the single simulation args[0], one block per
+// tile, with an XCD-aware block-to-tile map (neighbouring pencils share an L2).
+template <typename T, int BS, int KMAX, bool MHD>
+__global__ void __launch_bounds__(BS) fluid_sweep_batch_kernel(const FluidArgs<T>* args,
+                                                               const BatchTile* tiles,
+                                                               int ntiles) {
+    if (!tiles) {
+        const long long t = args[0].xcd_remap ? xcd_tile(blockIdx.x, gridDim.x)
+                                               : (long long)blockIdx.x;
+        fluid_sweep_body<T, BS, KMAX, MHD>(args[0], t);
+        return;
+    }
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const BatchTile bt = tiles[t];
+        fluid_sweep_body<T, BS, KMAX, MHD>(args[bt.arg], bt.tile);
+        __syncthreads();
+    }
 }
 
-template <>
-hipError_t launch_fluid_sweep<IMH_FLUID_T>(const FluidArgs<IMH_FLUID_T>& a, int bs, int kmax,
-                                           bool mhd, hipStream_t st) {
-    using T = IMH_FLUID_T;
-    long long nblocks = (a.npencil + a.PX - 1) / a.PX;
-    if (nblocks == 0) return hipSuccess;
-#define IMH_CASE(B, KM) \
-    if (bs == B && kmax == KM) return launch_fluid_t<T, B, KM>(a, mhd, nblocks, st);
-    IMH_FLUID_VARIANTS
-#undef IMH_CASE
-    return hipErrorInvalidValue;
+// ------------------------------------------------------------------------------------
+// Launcher of this translation unit's variant.  fluid_sweep_variant.hip is compiled once
+// per entry of IMH_FLUID_VARIANTS (imh_kernels.h) with IMH_FLUID_ESZ / IMH_FLUID_BS /
+// IMH_FLUID_KMAX defined, so the variants build in parallel; launch_fluid_batch
+// (fluid_dispatch.cpp) picks one per sweep dimension.
+// ------------------------------------------------------------------------------------
+#if IMH_FLUID_ESZ == 8
+using FluidT = double;
+#else
+using FluidT = float;
+#endif
+
+hipError_t IMH_FLUID_LAUNCHER(IMH_FLUID_ESZ, IMH_FLUID_BS, IMH_FLUID_KMAX)(
+    const void* args, const BatchTile* tiles, int ntiles, int nblocks, bool mhd, hipStream_t st) {
+    const FluidArgs<FluidT>* a = (const FluidArgs<FluidT>*)args;
+    if (mhd)
+        hipLaunchKernelGGL((fluid_sweep_batch_kernel<FluidT, IMH_FLUID_BS, IMH_FLUID_KMAX, true>),
+                           dim3(nblocks), dim3(IMH_FLUID_BS), 0, st, a, tiles, ntiles);
+    else
+        hipLaunchKernelGGL((fluid_sweep_batch_kernel<FluidT, IMH_FLUID_BS, IMH_FLUID_KMAX, false>),
+                           dim3(nblocks), dim3(IMH_FLUID_BS), 0, st, a, tiles, ntiles);
+    return hipGetLastError();
 }
 
 }  // namespace imh

@@ -108,11 +108,7 @@ static int next_pow2(int v) {
 // (256,4), (512,4), (512,8)}.  Pencils along x are contiguous (threads of a pencil are
 // adjacent lanes); along y/z several adjacent pencils share a block so that each row
 // access is a contiguous run.
-bool fluid_variant_exists(int esz, int bs, int kmax) {
-    if (esz == 8)
-        return (bs == 256 && (kmax == 1 || kmax == 2)) || (bs == 512 && (kmax == 2 || kmax == 4));
-    return (bs == 256 && (kmax == 2 || kmax == 4)) || (bs == 512 && (kmax == 4 || kmax == 8));
-}
+bool fluid_variant_exists(int esz, int bs, int kmax) { return fluid_variant_compiled(esz, bs, kmax); }
 
 FluidTile choose_fluid_tile(int N, int d, int esz, const TileOverride* ov) {
     FluidTile t;
@@ -176,12 +172,106 @@ static void ctx_free(imh_ctx* c) {
     if (c->eoff_dev) hipFree(c->eoff_dev);
     if (c->sim) hipFree(c->sim);
     if (c->par) hipFree(c->par);
-    if (c->par_uncond) hipFree(c->par_uncond);
+    if (c->arg_host) hipHostFree(c->arg_host);
+    if (c->arg_dev) hipFree(c->arg_dev);
+    for (int i = 0; i < 6; ++i)
+        if (c->cap_dev[i]) hipFree(c->cap_dev[i]);
+    delete[] c->cap_host;
+    if (c->sim_ptr) hipFree(c->sim_ptr);
+    if (c->par_ptr) hipFree(c->par_ptr);
     if (c->fade_dev) hipFree(c->fade_dev);
     if (c->csum_part) hipFree(c->csum_part);
     if (c->csum_out) hipFree(c->csum_out);
-    if (c->st) hipStreamDestroy(c->st);
+    if (c->st && !c->external_stream) hipStreamDestroy(c->st);
     delete c;
+}
+
+void imh_ctx_free(imh_ctx* c) { ctx_free(c); }
+
+namespace imh {
+
+void prof_begin(Profiler* p, hipStream_t st, int kind, double bytes) {
+    ProfRec r{kind, bytes, nullptr, nullptr};
+    if (hipEventCreate(&r.a) != hipSuccess || hipEventCreate(&r.b) != hipSuccess ||
+        hipEventRecord(r.a, st) != hipSuccess)
+        p->failed = true;
+    p->recs.push_back(r);
+}
+
+void prof_end(Profiler* p, hipStream_t st) {
+    if (p->recs.empty() || hipEventRecord(p->recs.back().b, st) != hipSuccess) p->failed = true;
+}
+
+int prof_collect(Profiler* p, double* ms, double* bytes, int* count) {
+    for (int k = 0; k < IMH_NKIND; ++k) {
+        if (ms) ms[k] = 0;
+        if (bytes) bytes[k] = 0;
+        if (count) count[k] = 0;
+    }
+    hipError_t err = hipSuccess;
+    for (ProfRec& r : p->recs) {
+        float t = 0;
+        hipError_t e = (r.a && r.b) ? hipEventSynchronize(r.b) : hipErrorInvalidValue;
+        if (e == hipSuccess) e = hipEventElapsedTime(&t, r.a, r.b);
+        if (e != hipSuccess && err == hipSuccess) err = e;
+        if (e == hipSuccess && r.kind >= 0 && r.kind < IMH_NKIND) {
+            if (ms) ms[r.kind] += t;
+            if (bytes) bytes[r.kind] += r.bytes;
+            if (count) count[r.kind] += 1;
+        }
+        if (r.a) hipEventDestroy(r.a);
+        if (r.b) hipEventDestroy(r.b);
+    }
+    p->recs.clear();
+    if (p->failed) return fail(IMH_EHIP, "event creation or recording failed while profiling");
+    return err == hipSuccess ? IMH_OK : hip_fail(err, "profile events");
+}
+
+}  // namespace imh
+
+// Event pair around one launch on c->st while a profile runs.
+struct ProfScope {
+    imh_ctx* c;
+    ProfScope(imh_ctx* ctx, int kind, double bytes) : c(ctx) {
+        if (c->prof) prof_begin(c->prof, c->st, kind, bytes);
+    }
+    ~ProfScope() {
+        if (c->prof) prof_end(c->prof, c->st);
+    }
+};
+
+const void* imh_stage_args(imh_ctx* c, const void* a, size_t n) {
+    if (n > ARG_SLOT) {
+        fail(IMH_EINVAL, "argument block of %zu bytes exceeds a slot", n);
+        return nullptr;
+    }
+    if (c->capture_row >= 0) {
+        if (c->cap_count >= CAP_SLOTS) {
+            fail(IMH_EINVAL, "more than %d launches in one captured iteration", CAP_SLOTS);
+            return nullptr;
+        }
+        const int k = c->cap_count++;
+        memcpy(c->cap_host + k * ARG_SLOT, a, n);
+        return c->cap_dev[c->capture_row] + k * ARG_SLOT;
+    }
+    if (c->arg_next == ARG_RING) {
+        // every earlier copy out of the ring has to be done before a slot is rewritten
+        hipError_t e = hipStreamSynchronize(c->st);
+        if (e != hipSuccess) {
+            hip_fail(e, "argument ring");
+            return nullptr;
+        }
+        c->arg_next = 0;
+    }
+    const int k = c->arg_next++;
+    char* h = c->arg_host + k * ARG_SLOT;
+    memcpy(h, a, n);
+    hipError_t e = hipMemcpyAsync(c->arg_dev + k * ARG_SLOT, h, n, hipMemcpyHostToDevice, c->st);
+    if (e != hipSuccess) {
+        hip_fail(e, "argument copy");
+        return nullptr;
+    }
+    return c->arg_dev + k * ARG_SLOT;
 }
 
 // Offsets of the edge stores (objects x sides x dims) for trans/wall sides.
@@ -262,8 +352,16 @@ int imh_create(const imh_config* cfg, imh_ctx** out) {
     CHK(hipMemcpy(c->fade_dev, c->fade, sizeof(c->fade), hipMemcpyHostToDevice), "fade upload");
     CHK(hipMalloc(&c->sim, sizeof(SimDev)), "hipMalloc(sim)");
     CHK(hipMalloc(&c->par, sizeof(SimParams)), "hipMalloc(par)");
+    CHK(hipMalloc(&c->sim_ptr, sizeof(SimDev*)), "hipMalloc(sim_ptr)");
+    CHK(hipMalloc(&c->par_ptr, sizeof(SimParams*)), "hipMalloc(par_ptr)");
+    CHK(hipMemcpy(c->sim_ptr, &c->sim, sizeof(SimDev*), hipMemcpyHostToDevice), "sim_ptr");
+    CHK(hipMemcpy(c->par_ptr, &c->par, sizeof(SimParams*), hipMemcpyHostToDevice), "par_ptr");
     CHK(hipMalloc(&c->csum_part, sizeof(double) * 2 * 1024 * 8), "hipMalloc(csum)");
     CHK(hipMalloc(&c->csum_out, sizeof(double) * 8), "hipMalloc(csum_out)");
+    CHK(hipHostMalloc((void**)&c->arg_host, ARG_SLOT * ARG_RING, hipHostMallocDefault),
+        "hipHostMalloc(args)");
+    CHK(hipMalloc(&c->arg_dev, ARG_SLOT * ARG_RING), "hipMalloc(args)");
+    c->cap_host = new char[ARG_SLOT * CAP_SLOTS];
     SimDev sd;
     memset(&sd, 0, sizeof(sd));
     sd.iteration = 1;  // imogen.m:50
@@ -354,7 +452,7 @@ static void host_edges_from(imh_ctx* c, std::vector<double>& h, int obj, const d
 }
 
 template <typename T>
-static int launch_cellmag(imh_ctx* c, int k, int force) {
+CellMagArgs<T> make_cellmag_args(const imh_ctx* c, int k, int force) {
     CellMagArgs<T> a;
     memset(&a, 0, sizeof(a));
     a.b = (const T*)c->f[IMH_F_MAGX + k];
@@ -372,7 +470,19 @@ static int launch_cellmag(imh_ctx* c, int k, int force) {
     a.bc_inf = c->cfg.bc_infinity;
     a.force = force;
     a.fade = c->fade_dev;
-    hipError_t e = launch_cell_centered(a, c->st);
+    return a;
+}
+template CellMagArgs<double> make_cellmag_args<double>(const imh_ctx*, int, int);
+template CellMagArgs<float> make_cellmag_args<float>(const imh_ctx*, int, int);
+
+template <typename T>
+static int launch_cellmag(imh_ctx* c, int k, int force) {
+    CellMagArgs<T> a = make_cellmag_args<T>(c, k, force);
+    const void* da = imh_stage_args(c, &a, sizeof(a));
+    if (!da) return IMH_EHIP;
+    const int nt = int((c->ncell + 255) / 256);
+    ProfScope ps(c, IMH_K_CELLMAG, 2.0 * c->ncell * sizeof(T));
+    hipError_t e = launch_cell_centered_batch((const CellMagArgs<T>*)da, nullptr, nt, nt, c->st);
     return e == hipSuccess ? IMH_OK : hip_fail(e, "cell_centered launch");
 }
 
@@ -459,7 +569,7 @@ extern "C" int imh_download_field(imh_ctx* c, int field, double* out, size_t n) 
 // ---- sweeps -------------------------------------------------------------------------
 
 template <typename T>
-static int fluid_sweep_t(imh_ctx* c, int d, int force) {
+FluidArgs<T> make_fluid_args(const imh_ctx* c, int d, int force) {
     FluidArgs<T> a;
     memset(&a, 0, sizeof(a));
     for (int v = 0; v < 5; ++v) a.q[v] = (T*)c->f[v];
@@ -490,7 +600,21 @@ static int fluid_sweep_t(imh_ctx* c, int d, int force) {
     a.xcd_remap = 1;
     a.force = force;
     a.fade = c->fade_dev;
-    hipError_t e = launch_fluid_sweep(a, t.bs, t.kmax, c->mhd_kernel, c->st);
+    return a;
+}
+template FluidArgs<double> make_fluid_args<double>(const imh_ctx*, int, int);
+template FluidArgs<float> make_fluid_args<float>(const imh_ctx*, int, int);
+
+template <typename T>
+static int fluid_sweep_t(imh_ctx* c, int d, int force) {
+    FluidArgs<T> a = make_fluid_args<T>(c, d, force);
+    const FluidTile& t = c->ftile[d];
+    const void* da = imh_stage_args(c, &a, sizeof(a));
+    if (!da) return IMH_EHIP;
+    const int nt = int((a.npencil + t.PX - 1) / t.PX);
+    ProfScope ps(c, IMH_K_FLUID, double(10 + (c->mhd_kernel ? 3 : 0)) * c->ncell * sizeof(T));
+    hipError_t e = launch_fluid_batch((const FluidArgs<T>*)da, nullptr, nt, nt, t.bs, t.kmax,
+                                      c->mhd_kernel, c->st);
     if (e != hipSuccess) return hip_fail(e, "fluid sweep launch");
     c->fluid_init = true;
     return IMH_OK;
@@ -506,56 +630,78 @@ int imh_fluid_sweep_internal(imh_ctx* c, int X, int force) {
 }
 
 template <typename T>
+MagnetArgs<T> make_magnet_args(const imh_ctx* c, int dX, int dI, int force) {
+    MagnetArgs<T> a;
+    memset(&a, 0, sizeof(a));
+    a.rho = (const T*)c->f[IMH_F_MASS];
+    a.mom = (const T*)c->f[IMH_F_MOMX + dX];
+    a.bI = (T*)c->f[IMH_F_MAGX + dI];
+    a.bX = (T*)c->f[IMH_F_MAGX + dX];
+    a.flux = (T*)c->scratch;
+    a.edges = c->edges;
+    a.eoff = c->eoff_dev;
+    a.sim = c->sim;
+    a.n0 = c->n[0];
+    a.n1 = c->n[1];
+    a.n2 = c->n[2];
+    a.dX = dX;
+    a.dI = dI;
+    a.ndims = c->ndims;
+    a.modes = pack_modes(c->cfg.bc);
+    a.active_linX = c->active_lin[dX];
+    a.active_linI = c->active_lin[dI];
+    a.limiter = c->cfg.limiter[dX];
+    a.dxX = c->cfg.dgrid[dX];
+    a.dxI = c->cfg.dgrid[dI];
+    int pair = 3 * dI + dX;
+    a.init_vg = !c->obj_init[OBJ_VG + pair];
+    a.init_store = !c->obj_init[OBJ_MSFR + pair];
+    a.init_wmag = !c->obj_init[OBJ_WMAG + pair];
+    a.init_flux = !c->obj_init[OBJ_MFLUX + pair];
+    a.PX = c->mtile[dX].PX;
+    a.TL = c->mtile[dX].TL;
+    a.npencil = c->nface[dX];
+    a.bc_inf = c->cfg.bc_infinity;
+    a.force = force;
+    a.fade = c->fade_dev;
+    return a;
+}
+template MagnetArgs<double> make_magnet_args<double>(const imh_ctx*, int, int, int);
+template MagnetArgs<float> make_magnet_args<float>(const imh_ctx*, int, int, int);
+
+template <typename T>
 static int magnet_sweep_t(imh_ctx* c, int dX, const int I[2], int force) {
     int lastI = I[1] - 1;
     for (int t = 0; t < 2; ++t) {
         int dI = I[t] - 1;
-        MagnetArgs<T> a;
-        memset(&a, 0, sizeof(a));
-        a.rho = (const T*)c->f[IMH_F_MASS];
-        a.mom = (const T*)c->f[IMH_F_MOMX + dX];
-        a.bI = (T*)c->f[IMH_F_MAGX + dI];
-        a.bX = (T*)c->f[IMH_F_MAGX + dX];
-        a.flux = (T*)c->scratch;
-        a.edges = c->edges;
-        a.eoff = c->eoff_dev;
-        a.sim = c->sim;
-        a.n0 = c->n[0];
-        a.n1 = c->n[1];
-        a.n2 = c->n[2];
-        a.dX = dX;
-        a.dI = dI;
-        a.ndims = c->ndims;
-        a.modes = pack_modes(c->cfg.bc);
-        a.active_linX = c->active_lin[dX];
-        a.active_linI = c->active_lin[dI];
-        a.limiter = c->cfg.limiter[dX];
-        a.dxX = c->cfg.dgrid[dX];
-        a.dxI = c->cfg.dgrid[dI];
-        int pair = 3 * dI + dX;
-        a.init_vg = !c->obj_init[OBJ_VG + pair];
-        a.init_store = !c->obj_init[OBJ_MSFR + pair];
-        a.init_wmag = !c->obj_init[OBJ_WMAG + pair];
-        a.init_flux = !c->obj_init[OBJ_MFLUX + pair];
-        a.PX = c->mtile[dX].PX;
-        a.TL = c->mtile[dX].TL;
-        a.npencil = c->nface[dX];
-        a.bc_inf = c->cfg.bc_infinity;
-        a.force = force;
-        a.fade = c->fade_dev;
-        hipError_t e = launch_magnet_advect(a, 256, c->st);
+        MagnetArgs<T> a = make_magnet_args<T>(c, dX, dI, force);
+        const void* da = imh_stage_args(c, &a, sizeof(a));
+        if (!da) return IMH_EHIP;
+        const int N = c->n[dX];
+        const int kmax = magnet_kmax_for(N, a.TL);
+        const size_t smem = sizeof(T) * 2 * a.PX * (size_t)(N + 4);
+        const int nt = int((a.npencil + a.PX - 1) / a.PX);
+        if (kmax < 0) return fail(IMH_ENOTSUP, "magnet pencil of %d cells is too long", N);
+        hipError_t e;
+        {
+            ProfScope ps(c, IMH_K_MAG_ADV, 5.0 * c->ncell * sizeof(T));
+            e = launch_magnet_batch(0, (const MagnetArgs<T>*)da, nullptr, nt, nt, kmax, smem, c->st);
+        }
         if (e != hipSuccess) return hip_fail(e, "magnet advect launch");
-        e = launch_magnet_ct(a, c->st);
+        {
+            ProfScope ps(c, IMH_K_MAG_CT, 3.0 * c->ncell * sizeof(T));
+            e = launch_magnet_batch(1, (const MagnetArgs<T>*)da, nullptr, nt, nt, kmax, smem / 2,
+                                    c->st);
+        }
         if (e != hipSuccess) return hip_fail(e, "magnet ct launch");
+        int pair = 3 * dI + dX;
         c->obj_init[OBJ_VG + pair] = c->obj_init[OBJ_MSFR + pair] = true;
         c->obj_init[OBJ_WMAG + pair] = c->obj_init[OBJ_MFLUX + pair] = true;
     }
     // mag(I).updateCellCentered() for the last I only, then mag(X) (magnetFlux.m:30-31)
-    int rc = c->cfg.precision == IMH_FP64 ? launch_cellmag<double>(c, lastI, force)
-                                          : launch_cellmag<float>(c, lastI, force);
+    int rc = launch_cellmag<T>(c, lastI, force);
     if (rc) return rc;
-    return c->cfg.precision == IMH_FP64 ? launch_cellmag<double>(c, dX, force)
-                                        : launch_cellmag<float>(c, dX, force);
+    return launch_cellmag<T>(c, dX, force);
 }
 
 int imh_magnet_sweep_internal(imh_ctx* c, int X, const int I[2], int force) {
@@ -571,25 +717,51 @@ int imh_magnet_sweep_internal(imh_ctx* c, int X, const int I[2], int force) {
 }
 
 template <typename T>
-static int cfl_t(imh_ctx* c) {
+CflArgs<T> make_cfl_args(const imh_ctx* c) {
     CflArgs<T> a;
     for (int v = 0; v < 5; ++v) a.q[v] = (const T*)c->f[v];
-    for (int k = 0; k < 3; ++k) a.bc[k] = (const T*)c->f[IMH_F_CMAGX + k];
+    // components that pressure.m leaves out (MagnetArray.isZero) are not read at all
+    for (int k = 0; k < 3; ++k)
+        a.bc[k] = c->mag_is_zero[k] ? nullptr : (const T*)c->f[IMH_F_CMAGX + k];
     a.sim = c->sim;
     a.ncell = c->ncell;
+    a.chunk = cfl_chunk();
     a.use = {!c->mag_is_zero[0], !c->mag_is_zero[1], !c->mag_is_zero[2]};
     a.gamma = c->cfg.gamma;
-    hipError_t e = launch_cfl(a, c->st);
+    return a;
+}
+template CflArgs<double> make_cfl_args<double>(const imh_ctx*);
+template CflArgs<float> make_cfl_args<float>(const imh_ctx*);
+
+template <typename T>
+static int cfl_t(imh_ctx* c) {
+    CflArgs<T> a = make_cfl_args<T>(c);
+    // one simulation: at most 2048 tiles of a multiple of 256 cells
+    a.chunk = ((c->ncell + 2048LL * 256 - 1) / (2048LL * 256)) * 256;
+    const void* da = imh_stage_args(c, &a, sizeof(a));
+    if (!da) return IMH_EHIP;
+    const int nt = int((c->ncell + a.chunk - 1) / a.chunk);
+    int nb = 5;
+    for (int k = 0; k < 3; ++k) nb += a.bc[k] ? 1 : 0;
+    ProfScope ps(c, IMH_K_CFL, double(nb) * c->ncell * sizeof(T));
+    hipError_t e = launch_cfl_batch((const CflArgs<T>*)da, nullptr, nt, nt, c->st);
     return e == hipSuccess ? IMH_OK : hip_fail(e, "cfl launch");
 }
 
 int imh_time_update_internal(imh_ctx* c, int step) {
     if (step > 1 && c->cfg.update_mode != 0) return IMH_OK;  // TimeManager.m:61-63
-    hipError_t e = launch_cfl_reset(c->sim, 1, c->st);
+    hipError_t e;
+    {
+        ProfScope ps(c, IMH_K_BOOK, 0.0);
+        e = launch_cfl_reset(c->sim_ptr, 1, c->st);
+    }
     if (e != hipSuccess) return hip_fail(e, "cfl reset");
     int rc = c->cfg.precision == IMH_FP64 ? cfl_t<double>(c) : cfl_t<float>(c);
     if (rc) return rc;
-    e = launch_time_finalize(c->sim, c->par, 1, step, c->st);
+    {
+        ProfScope ps(c, IMH_K_BOOK, 0.0);
+        e = launch_time_finalize(c->sim_ptr, c->par_ptr, 1, step, c->st);
+    }
     return e == hipSuccess ? IMH_OK : hip_fail(e, "time finalize");
 }
 
@@ -615,27 +787,25 @@ int imh_flux_internal(imh_ctx* c, int order, int force) {
 }
 
 // One imogen.m:61-71 loop body; `conditional` = 0 runs it unconditionally (imh_iteration).
-static int iteration_body(imh_ctx* c, int conditional) {
+int imh_iteration_body(imh_ctx* c, int conditional) {
     int rc;
-    hipError_t e = launch_begin_iteration(c->sim, conditional ? c->par : c->par_uncond, 1, c->st);
+    hipError_t e;
+    {
+        ProfScope ps(c, IMH_K_BOOK, 0.0);
+        e = launch_begin_iteration(c->sim_ptr, c->par_ptr, 1, conditional ? 0 : 1, c->st);
+    }
     if (e != hipSuccess) return hip_fail(e, "begin iteration");
     for (int i = 1; i <= 2; ++i) {
         if ((rc = imh_time_update_internal(c, i))) return rc;
         if ((rc = imh_flux_internal(c, i == 1 ? 1 : -1, 0))) return rc;
         // treadmill, gravity: inactive; source(): ener - dt*0 leaves every value unchanged
     }
-    e = launch_end_iteration(c->sim, 1, c->st);
+    {
+        ProfScope ps(c, IMH_K_BOOK, 0.0);
+        e = launch_end_iteration(c->sim_ptr, 1, c->st);
+    }
     if (e != hipSuccess) return hip_fail(e, "end iteration");
     return IMH_OK;
-}
-
-static int ensure_uncond(imh_ctx* c) {
-    if (c->par_uncond) return IMH_OK;
-    SimParams p = c->host_par;
-    p.unconditional = 1;
-    hipError_t e = hipMalloc(&c->par_uncond, sizeof(SimParams));
-    if (e == hipSuccess) e = hipMemcpy(c->par_uncond, &p, sizeof(p), hipMemcpyHostToDevice);
-    return e == hipSuccess ? IMH_OK : hip_fail(e, "params");
 }
 
 extern "C" {
@@ -654,9 +824,7 @@ int imh_time_update(imh_ctx* c, int step) {
     if (!c) return fail(IMH_EINVAL, "null context");
     if (!c->have_state) return fail(IMH_ESTATE, "state not uploaded");
     // a direct call behaves as inside an unconditional loop body
-    int rc = ensure_uncond(c);
-    if (rc) return rc;
-    hipError_t e = launch_begin_iteration(c->sim, c->par_uncond, 1, c->st);
+    hipError_t e = launch_begin_iteration(c->sim_ptr, c->par_ptr, 1, 1, c->st);
     if (e != hipSuccess) return hip_fail(e, "begin");
     return imh_time_update_internal(c, step);
 }
@@ -670,9 +838,7 @@ int imh_flux(imh_ctx* c, int order) {
 int imh_iteration(imh_ctx* c) {
     if (!c) return fail(IMH_EINVAL, "null context");
     if (!c->have_state) return fail(IMH_ESTATE, "state not uploaded");
-    int rc = ensure_uncond(c);
-    if (rc) return rc;
-    rc = iteration_body(c, 0);
+    int rc = imh_iteration_body(c, 0);
     if (rc) return rc;
     c->host_iteration += 1;
     return IMH_OK;
@@ -681,8 +847,7 @@ int imh_iteration(imh_ctx* c) {
 int imh_run(imh_ctx* c, int max_iterations, int* done) {
     if (!c) return fail(IMH_EINVAL, "null context");
     if (!c->have_state) return fail(IMH_ESTATE, "state not uploaded");
-    int rc = ensure_uncond(c);
-    if (rc) return rc;
+    int rc;
     SimDev sd;
     hipError_t e = hipStreamSynchronize(c->st);
     if (e == hipSuccess) e = hipMemcpy(&sd, c->sim, sizeof(sd), hipMemcpyDeviceToHost);
@@ -693,18 +858,28 @@ int imh_run(imh_ctx* c, int max_iterations, int* done) {
     for (int it = 0; it < todo; ++it) {
         bool inited = c->fluid_init || !c->cfg.fluid_active;
         if (!inited || c->no_graphs) {
-            if ((rc = iteration_body(c, 1))) return rc;
+            if ((rc = imh_iteration_body(c, 1))) return rc;
         } else {
             int row = ((c->host_iteration - 1) % 6 + 6) % 6;
             if (!c->graph[row]) {
+                if (!c->cap_dev[row]) {
+                    e = hipMalloc(&c->cap_dev[row], ARG_SLOT * CAP_SLOTS);
+                    if (e != hipSuccess) return hip_fail(e, "hipMalloc(graph args)");
+                }
                 hipGraph_t g;
                 e = hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal);
                 if (e != hipSuccess) return hip_fail(e, "begin capture");
-                rc = iteration_body(c, 1);
+                c->capture_row = row;
+                c->cap_count = 0;
+                rc = imh_iteration_body(c, 1);
+                c->capture_row = -1;
                 e = hipStreamEndCapture(c->st, &g);
                 if (rc) return rc;
                 if (e != hipSuccess) return hip_fail(e, "end capture");
-                e = hipGraphInstantiate(&c->graph[row], g, nullptr, nullptr, 0);
+                // the graph's kernels read their argument blocks from cap_dev[row]
+                e = hipMemcpy(c->cap_dev[row], c->cap_host, ARG_SLOT * c->cap_count,
+                              hipMemcpyHostToDevice);
+                if (e == hipSuccess) e = hipGraphInstantiate(&c->graph[row], g, nullptr, nullptr, 0);
                 hipGraphDestroy(g);
                 if (e != hipSuccess) return hip_fail(e, "graph instantiate");
             }
@@ -824,6 +999,26 @@ int imh_set_graphs(imh_ctx* c, int enable) {
     if (!c) return fail(IMH_EINVAL, "null context");
     c->no_graphs = !enable;
     return IMH_OK;
+}
+
+int imh_profile_iterations(imh_ctx* c, int iterations, double* ms, double* bytes, int* count) {
+    if (!c) return fail(IMH_EINVAL, "null context");
+    if (!c->have_state) return fail(IMH_ESTATE, "state not uploaded");
+    SimDev sd;
+    hipError_t e = hipStreamSynchronize(c->st);
+    if (e == hipSuccess) e = hipMemcpy(&sd, c->sim, sizeof(sd), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(e, "time state");
+    c->host_iteration = sd.iteration;
+    Profiler p;
+    c->prof = &p;
+    int rc = IMH_OK;
+    for (int it = 0; it < iterations && rc == IMH_OK; ++it) {
+        rc = imh_iteration_body(c, 1);
+        c->host_iteration += 1;
+    }
+    c->prof = nullptr;
+    int rc2 = prof_collect(&p, ms, bytes, count);
+    return rc ? rc : rc2;
 }
 
 hipStream_t imh_stream(imh_ctx* c) { return c ? c->st : nullptr; }

@@ -4,11 +4,29 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
 #include "imh_common.h"
 #include "imh_kernels.h"
+#include "imogen_hip_ext.h"
 
 namespace imh {
+
+// Kernel timing with HIP events recorded on the launching stream around every launch,
+// with the launch's algorithmic HBM bytes (compulsory reads and writes of the fields).
+struct ProfRec {
+    int kind;
+    double bytes;
+    hipEvent_t a, b;
+};
+struct Profiler {
+    std::vector<ProfRec> recs;
+    bool failed = false;
+};
+void prof_begin(Profiler* p, hipStream_t st, int kind, double bytes);
+void prof_end(Profiler* p, hipStream_t st);
+// Waits for the events, sums them per kind into ms/bytes/count[IMH_NKIND] and frees them.
+int prof_collect(Profiler* p, double* ms, double* bytes, int* count);
 
 struct FluidTile {
     int bs, kmax, PX, TL, K, xmajor;
@@ -49,7 +67,8 @@ struct imh_ctx {
     long long* eoff_dev = nullptr;
     imh::SimDev* sim = nullptr;
     imh::SimParams* par = nullptr;
-    imh::SimParams* par_uncond = nullptr;
+    imh::SimDev** sim_ptr = nullptr;        // device array {sim}
+    imh::SimParams** par_ptr = nullptr;     // device array {par}
     imh::SimParams host_par;
     double* csum_part = nullptr;
     double* csum_out = nullptr;
@@ -58,6 +77,7 @@ struct imh_ctx {
     bool have_state = false;
     bool mhd_kernel = false;
     bool no_graphs = false;
+    bool external_stream = false;   // the stream belongs to a batch
     int mag_is_zero[3] = {1, 1, 1};
     double fade[imh::MAX_FADE] = {};
     double* fade_dev = nullptr;
@@ -65,10 +85,44 @@ struct imh_ctx {
     imh::FluidTile ftile[3];
     imh::MagTile mtile[3];
     hipGraphExec_t graph[6] = {};
+    // Kernel argument blocks live in device memory (the kernels take a pointer, so one
+    // kernel serves a single simulation and a batch).  Eager launches stage them through
+    // a pinned ring copied on the stream; captured launches take a slot of the row's
+    // graph buffer, filled once after the capture.
+    char* arg_host = nullptr;           // pinned, ARG_RING slots
+    char* arg_dev = nullptr;            // device, ARG_RING slots
+    int arg_next = 0;
+    int capture_row = -1;               // >= 0 while a row's graph is being captured
+    int cap_count = 0;
+    char* cap_host = nullptr;           // host image of the captured row's slots
+    char* cap_dev[6] = {};              // device, CAP_SLOTS slots per row
+    imh::Profiler* prof = nullptr;      // set while imh_profile_iterations runs
 };
+
+namespace imh {
+constexpr size_t ARG_SLOT = 512;       // bytes per staged argument block
+constexpr int ARG_RING = 256;
+constexpr int CAP_SLOTS = 96;          // launches in one captured iteration (44 at most)
+}  // namespace imh
+
+// Device copy of an argument block for the next launch on c->st (nullptr on failure,
+// with the error recorded).
+const void* imh_stage_args(imh_ctx* c, const void* a, size_t n);
 
 // internal entry points (force = run regardless of the loop's running flag)
 int imh_fluid_sweep_internal(imh_ctx* c, int X, int force);
 int imh_magnet_sweep_internal(imh_ctx* c, int X, const int I[2], int force);
 int imh_time_update_internal(imh_ctx* c, int step);
 int imh_flux_internal(imh_ctx* c, int order, int force);
+
+// argument blocks of one simulation (imh_api.cpp), shared with the batched path
+template <typename T>
+imh::FluidArgs<T> make_fluid_args(const imh_ctx* c, int d, int force);
+template <typename T>
+imh::MagnetArgs<T> make_magnet_args(const imh_ctx* c, int dX, int dI, int force);
+template <typename T>
+imh::CellMagArgs<T> make_cellmag_args(const imh_ctx* c, int k, int force);
+template <typename T>
+imh::CflArgs<T> make_cfl_args(const imh_ctx* c);
+int imh_iteration_body(imh_ctx* c, int conditional);
+void imh_ctx_free(imh_ctx* c);

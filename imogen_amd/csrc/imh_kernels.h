@@ -47,9 +47,32 @@ struct FluidArgs {
     const double* fade;      // fade_shift.m coefficients (device)
 };
 
+// One entry of a batched launch: argument block `arg` (one simulation and sweep) and the
+// tile (pencil group, or cell chunk) within it.  Every launcher below also takes
+// tiles == nullptr: then args[0] is one simulation and ntiles its tile count.
+struct BatchTile {
+    int arg;
+    int pad;
+    long long tile;
+};
+
 template <typename T>
-hipError_t launch_fluid_sweep(const FluidArgs<T>& a, int bs, int kmax, bool mhd,
-                              hipStream_t st);
+hipError_t launch_fluid_batch(const FluidArgs<T>* args, const BatchTile* tiles, int ntiles,
+                              int nblocks, int bs, int kmax, bool mhd, hipStream_t st);
+
+// Compiled fluid sweep variants X(element size, block size, KMAX): FP64 keeps K <= KMAX
+// cells per thread in registers with KMAX 1..4, FP32 with 2..8.  Pencils up to BS * KMAX
+// cells fit one block.  Each variant is its own translation unit.
+#define IMH_FLUID_VARIANTS(X) \
+    X(8, 256, 1) X(8, 256, 2) X(8, 512, 2) X(8, 512, 4) \
+    X(4, 256, 2) X(4, 256, 4) X(4, 512, 4) X(4, 512, 8)
+#define IMH_FLUID_LAUNCHER_(E, B, K) launch_fluid_v##E##_##B##_##K
+#define IMH_FLUID_LAUNCHER(E, B, K) IMH_FLUID_LAUNCHER_(E, B, K)
+#define IMH_FLUID_DECLARE(E, B, K)                                                         \
+    hipError_t IMH_FLUID_LAUNCHER(E, B, K)(const void* args, const BatchTile* tiles, int ntiles, \
+                                           int nblocks, bool mhd, hipStream_t st);
+IMH_FLUID_VARIANTS(IMH_FLUID_DECLARE)
+bool fluid_variant_compiled(int esz, int bs, int kmax);
 
 // One magnetFlux sub-step (component I advected along X) of one simulation
 // (magnet_sweep.hip).
@@ -78,10 +101,6 @@ struct MagnetArgs {
     const double* fade;
 };
 
-template <typename T>
-hipError_t launch_magnet_advect(const MagnetArgs<T>& a, int bs, hipStream_t st);
-template <typename T>
-hipError_t launch_magnet_ct(const MagnetArgs<T>& a, hipStream_t st);
 
 // updateCellCentered for component k: cmag_k = 0.5*(B_k + shift_k(B_k, +1)).
 template <typename T>
@@ -101,8 +120,16 @@ struct CellMagArgs {
     const double* fade;
 };
 
+
+// Batched magnet kernels: which = 0 advection sub-step, 1 CT step.  kmax from
+// magnet_kmax_for(N, TL); smem = sizeof(T) * 2 * PX * (N + 4) of the largest tile.
 template <typename T>
-hipError_t launch_cell_centered(const CellMagArgs<T>& a, hipStream_t st);
+hipError_t launch_magnet_batch(int which, const MagnetArgs<T>* args, const BatchTile* tiles,
+                               int ntiles, int nblocks, int kmax, size_t smem, hipStream_t st);
+template <typename T>
+hipError_t launch_cell_centered_batch(const CellMagArgs<T>* args, const BatchTile* tiles,
+                                      int ntiles, int nblocks, hipStream_t st);
+int magnet_kmax_for(int N, int TL);
 
 // CFL reduction (TimeManager.update) over one simulation.
 template <typename T>
@@ -111,19 +138,24 @@ struct CflArgs {
     const T* bc[3];
     SimDev* sim;
     long long ncell;
+    long long chunk;         // cells per tile (a multiple of 256)
     MagUse use;
     double gamma;
 };
 
 template <typename T>
-hipError_t launch_cfl(const CflArgs<T>& a, hipStream_t st);
+hipError_t launch_cfl_batch(const CflArgs<T>* args, const BatchTile* tiles, int ntiles,
+                            int nblocks, hipStream_t st);
+long long cfl_chunk();
 
-// Device-side bookkeeping kernels (time_kernels in cfl.hip).
-hipError_t launch_begin_iteration(SimDev* sim, const SimParams* p, int nsims, hipStream_t st);
-hipError_t launch_cfl_reset(SimDev* sim, int nsims, hipStream_t st);
-hipError_t launch_time_finalize(SimDev* sim, const SimParams* p, int nsims, int step,
-                                hipStream_t st);
-hipError_t launch_end_iteration(SimDev* sim, int nsims, hipStream_t st);
+// Device-side bookkeeping kernels (cfl.hip), one thread per simulation; `sim` and `p`
+// are device arrays of per-simulation pointers.
+hipError_t launch_begin_iteration(SimDev* const* sim, const SimParams* const* p, int nsims,
+                                  int unconditional, hipStream_t st);
+hipError_t launch_cfl_reset(SimDev* const* sim, int nsims, hipStream_t st);
+hipError_t launch_time_finalize(SimDev* const* sim, const SimParams* const* p, int nsims,
+                                int step, hipStream_t st);
+hipError_t launch_end_iteration(SimDev* const* sim, int nsims, hipStream_t st);
 
 // Conservation totals (Neumaier-compensated FP64, fixed-order tree).
 template <typename T>

@@ -108,7 +108,7 @@ __device__ inline long long coord_of(long long cell, const int n[3], int d) {
 // Advection sub-step for component I along X (one pencil per PX group of threads).
 // ------------------------------------------------------------------------------------
 template <typename T, int KMAX>
-__global__ void __launch_bounds__(MAG_BS) magnet_advect_kernel(const MagnetArgs<T> a) {
+__device__ __forceinline__ void magnet_advect_body(const MagnetArgs<T>& a, long long tile) {
     if (!a.force && !a.sim->active) return;
     extern __shared__ unsigned char smem_raw[];
     const int dX = a.dX, dI = a.dI;
@@ -121,7 +121,7 @@ __global__ void __launch_bounds__(MAG_BS) magnet_advect_kernel(const MagnetArgs<
     const int tid = threadIdx.x;
     const int px = tid / a.TL;
     const int ty = tid % a.TL;
-    const long long pen = (long long)blockIdx.x * a.PX + px;
+    const long long pen = tile * a.PX + px;
     const bool pen_ok = pen < a.npencil && px < a.PX;
     const int K = (N + a.TL - 1) / a.TL;
     const int i0 = ty * K;
@@ -428,7 +428,7 @@ __global__ void __launch_bounds__(MAG_BS) magnet_advect_kernel(const MagnetArgs<
 // mag(X) = mag(X) - dt/DGRID{I} .* flux   (relaxingMagnet.m:88-92).
 // ------------------------------------------------------------------------------------
 template <typename T, int KMAX>
-__global__ void __launch_bounds__(MAG_BS) magnet_ct_kernel(const MagnetArgs<T> a) {
+__device__ __forceinline__ void magnet_ct_body(const MagnetArgs<T>& a, long long tile) {
     if (!a.force && !a.sim->active) return;
     extern __shared__ unsigned char smem_raw[];
     const int dX = a.dX, dI = a.dI;
@@ -441,7 +441,7 @@ __global__ void __launch_bounds__(MAG_BS) magnet_ct_kernel(const MagnetArgs<T> a
     const int tid = threadIdx.x;
     const int px = tid / a.TL;
     const int ty = tid % a.TL;
-    const long long pen = (long long)blockIdx.x * a.PX + px;
+    const long long pen = tile * a.PX + px;
     const bool pen_ok = pen < a.npencil && px < a.PX;
     const int K = (N + a.TL - 1) / a.TL;
     const int i0 = ty * K;
@@ -521,9 +521,10 @@ __global__ void __launch_bounds__(MAG_BS) magnet_ct_kernel(const MagnetArgs<T> a
 // mag(k) edge tracker on the upper k face.
 // ------------------------------------------------------------------------------------
 template <typename T>
-__global__ void cell_centered_kernel(const CellMagArgs<T> a, long long ncell) {
+__device__ __forceinline__ void cell_centered_body(const CellMagArgs<T>& a, long long chunk) {
     if (!a.force && !a.sim->active) return;
-    long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long ncell = (long long)a.n0 * a.n1 * a.n2;
+    long long cell = chunk * 256 + threadIdx.x;
     if (cell >= ncell) return;
     const int d = a.k;
     const int nn[3] = {a.n0, a.n1, a.n2};
@@ -546,78 +547,108 @@ __global__ void cell_centered_kernel(const CellMagArgs<T> a, long long ncell) {
     a.cb[cell] = T(0.5) * (a.b[cell] + sh);
 }
 
+// Persistent batched forms: blocks walk a tile list spanning many simulations.
+// tiles == nullptr: the single simulation args[0], one block per tile.
+template <typename T, int KMAX>
+__global__ void __launch_bounds__(MAG_BS) magnet_advect_batch_kernel(const MagnetArgs<T>* args,
+                                                                     const BatchTile* tiles,
+                                                                     int ntiles) {
+    if (!tiles) {
+        magnet_advect_body<T, KMAX>(args[0], blockIdx.x);
+        return;
+    }
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const BatchTile bt = tiles[t];
+        magnet_advect_body<T, KMAX>(args[bt.arg], bt.tile);
+        __syncthreads();
+    }
+}
+
+template <typename T, int KMAX>
+__global__ void __launch_bounds__(MAG_BS) magnet_ct_batch_kernel(const MagnetArgs<T>* args,
+                                                                 const BatchTile* tiles,
+                                                                 int ntiles) {
+    if (!tiles) {
+        magnet_ct_body<T, KMAX>(args[0], blockIdx.x);
+        return;
+    }
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const BatchTile bt = tiles[t];
+        magnet_ct_body<T, KMAX>(args[bt.arg], bt.tile);
+        __syncthreads();
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) cell_centered_batch_kernel(const CellMagArgs<T>* args,
+                                                                  const BatchTile* tiles,
+                                                                  int ntiles) {
+    if (!tiles) {
+        cell_centered_body<T>(args[0], blockIdx.x);
+        return;
+    }
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const BatchTile bt = tiles[t];
+        cell_centered_body<T>(args[bt.arg], bt.tile);
+    }
+}
+
 // ------------------------------------------------------------------------------------
 // Launchers
 // ------------------------------------------------------------------------------------
 template <typename T, int KMAX>
-static hipError_t launch_adv_k(const MagnetArgs<T>& a, size_t smem, long long nb,
-                               hipStream_t st) {
-    static bool attr_set = false;  // LDS above 64 KiB needs the opt-in attribute
-    if (smem > 64 * 1024 && !attr_set) {
-        hipError_t e = hipFuncSetAttribute((const void*)magnet_advect_kernel<T, KMAX>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           150 * 1024);
-        if (e != hipSuccess) return e;
-        attr_set = true;
-    }
-    hipLaunchKernelGGL((magnet_advect_kernel<T, KMAX>), dim3(nb), dim3(MAG_BS), smem, st, a);
-    return hipGetLastError();
+static hipError_t set_lds_attr(const void* fn, size_t smem) {
+    if (smem > 64 * 1024)
+        return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024);
+    return hipSuccess;
 }
 
 template <typename T>
-hipError_t launch_magnet_advect(const MagnetArgs<T>& a, int bs, hipStream_t st) {
-    (void)bs;
-    const int N = sel3i(a.dX, a.n0, a.n1, a.n2);
-    if (a.PX < 1 || a.PX > MAG_PXMAX || a.PX * a.TL != MAG_BS) return hipErrorInvalidValue;
-    const int K = (N + a.TL - 1) / a.TL;
-    size_t smem = sizeof(T) * 2 * a.PX * (size_t)(N + 4);
+hipError_t launch_magnet_batch(int which, const MagnetArgs<T>* args, const BatchTile* tiles,
+                               int ntiles, int nblocks, int kmax, size_t smem, hipStream_t st) {
+    if (ntiles == 0) return hipSuccess;
     if (smem > 150 * 1024) return hipErrorInvalidValue;
-    long long nb = (a.npencil + a.PX - 1) / a.PX;
-    if (nb == 0) return hipSuccess;
-    if (K <= 1) return launch_adv_k<T, 1>(a, smem, nb, st);
-    if (K <= 2) return launch_adv_k<T, 2>(a, smem, nb, st);
-    if (K <= 4) return launch_adv_k<T, 4>(a, smem, nb, st);
-    if (K <= 8) return launch_adv_k<T, 8>(a, smem, nb, st);
-    if (K <= 16) return launch_adv_k<T, 16>(a, smem, nb, st);
-    return hipErrorInvalidValue;
-}
-
-template <typename T, int KMAX>
-static hipError_t launch_ct_k(const MagnetArgs<T>& a, size_t smem, long long nb,
-                              hipStream_t st) {
-    hipLaunchKernelGGL((magnet_ct_kernel<T, KMAX>), dim3(nb), dim3(MAG_BS), smem, st, a);
-    return hipGetLastError();
-}
-
-template <typename T>
-hipError_t launch_magnet_ct(const MagnetArgs<T>& a, hipStream_t st) {
-    const int N = sel3i(a.dX, a.n0, a.n1, a.n2);
-    if (a.PX < 1 || a.PX > MAG_PXMAX || a.PX * a.TL != MAG_BS) return hipErrorInvalidValue;
-    const int K = (N + a.TL - 1) / a.TL;
-    size_t smem = sizeof(T) * a.PX * (size_t)(N + 4);
-    long long nb = (a.npencil + a.PX - 1) / a.PX;
-    if (nb == 0) return hipSuccess;
-    if (K <= 1) return launch_ct_k<T, 1>(a, smem, nb, st);
-    if (K <= 2) return launch_ct_k<T, 2>(a, smem, nb, st);
-    if (K <= 4) return launch_ct_k<T, 4>(a, smem, nb, st);
-    if (K <= 8) return launch_ct_k<T, 8>(a, smem, nb, st);
-    if (K <= 16) return launch_ct_k<T, 16>(a, smem, nb, st);
+#define IMH_MB(KM)                                                                          \
+    if (kmax == KM) {                                                                       \
+        if (which == 0) {                                                                   \
+            hipError_t e = set_lds_attr<T, KM>((const void*)magnet_advect_batch_kernel<T, KM>, smem); \
+            if (e != hipSuccess) return e;                                                  \
+            hipLaunchKernelGGL((magnet_advect_batch_kernel<T, KM>), dim3(nblocks), dim3(MAG_BS), \
+                               smem, st, args, tiles, ntiles);                              \
+        } else {                                                                            \
+            hipError_t e = set_lds_attr<T, KM>((const void*)magnet_ct_batch_kernel<T, KM>, smem); \
+            if (e != hipSuccess) return e;                                                  \
+            hipLaunchKernelGGL((magnet_ct_batch_kernel<T, KM>), dim3(nblocks), dim3(MAG_BS),  \
+                               smem, st, args, tiles, ntiles);                              \
+        }                                                                                   \
+        return hipGetLastError();                                                           \
+    }
+    IMH_MB(1) IMH_MB(2) IMH_MB(4) IMH_MB(8) IMH_MB(16)
+#undef IMH_MB
     return hipErrorInvalidValue;
 }
 
 template <typename T>
-hipError_t launch_cell_centered(const CellMagArgs<T>& a, hipStream_t st) {
-    long long ncell = (long long)a.n0 * a.n1 * a.n2;
-    long long nb = (ncell + 255) / 256;
-    hipLaunchKernelGGL((cell_centered_kernel<T>), dim3(nb), dim3(256), 0, st, a, ncell);
+hipError_t launch_cell_centered_batch(const CellMagArgs<T>* args, const BatchTile* tiles,
+                                      int ntiles, int nblocks, hipStream_t st) {
+    if (ntiles == 0) return hipSuccess;
+    hipLaunchKernelGGL((cell_centered_batch_kernel<T>), dim3(nblocks), dim3(256), 0, st, args,
+                       tiles, ntiles);
     return hipGetLastError();
 }
 
-template hipError_t launch_magnet_advect<double>(const MagnetArgs<double>&, int, hipStream_t);
-template hipError_t launch_magnet_advect<float>(const MagnetArgs<float>&, int, hipStream_t);
-template hipError_t launch_magnet_ct<double>(const MagnetArgs<double>&, hipStream_t);
-template hipError_t launch_magnet_ct<float>(const MagnetArgs<float>&, hipStream_t);
-template hipError_t launch_cell_centered<double>(const CellMagArgs<double>&, hipStream_t);
-template hipError_t launch_cell_centered<float>(const CellMagArgs<float>&, hipStream_t);
+int magnet_kmax_for(int N, int TL) {
+    int K = (N + TL - 1) / TL;
+    return K <= 1 ? 1 : K <= 2 ? 2 : K <= 4 ? 4 : K <= 8 ? 8 : K <= 16 ? 16 : -1;
+}
+
+template hipError_t launch_magnet_batch<double>(int, const MagnetArgs<double>*, const BatchTile*,
+                                                int, int, int, size_t, hipStream_t);
+template hipError_t launch_magnet_batch<float>(int, const MagnetArgs<float>*, const BatchTile*,
+                                               int, int, int, size_t, hipStream_t);
+template hipError_t launch_cell_centered_batch<double>(const CellMagArgs<double>*,
+                                                       const BatchTile*, int, int, hipStream_t);
+template hipError_t launch_cell_centered_batch<float>(const CellMagArgs<float>*,
+                                                      const BatchTile*, int, int, hipStream_t);
 
 }  // namespace imh

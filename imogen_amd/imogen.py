@@ -206,6 +206,11 @@ class ImogenManager:
     def set_graphs(self, enable):
         check(lib().imh_set_graphs(self._ctx, int(bool(enable))))
 
+    def profile_iterations(self, n):
+        """Run n iterations eagerly with HIP events around every kernel; returns
+        {kind: (ms, algorithmic bytes, launches)}."""
+        return _lib.run_profile(lib().imh_profile_iterations, self._ctx, n)
+
 
 class _DeviceArray:
     """A reference array object whose data lives on the device (ImogenArray.array)."""

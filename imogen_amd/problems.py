@@ -165,7 +165,10 @@ def mhd_riemann(n=1024, seed=0, gamma=5.0 / 3.0, limiter="vanleer", iter_max=100
             mag[k][sl] = bb[k]
         ener[sl] = p / (gamma - 1.0) + 0.5 * rho * float(v @ v) + 0.5 * float(bb @ bb)
     state = {"mass": mass, "mom": mom, "ener": ener, "mag": mag}
-    ini = _base_ini((n, 1, 1), (1.0 / n, 1.0, 1.0), gamma, 0.35,
+    # The size-1 dims get the x spacing: TimeManager.update takes dt = CFL*MINDGRID(gi)/c
+    # with gi the direction of the largest signal speed, and the transverse velocities
+    # here can be the largest (a unit spacing there would give a 1/dx times too large dt).
+    ini = _base_ini((n, 1, 1), (1.0 / n, 1.0 / n, 1.0 / n), gamma, 0.35,
                     {"x": "const", "y": "circ", "z": "circ"}, limiter, True, iter_max)
     return ini, state
 

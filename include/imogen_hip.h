@@ -162,7 +162,12 @@ int imh_batch_upload_state(imh_batch *b, int sim, const double *mass,
                            const double *const mom[3], const double *ener,
                            const double *const mag[3], size_t n);
 int imh_batch_download_field(imh_batch *b, int sim, int field, double *out, size_t n);
-/* Advance every simulation that is still running by up to `iterations` iterations. */
+/* A batch over existing contexts (same device and precision), which stay owned by the
+ * caller.  The serving loop re-forms its batch this way as requests come and go. */
+int imh_batch_create_view(imh_ctx *const *ctxs, int n, imh_batch **out);
+/* Advance every simulation that is still running by `iterations` batched iterations.  A
+ * simulation that has not run an iteration yet first runs one on its own (its flux
+ * objects create their edge stores there), so it advances iterations + 1. */
 int imh_batch_step(imh_batch *b, int iterations);
 int imh_batch_get_time(imh_batch *b, int sim, double *time, double *dtime, int *iteration);
 int imh_batch_synchronize(imh_batch *b);

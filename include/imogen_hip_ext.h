@@ -1,0 +1,45 @@
+/*
+ * imogen_hip_ext.h -- tuning and measurement entry points of libimogen_hip.so.
+ *
+ * None of these has a counterpart in the reference: they serve the tile autotuner, the
+ * benchmark and the tests (kernel timings with HIP events, tile overrides, graph
+ * replay on/off).  The drop-in boundary is include/imogen_hip.h.
+ */
+#ifndef IMOGEN_HIP_EXT_H
+#define IMOGEN_HIP_EXT_H
+
+#include "imogen_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Kernel kinds reported by the profiling calls. */
+#define IMH_K_CFL 0       /* CFL reduction (TimeManager.update) */
+#define IMH_K_FLUID 1     /* fused relaxingFluid sweep */
+#define IMH_K_MAG_ADV 2   /* relaxingMagnet advection sub-step */
+#define IMH_K_MAG_CT 3    /* relaxingMagnet constrained-transport step */
+#define IMH_K_CELLMAG 4   /* MagnetArray.updateCellCentered */
+#define IMH_K_BOOK 5      /* per-iteration time bookkeeping */
+#define IMH_NKIND 6
+
+/* Pin the fluid sweep tile along dim X (1..3): block size, registers per thread (KMAX)
+ * and pencils per block; px = 0 restores the automatic choice.  Drops captured graphs. */
+int imh_set_fluid_tile(imh_ctx *ctx, int X, int bs, int kmax, int px);
+int imh_get_fluid_tile(imh_ctx *ctx, int X, int *bs, int *kmax, int *px);
+/* enable = 0 runs imh_run eagerly (one launch at a time) instead of replaying graphs. */
+int imh_set_graphs(imh_ctx *ctx, int enable);
+
+/* Run `iterations` loop bodies eagerly with HIP events around every kernel and return,
+ * per kernel kind, the summed time (ms), the summed algorithmic HBM bytes and the number
+ * of launches.  The simulation advances exactly as under imh_run. */
+int imh_profile_iterations(imh_ctx *ctx, int iterations, double ms[IMH_NKIND],
+                           double bytes[IMH_NKIND], int count[IMH_NKIND]);
+/* The same for `iterations` batched iterations of a batch. */
+int imh_batch_profile_step(imh_batch *b, int iterations, double ms[IMH_NKIND],
+                           double bytes[IMH_NKIND], int count[IMH_NKIND]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

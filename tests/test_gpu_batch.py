@@ -1,0 +1,90 @@
+"""The batched ensemble path: a simulation advanced inside a batch of uneven simulations
+gives the same result as the same simulation on its own, and the FP32-bulk path stays
+within the stated error of the FP64 oracle.
+
+Tolerances:
+  * FP64 batch vs FP64 alone: <= 1e-12 relative L-inf per field (the same device bodies
+    and argument blocks run in both, so differences are expected to be exactly 0);
+  * FP32 bulk vs the FP64 oracle: L1 density error <= 1e-4 after 40 iterations.
+"""
+import numpy as np
+import pytest
+
+from imogen_amd import problems
+from imogen_amd.ensemble import Ensemble
+
+from .helpers import FLUID_FIELDS, MAG_FIELDS, make_gpu, make_oracle, rel_l1, rel_linf
+
+pytestmark = pytest.mark.gpu
+
+
+def mixed_sims():
+    """Uneven grids, dimensions, boundary conditions and limiters in one batch."""
+    sims = [problems.mhd_riemann(n, seed=s) for s, n in enumerate([256, 700, 1500, 4096])]
+    sims += [problems.kelvin_helmholtz(n, seed=10 + n) for n in (64, 96)]
+    sims.append(problems.orszag_tang(48))
+    sims.append(problems.sod(512, limiter="minmond"))
+    sims.append(problems.sedov(16, time_max=1.0))
+    ini, st = problems.sod(200, ny=8, bc="trans", limiter="superbee")
+    sims.append((ini, st))
+    return sims
+
+
+def fields_of(ini):
+    return FLUID_FIELDS + (MAG_FIELDS if ini["mode"]["magnet"] else [])
+
+
+@pytest.mark.parametrize("iters", [1, 13])
+def test_batch_fp64_matches_alone(iters):
+    sims = mixed_sims()
+    ens = Ensemble(sims, "fp64")
+    ens.step(iters)  # the first iteration runs alone, then `iters` batched ones
+    worst = 0.0
+    for i, (ini, st) in enumerate(sims):
+        run = make_gpu(ini, st, "fp64")
+        run.run_iterations(iters + 1)
+        t_b, dt_b, it_b = ens.time(i)
+        assert it_b == run.time.iteration, (i, it_b, run.time.iteration)
+        assert abs(t_b - run.time.time) <= 1e-12 * abs(run.time.time), i
+        for f in fields_of(ini):
+            e = rel_linf(ens.download(i, f), run.download(f))
+            assert e <= 1e-12, (i, f, e)
+            worst = max(worst, e)
+        run.close()
+    print("batch vs alone worst relative L-inf: %.3e" % worst, flush=True)
+    ens.close()
+
+
+def test_batch_fp32_error_vs_oracle():
+    sims = [problems.mhd_riemann(n, seed=100 + n) for n in (512, 2048)]
+    sims += [problems.kelvin_helmholtz(128, seed=5), problems.orszag_tang(64)]
+    iters = 40
+    ens = Ensemble(sims, "fp32")
+    ens.step(iters - 1)
+    for i, (ini, st) in enumerate(sims):
+        orc = make_oracle(ini, st)
+        for _ in range(iters):
+            orc.iteration()
+        assert ens.time(i)[2] == orc.iteration_count
+        e1 = rel_l1(ens.download(i, "mass"), orc.get("mass"))
+        einf = rel_linf(ens.download(i, "mass"), orc.get("mass"))
+        print("sim %d %s: density L1 %.2e L-inf %.2e" % (i, ini["grid"], e1, einf), flush=True)
+        assert e1 <= 1e-4, (i, e1)
+    ens.close()
+
+
+def test_batch_handles_finished_simulations():
+    """A simulation whose TIMEMAX is reached stops advancing while the others go on."""
+    a = problems.sod(256, time_max=0.01)
+    b = problems.sod(256)
+    ens = Ensemble([a, b], "fp64")
+    ens.step(60)
+    t_a, _, it_a = ens.time(0)
+    t_b, _, it_b = ens.time(1)
+    assert t_a == pytest.approx(0.01, rel=0, abs=1e-15)
+    assert it_b == 62 and it_a < it_b
+    run = make_gpu(*a, "fp64")
+    run.run_iterations(1000)
+    assert run.time.iteration == it_a
+    np.testing.assert_array_equal(ens.download(0, "mass"), run.download("mass"))
+    ens.close()
