@@ -79,6 +79,7 @@ SIGNATURES = {
     "imh_synchronize": (_I, [_P]),
     "imh_conservation_sums": (_I, [_P, _D]),
     "imh_memory_bytes": (_I, [_P, ctypes.POINTER(ctypes.c_size_t)]),
+    "imh_memory_estimate": (_I, [ctypes.POINTER(ImhConfig), ctypes.POINTER(ctypes.c_size_t)]),
     "imh_batch_create": (_I, [ctypes.POINTER(ImhConfig), _I, ctypes.POINTER(_P)]),
     "imh_batch_destroy": (_I, [_P]),
     "imh_batch_upload_state": (_I, [_P, _I, _D, _D * 3, _D, _D * 3, ctypes.c_size_t]),
@@ -88,6 +89,17 @@ SIGNATURES = {
     "imh_batch_get_time": (_I, [_P, _I, _D, _D, ctypes.POINTER(_I)]),
     "imh_batch_synchronize": (_I, [_P]),
     "imh_batch_memory_bytes": (_I, [_P, ctypes.POINTER(ctypes.c_size_t)]),
+    "imh_serve_create": (_I, [_I, _I, ctypes.c_size_t, _I, ctypes.POINTER(_P)]),
+    "imh_serve_destroy": (_I, [_P]),
+    "imh_serve_submit": (_I, [_P, ctypes.POINTER(ImhConfig), _D, _D * 3, _D, _D * 3,
+                              ctypes.c_size_t, ctypes.POINTER(ctypes.c_longlong)]),
+    "imh_serve_step": (_I, [_P, _I, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+    "imh_serve_status": (_I, [_P, ctypes.c_longlong, ctypes.POINTER(_I), _D,
+                              ctypes.POINTER(_I), _D]),
+    "imh_serve_download": (_I, [_P, ctypes.c_longlong, _I, _D, ctypes.c_size_t]),
+    "imh_serve_release": (_I, [_P, ctypes.c_longlong]),
+    "imh_serve_stats": (_I, [_P, ctypes.POINTER(ctypes.c_size_t),
+                             ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_longlong)]),
 }
 
 # Tuning and measurement entry points (include/imogen_hip_ext.h; not part of the drop-in

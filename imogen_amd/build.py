@@ -24,7 +24,8 @@ COMMON = [
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
 
-SOURCES = ["magnet_sweep.hip", "cfl.hip", "fluid_dispatch.cpp", "imh_api.cpp", "batch.cpp"]
+SOURCES = ["magnet_sweep.hip", "cfl.hip", "fluid_dispatch.cpp", "imh_api.cpp", "batch.cpp",
+           "serve.cpp"]
 VARIANT_SRC = "fluid_sweep_variant.hip"
 
 

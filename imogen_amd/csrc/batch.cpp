@@ -613,3 +613,5 @@ imh_ctx* imh_batch_context(imh_batch* b, int sim) {
 }
 
 }  // extern "C"
+
+hipStream_t imh_batch_stream(imh_batch* b) { return b ? b->st : nullptr; }

@@ -274,15 +274,18 @@ const void* imh_stage_args(imh_ctx* c, const void* a, size_t n) {
     return c->arg_dev + k * ARG_SLOT;
 }
 
-// Offsets of the edge stores (objects x sides x dims) for trans/wall sides.
-static void layout_edges(imh_ctx* c) {
+// Offsets of the edge stores (objects x sides x dims) for trans/wall sides; returns the
+// number of doubles they need.
+static long long edges_layout(const imh_config& cfg, long long eoff[NOBJ][2][3]) {
+    const int ndims = cfg.grid[2] > 1 ? 3 : 2;
+    const long long ncell = (long long)cfg.grid[0] * cfg.grid[1] * cfg.grid[2];
     long long off = 0;
     for (int o = 0; o < NOBJ; ++o)
         for (int s = 0; s < 2; ++s)
-            for (int d = 0; d < 3; ++d) c->eoff[o][s][d] = 0;
+            for (int d = 0; d < 3; ++d) eoff[o][s][d] = 0;
     auto exists = [&](int o) {
         if (o < OBJ_MAG + 3) return true;
-        if (!c->cfg.magnet_active) return false;
+        if (!cfg.magnet_active) return false;
         int k = (o - OBJ_VG) % 9;
         return (k / 3) != (k % 3);  // (I, X) pairs with I != X
     };
@@ -290,14 +293,27 @@ static void layout_edges(imh_ctx* c) {
         if (o >= OBJ_MFLUX + 9) break;
         if (!exists(o)) continue;
         for (int s = 0; s < 2; ++s)
-            for (int d = 0; d < c->ndims; ++d) {
-                int m = c->cfg.bc[s][d];
+            for (int d = 0; d < ndims; ++d) {
+                int m = cfg.bc[s][d];
                 if (m != IMH_BC_TRANS && m != IMH_BC_WALL) continue;
-                c->eoff[o][s][d] = off;
-                off += c->nface[d];
+                eoff[o][s][d] = off;
+                off += ncell / cfg.grid[d];
             }
     }
-    c->edges_len = off > 0 ? off : 1;
+    return off > 0 ? off : 1;
+}
+
+static void layout_edges(imh_ctx* c) { c->edges_len = (size_t)edges_layout(c->cfg, c->eoff); }
+
+// Device bytes of a context (fields, CT scratch, edge stores, small bookkeeping).
+static size_t ctx_bytes(const imh_config& cfg, size_t edges_len) {
+    const size_t es = cfg.precision == IMH_FP64 ? 8 : 4;
+    const size_t ncell = (size_t)cfg.grid[0] * cfg.grid[1] * cfg.grid[2];
+    size_t b = ncell * es * (IMH_NFIELDS + (cfg.magnet_active ? 1 : 0));
+    b += edges_len * sizeof(double) + sizeof(long long) * NOBJ * 6 + 2 * sizeof(SimDev);
+    b += ARG_SLOT * ARG_RING + 6 * ARG_SLOT * CAP_SLOTS + sizeof(double) * (2 * 1024 * 8 + 8) +
+         sizeof(double) * MAX_FADE;
+    return b;
 }
 
 extern "C" {
@@ -964,9 +980,16 @@ int imh_conservation_sums(imh_ctx* c, double out[8]) {
 
 int imh_memory_bytes(imh_ctx* c, size_t* bytes) {
     if (!c || !bytes) return fail(IMH_EINVAL, "null argument");
-    size_t b = (size_t)c->ncell * esize(c) * (IMH_NFIELDS + (c->scratch ? 1 : 0));
-    b += c->edges_len * sizeof(double) + sizeof(c->eoff) + 2 * sizeof(SimDev);
-    *bytes = b;
+    *bytes = ctx_bytes(c->cfg, c->edges_len);
+    return IMH_OK;
+}
+
+int imh_memory_estimate(const imh_config* cfg, size_t* bytes) {
+    if (!cfg || !bytes) return fail(IMH_EINVAL, "null argument");
+    int rc = validate_config(cfg);
+    if (rc) return rc;
+    long long eoff[NOBJ][2][3];
+    *bytes = ctx_bytes(*cfg, (size_t)edges_layout(*cfg, eoff));
     return IMH_OK;
 }
 

@@ -150,6 +150,8 @@ int imh_conservation_sums(imh_ctx *ctx, double out[8]);
 
 /* Device bytes held by the context. */
 int imh_memory_bytes(imh_ctx *ctx, size_t *bytes);
+/* Device bytes a context with this configuration would hold (validates cfg first). */
+int imh_memory_estimate(const imh_config *cfg, size_t *bytes);
 
 /* ---- batched ensemble (many independent simulations, one launch per sweep phase) -- */
 typedef struct imh_batch imh_batch;
@@ -172,6 +174,43 @@ int imh_batch_step(imh_batch *b, int iterations);
 int imh_batch_get_time(imh_batch *b, int sim, double *time, double *dtime, int *iteration);
 int imh_batch_synchronize(imh_batch *b);
 int imh_batch_memory_bytes(imh_batch *b, size_t *bytes);
+
+/* ---- serving loop (continuous batching of solve requests on one device) ------------ */
+/* A request is one simulation run while TimeManager.running holds (to its TIMEMAX or
+ * ITERMAX).  Requests queue in FIFO order and are admitted while fewer than max_active
+ * run and their device memory (imh_memory_estimate) fits the HBM budget; admitted
+ * requests stay resident and advance together with the running ones; finished ones leave
+ * the batch at the end of the round and keep their result on the device until
+ * imh_serve_release.  No counterpart in the reference (its main loop runs one simulation). */
+typedef struct imh_server imh_server;
+
+#define IMH_REQ_QUEUED 0
+#define IMH_REQ_RUNNING 1
+#define IMH_REQ_DONE 2
+#define IMH_REQ_FAILED 3
+
+/* hbm_budget 0: 90% of the device memory free at creation (also the upper limit). */
+int imh_serve_create(int device, int precision, size_t hbm_budget, int max_active,
+                     imh_server **out);
+int imh_serve_destroy(imh_server *s);
+/* Copies the initial state (host FP64, n cells per field); cfg.precision and cfg.device
+ * are taken from the server. */
+int imh_serve_submit(imh_server *s, const imh_config *cfg, const double *mass,
+                     const double *const mom[3], const double *ener,
+                     const double *const mag[3], size_t n, long long *id);
+/* One scheduling round: admit, advance the running batch by `iterations` iterations,
+ * retire the finished requests and admit into the freed slots. */
+int imh_serve_step(imh_server *s, int iterations, int *active, int *queued);
+/* Status and host wall-clock seconds (since imh_serve_create) of submission, admission
+ * and completion (-1 while not reached); time and iteration as of the last round. */
+int imh_serve_status(imh_server *s, long long id, int *status, double *time, int *iteration,
+                     double t_events[3]);
+int imh_serve_download(imh_server *s, long long id, int field, double *out, size_t n);
+int imh_serve_release(imh_server *s, long long id);
+/* Device bytes held by admitted and finished requests, the budget, and the cell-updates
+ * (2 x cells per iteration) of every finished request. */
+int imh_serve_stats(imh_server *s, size_t *used_bytes, size_t *budget_bytes,
+                    long long *cell_updates);
 
 #ifdef __cplusplus
 }
