@@ -135,14 +135,17 @@ def plain_leg(sims, args):
 
 
 def cpu_leg(sims, args):
-    """The CPU oracle, single thread, on a bounded sample: the ensemble's simulations in
-    order (Riemann problems and every 8th KH box) until cpu_seconds of work is done, each
-    advanced warmup + 1 + steps iterations like the GPU legs."""
+    """The CPU oracle, single thread, on a bounded sample: the ensemble's Riemann problems
+    and its KH boxes of side <= 192, interleaved 8:1 in ensemble order, until cpu_seconds
+    of work is done, each advanced warmup + 1 + steps iterations like the GPU legs."""
     sys.path.insert(0, os.path.join(HERE, "tests"))
     from helpers import make_oracle
     n_iter = args.warmup + 1 + args.steps
-    order = [i for i in range(len(sims)) if sims[i][0]["grid"][1] == 1][:64]
-    order += [i for i in range(len(sims)) if sims[i][0]["grid"][1] > 1][::8]
+    rie = [i for i in range(len(sims)) if sims[i][0]["grid"][1] == 1]
+    kh = [i for i in range(len(sims)) if 1 < sims[i][0]["grid"][1] <= 192]
+    order = []
+    for k in range(max(len(rie) // 8, len(kh))):
+        order += rie[8 * k:8 * k + 8] + kh[k:k + 1]
     done, t_used, cu = [], 0.0, 0.0
     for i in order:
         ini, st = sims[i]

@@ -24,9 +24,10 @@ COMMON = [
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
 
-SOURCES = ["magnet_sweep.hip", "cfl.hip", "fluid_dispatch.cpp", "imh_api.cpp", "batch.cpp",
+SOURCES = ["magnet_dispatch.hip", "cfl.hip", "fluid_dispatch.cpp", "imh_api.cpp", "batch.cpp",
            "serve.cpp"]
 VARIANT_SRC = "fluid_sweep_variant.hip"
+MAG_VARIANT_SRC = "magnet_sweep_variant.hip"
 
 
 def fluid_variants():
@@ -37,12 +38,22 @@ def fluid_variants():
     return [tuple(int(v) for v in t) for t in re.findall(r"X\((\d+), (\d+), (\d+)\)", m.group(1))]
 
 
+def magnet_variants():
+    """(element size, KMAX) of every magnet variant (IMH_MAG_VARIANTS in imh_kernels.h)."""
+    text = open(os.path.join(CSRC, "imh_kernels.h")).read()
+    m = re.search(r"#define IMH_MAG_VARIANTS\(X\)((?:[^\n]*\\\n)*[^\n]*)", text)
+    return [tuple(int(v) for v in t) for t in re.findall(r"X\((\d+), (\d+)\)", m.group(1))]
+
+
 def units():
     """(source, object name, extra flags) of every translation unit."""
     out = [(s, os.path.splitext(s)[0], []) for s in SOURCES]
     for e, b, k in fluid_variants():
         out.append((VARIANT_SRC, "fluid_sweep_v%d_%d_%d" % (e, b, k),
                     ["-DIMH_FLUID_ESZ=%d" % e, "-DIMH_FLUID_BS=%d" % b, "-DIMH_FLUID_KMAX=%d" % k]))
+    for e, k in magnet_variants():
+        out.append((MAG_VARIANT_SRC, "magnet_sweep_v%d_%d" % (e, k),
+                    ["-DIMH_MAG_ESZ=%d" % e, "-DIMH_MAG_KMAX=%d" % k]))
     return out
 
 
@@ -74,7 +85,7 @@ def build(verbose=False, jobs=None, incremental=False):
     os.makedirs(BUILD, exist_ok=True)
     us = units()
     # the long fluid variants first, so they overlap with the short units
-    us.sort(key=lambda u: 0 if u[0] == VARIANT_SRC else 1)
+    us.sort(key=lambda u: 0 if u[0] in (VARIANT_SRC, MAG_VARIANT_SRC) else 1)
     jobs = jobs or min(len(us), max(1, (os.cpu_count() or 2)))
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         results = list(ex.map(lambda u: _compile(u, verbose, incremental), us))

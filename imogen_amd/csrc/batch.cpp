@@ -590,6 +590,16 @@ int imh_batch_get_time(imh_batch* b, int sim, double* time, double* dtime, int* 
     return IMH_OK;
 }
 
+int imh_batch_set_fluid_tile(imh_batch* b, int sim, int X, int bs, int kmax, int px) {
+    if (!b || sim < 0 || sim >= b->n) return fail(IMH_EINVAL, "bad simulation index");
+    hipError_t e = hipStreamSynchronize(b->st);
+    if (e != hipSuccess) return hip_fail(e, "batch sync");
+    int rc = imh_set_fluid_tile(b->ctx[sim], X, bs, kmax, px);
+    if (rc) return rc;
+    free_plans(b);  // argument blocks and plans are rebuilt before the next step
+    return IMH_OK;
+}
+
 int imh_batch_synchronize(imh_batch* b) {
     if (!b) return fail(IMH_EINVAL, "null batch");
     hipError_t e = hipStreamSynchronize(b->st);

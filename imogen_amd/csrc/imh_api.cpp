@@ -134,6 +134,18 @@ FluidTile choose_fluid_tile(int N, int d, int esz, const TileOverride* ov) {
             if (tl < 8) tl = 8;
         }
         t.PX = t.bs / (tl < t.bs ? tl : t.bs);
+        // pencils longer than BS * KMAX of the preferred variant: one pencil per 512-thread
+        // block with the smallest compiled KMAX that holds it
+        const int need = (N + 511) / 512;
+        if (need > t.kmax) {
+            for (int k = need; k <= 64; ++k)
+                if (fluid_variant_exists(esz, 512, k)) {
+                    t.bs = 512;
+                    t.kmax = k;
+                    t.PX = 1;
+                    break;
+                }
+        }
     }
     if (t.PX > 32) t.PX = 32;
     if (t.PX < 1) t.PX = 1;

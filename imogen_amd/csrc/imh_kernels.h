@@ -22,7 +22,7 @@ __host__ __device__ inline int sel3i(int i, int a, int b, int c) {
     return i == 0 ? a : (i == 1 ? b : c);
 }
 
-// One fluid sweep of one simulation (fluid_sweep.hip).
+// One fluid sweep of one simulation (fluid_sweep_impl.h).
 template <typename T>
 struct FluidArgs {
     T* q[5];                 // mass, mom(1..3), ener -- updated in place
@@ -61,10 +61,11 @@ hipError_t launch_fluid_batch(const FluidArgs<T>* args, const BatchTile* tiles, 
                               int nblocks, int bs, int kmax, bool mhd, hipStream_t st);
 
 // Compiled fluid sweep variants X(element size, block size, KMAX): FP64 keeps K <= KMAX
-// cells per thread in registers with KMAX 1..4, FP32 with 2..8.  Pencils up to BS * KMAX
-// cells fit one block.  Each variant is its own translation unit.
+// cells per thread in registers with KMAX 1..4 (8 for pencils of 2049..4096 cells, with
+// register spills), FP32 with 2..8.  Pencils up to BS * KMAX cells fit one block.  Each
+// variant is its own translation unit.
 #define IMH_FLUID_VARIANTS(X) \
-    X(8, 256, 1) X(8, 256, 2) X(8, 512, 2) X(8, 512, 4) \
+    X(8, 256, 1) X(8, 256, 2) X(8, 512, 2) X(8, 512, 4) X(8, 512, 8) \
     X(4, 256, 2) X(4, 256, 4) X(4, 512, 4) X(4, 512, 8)
 #define IMH_FLUID_LAUNCHER_(E, B, K) launch_fluid_v##E##_##B##_##K
 #define IMH_FLUID_LAUNCHER(E, B, K) IMH_FLUID_LAUNCHER_(E, B, K)
@@ -75,7 +76,7 @@ IMH_FLUID_VARIANTS(IMH_FLUID_DECLARE)
 bool fluid_variant_compiled(int esz, int bs, int kmax);
 
 // One magnetFlux sub-step (component I advected along X) of one simulation
-// (magnet_sweep.hip).
+// (magnet_sweep_impl.h).
 template <typename T>
 struct MagnetArgs {
     const T* rho;            // mass
@@ -130,6 +131,17 @@ template <typename T>
 hipError_t launch_cell_centered_batch(const CellMagArgs<T>* args, const BatchTile* tiles,
                                       int ntiles, int nblocks, hipStream_t st);
 int magnet_kmax_for(int N, int TL);
+
+// Compiled magnet variants X(element size, KMAX): each a translation unit with the
+// advection and CT kernels for pencils of up to KMAX * TL cells.
+#define IMH_MAG_VARIANTS(X) \
+    X(8, 1) X(8, 2) X(8, 4) X(8, 8) X(8, 16) X(4, 1) X(4, 2) X(4, 4) X(4, 8) X(4, 16)
+#define IMH_MAG_LAUNCHER_(E, K) launch_magnet_v##E##_##K
+#define IMH_MAG_LAUNCHER(E, K) IMH_MAG_LAUNCHER_(E, K)
+#define IMH_MAG_DECLARE(E, K)                                                               \
+    hipError_t IMH_MAG_LAUNCHER(E, K)(int which, const void* args, const BatchTile* tiles,  \
+                                      int ntiles, int nblocks, size_t smem, hipStream_t st);
+IMH_MAG_VARIANTS(IMH_MAG_DECLARE)
 
 // CFL reduction (TimeManager.update) over one simulation.
 template <typename T>

@@ -1,4 +1,4 @@
-// magnet_sweep.hip -- constrained-transport magnetic sweep (magnet/magnetFlux.m,
+// magnet_sweep_impl.h -- constrained-transport magnetic sweep (magnet/magnetFlux.m,
 // magnet/relaxingMagnet.m, MagnetArray.updateCellCentered).
 //
 // One magnetFlux(X, [I1 I2]) call is, per component I:
@@ -14,7 +14,7 @@
 // one exchange of the whole pencil (write, barrier, ghost cells from the BC rule with the
 // object's edge tracker, barrier, read).  The magnet sweeps are a smaller share of the
 // MHD step than the fluid sweep, so this kernel favours generality (all BC modes, +-2
-// shifts, fade windows) over the register-chunk layout of fluid_sweep.hip.
+// shifts, fade windows) over the register-chunk layout of the fluid sweep.
 #include "imh_common.h"
 #include "imh_kernels.h"
 
@@ -594,61 +594,37 @@ __global__ void __launch_bounds__(256) cell_centered_batch_kernel(const CellMagA
 }
 
 // ------------------------------------------------------------------------------------
-// Launchers
+// Launchers.  magnet_sweep_variant.hip is compiled once per entry of IMH_MAG_VARIANTS
+// (imh_kernels.h) with IMH_MAG_ESZ / IMH_MAG_KMAX defined and holds that variant's
+// advection and CT kernels; magnet_dispatch.hip holds the cell-centred kernels and picks
+// the variant of a launch.
 // ------------------------------------------------------------------------------------
-template <typename T, int KMAX>
-static hipError_t set_lds_attr(const void* fn, size_t smem) {
-    if (smem > 64 * 1024)
-        return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024);
-    return hipSuccess;
-}
+#ifdef IMH_MAG_KMAX
+#if IMH_MAG_ESZ == 8
+using MagT = double;
+#else
+using MagT = float;
+#endif
 
-template <typename T>
-hipError_t launch_magnet_batch(int which, const MagnetArgs<T>* args, const BatchTile* tiles,
-                               int ntiles, int nblocks, int kmax, size_t smem, hipStream_t st) {
-    if (ntiles == 0) return hipSuccess;
-    if (smem > 150 * 1024) return hipErrorInvalidValue;
-#define IMH_MB(KM)                                                                          \
-    if (kmax == KM) {                                                                       \
-        if (which == 0) {                                                                   \
-            hipError_t e = set_lds_attr<T, KM>((const void*)magnet_advect_batch_kernel<T, KM>, smem); \
-            if (e != hipSuccess) return e;                                                  \
-            hipLaunchKernelGGL((magnet_advect_batch_kernel<T, KM>), dim3(nblocks), dim3(MAG_BS), \
-                               smem, st, args, tiles, ntiles);                              \
-        } else {                                                                            \
-            hipError_t e = set_lds_attr<T, KM>((const void*)magnet_ct_batch_kernel<T, KM>, smem); \
-            if (e != hipSuccess) return e;                                                  \
-            hipLaunchKernelGGL((magnet_ct_batch_kernel<T, KM>), dim3(nblocks), dim3(MAG_BS),  \
-                               smem, st, args, tiles, ntiles);                              \
-        }                                                                                   \
-        return hipGetLastError();                                                           \
+hipError_t IMH_MAG_LAUNCHER(IMH_MAG_ESZ, IMH_MAG_KMAX)(int which, const void* args,
+                                                      const BatchTile* tiles, int ntiles,
+                                                      int nblocks, size_t smem, hipStream_t st) {
+    const MagnetArgs<MagT>* a = (const MagnetArgs<MagT>*)args;
+    const void* fn = which == 0 ? (const void*)magnet_advect_batch_kernel<MagT, IMH_MAG_KMAX>
+                                : (const void*)magnet_ct_batch_kernel<MagT, IMH_MAG_KMAX>;
+    if (smem > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           150 * 1024);
+        if (e != hipSuccess) return e;
     }
-    IMH_MB(1) IMH_MB(2) IMH_MB(4) IMH_MB(8) IMH_MB(16)
-#undef IMH_MB
-    return hipErrorInvalidValue;
-}
-
-template <typename T>
-hipError_t launch_cell_centered_batch(const CellMagArgs<T>* args, const BatchTile* tiles,
-                                      int ntiles, int nblocks, hipStream_t st) {
-    if (ntiles == 0) return hipSuccess;
-    hipLaunchKernelGGL((cell_centered_batch_kernel<T>), dim3(nblocks), dim3(256), 0, st, args,
-                       tiles, ntiles);
+    if (which == 0)
+        hipLaunchKernelGGL((magnet_advect_batch_kernel<MagT, IMH_MAG_KMAX>), dim3(nblocks),
+                           dim3(MAG_BS), smem, st, a, tiles, ntiles);
+    else
+        hipLaunchKernelGGL((magnet_ct_batch_kernel<MagT, IMH_MAG_KMAX>), dim3(nblocks),
+                           dim3(MAG_BS), smem, st, a, tiles, ntiles);
     return hipGetLastError();
 }
-
-int magnet_kmax_for(int N, int TL) {
-    int K = (N + TL - 1) / TL;
-    return K <= 1 ? 1 : K <= 2 ? 2 : K <= 4 ? 4 : K <= 8 ? 8 : K <= 16 ? 16 : -1;
-}
-
-template hipError_t launch_magnet_batch<double>(int, const MagnetArgs<double>*, const BatchTile*,
-                                                int, int, int, size_t, hipStream_t);
-template hipError_t launch_magnet_batch<float>(int, const MagnetArgs<float>*, const BatchTile*,
-                                               int, int, int, size_t, hipStream_t);
-template hipError_t launch_cell_centered_batch<double>(const CellMagArgs<double>*,
-                                                       const BatchTile*, int, int, hipStream_t);
-template hipError_t launch_cell_centered_batch<float>(const CellMagArgs<float>*,
-                                                      const BatchTile*, int, int, hipStream_t);
+#endif
 
 }  // namespace imh

@@ -75,6 +75,11 @@ class Ensemble:
         {kind: (ms, algorithmic bytes, launches)}."""
         return _lib.run_profile(lib().imh_batch_profile_step, self._b, iterations)
 
+    def set_fluid_tile(self, i, X, bs, kmax, px):
+        """Pin the fluid sweep tile of simulation i along X (1..3)."""
+        check(lib().imh_batch_set_fluid_tile(self._b, int(i), int(X), int(bs), int(kmax),
+                                             int(px)))
+
     def synchronize(self):
         check(lib().imh_batch_synchronize(self._b))
 

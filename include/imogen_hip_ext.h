@@ -27,6 +27,8 @@ extern "C" {
  * and pencils per block; px = 0 restores the automatic choice.  Drops captured graphs. */
 int imh_set_fluid_tile(imh_ctx *ctx, int X, int bs, int kmax, int px);
 int imh_get_fluid_tile(imh_ctx *ctx, int X, int *bs, int *kmax, int *px);
+/* The same for simulation `sim` of a batch (its launch plans are rebuilt). */
+int imh_batch_set_fluid_tile(imh_batch *b, int sim, int X, int bs, int kmax, int px);
 /* enable = 0 runs imh_run eagerly (one launch at a time) instead of replaying graphs. */
 int imh_set_graphs(imh_ctx *ctx, int enable);
 

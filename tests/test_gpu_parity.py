@@ -17,15 +17,35 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-12
 
 
+def where(run, orc, fields):
+    """The field and cell (z, y, x) of the largest relative difference (for messages)."""
+    worst = (-1.0, None, None)
+    for f in fields:
+        a, b = run.download(f), orc.get(f)
+        d = np.abs(a - b)
+        i = np.unravel_index(int(np.argmax(d)), d.shape)
+        e = rel_linf(a, b)
+        if e > worst[0]:
+            worst = (e, f, (i, float(a[i]), float(b[i])))
+    return worst
+
+
 def lockstep(ini, state, iterations, fields):
+    from imogen_amd import iteration
     orc = make_oracle(ini, state)
     run = make_gpu(ini, state)
-    worst = compare(run, orc, fields, TOL)  # constructor side effects (cellMag)
+    try:
+        worst = compare(run, orc, fields, TOL)  # constructor side effects (cellMag)
+    except AssertionError as e:
+        raise AssertionError("after upload: %s; worst %r" % (e, where(run, orc, fields)))
     for it in range(iterations):
         orc.iteration()
-        from imogen_amd import iteration
         iteration(run)
-        w = compare(run, orc, fields, TOL)
+        try:
+            w = compare(run, orc, fields, TOL)
+        except AssertionError as e:
+            raise AssertionError("iteration %d (dt gpu %.17g oracle %.17g): %s; worst %r" % (
+                it + 1, run.time.dTime, orc.dtime, e, where(run, orc, fields)))
         worst = max(worst, w)
         t_gpu, t_orc = run.time.time, orc.time
         assert abs(t_gpu - t_orc) <= TOL * abs(t_orc), (it, t_gpu, t_orc)
@@ -75,11 +95,45 @@ BC_CASES = [
 def test_bc_modes_2d_mhd(bcx, bcy):
     grid = (24, 22, 1)
     st = problems.random_state(grid, seed=11, magnet=True)
+    # dz small too: the random state has v_z != 0, and TimeManager.update takes dt from the
+    # spacing of the direction with the largest speed, z included
+    ini = {"grid": grid, "dGrid": (1.0 / 24, 1.0 / 22, 1.0 / 24), "gamma": 5.0 / 3.0,
+           "cfl": 0.35, "bcMode": {"x": bcx, "y": bcy, "z": "circ"}, "bcInfinity": 20,
+           "fluxLimiter": {"x": "vanleer", "y": "superbee"}, "minMass": 1e-5,
+           "mode": {"fluid": True, "magnet": True}, "iterMax": 100, "timeMax": 1e9}
+    lockstep(ini, st, 12, FLUID_FIELDS + MAG_FIELDS)
+
+
+@pytest.mark.parametrize("bcx,bcy", BC_CASES)
+def test_bc_modes_2d_mhd_sweep_by_sweep(bcx, bcy):
+    """The calls of two iterations one at a time (host-set dTime), compared after each."""
+    import imogen_amd
+    grid = (24, 22, 1)
+    st = problems.random_state(grid, seed=11, magnet=True)
     ini = {"grid": grid, "dGrid": (1.0 / 24, 1.0 / 22, 1.0), "gamma": 5.0 / 3.0, "cfl": 0.35,
            "bcMode": {"x": bcx, "y": bcy, "z": "circ"}, "bcInfinity": 20,
            "fluxLimiter": {"x": "vanleer", "y": "superbee"}, "minMass": 1e-5,
            "mode": {"fluid": True, "magnet": True}, "iterMax": 100, "timeMax": 1e9}
-    lockstep(ini, st, 3, FLUID_FIELDS + MAG_FIELDS)
+    orc = make_oracle(ini, st)
+    run = make_gpu(ini, st)
+    mass, mom, ener, mag = run._arrays
+    fields = FLUID_FIELDS + MAG_FIELDS
+    calls = [("fluid", 2, None), ("magnet", 2, [1, 3]), ("fluid", 1, None), ("magnet", 1, [2, 3]),
+             ("magnet", 1, [3, 2]), ("fluid", 1, None), ("magnet", 2, [3, 1]), ("fluid", 2, None)]
+    for k, (kind, X, I2) in enumerate(calls * 2):
+        orc.dtime = 2e-3
+        run.time.dTime = 2e-3
+        if kind == "fluid":
+            orc.relaxing_fluid(X)
+            imogen_amd.relaxingFluid(run, mass, mom, ener, mag, None, X)
+        else:
+            orc.magnet_flux(X, I2)
+            imogen_amd.magnetFlux(run, mass, mom, mag, X, I2)
+        try:
+            compare(run, orc, fields, TOL)
+        except AssertionError as e:
+            raise AssertionError("call %d %s X=%d I=%s: %s; worst %r" % (
+                k, kind, X, I2, e, where(run, orc, fields)))
 
 
 @pytest.mark.parametrize("bc", ["trans", "mirror", "const"])
