@@ -50,6 +50,9 @@ def parse():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle leg")
     ap.add_argument("--no-large", action="store_true", help="skip the 256^3 plain roofline")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="also run one device copy of this many MiB (the reference kernel "
+                         "for scaling HBM counters)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--profile-steps", type=int, default=6)
     return ap.parse_args()
@@ -230,6 +233,15 @@ def main():
 
     if not args.no_large:
         out["plain_roofline"] = large_plain_roofline(args)
+
+    if args.copy_probe:
+        import torch
+        n = args.copy_probe * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        out["copy_probe_bytes"] = {"read": 4 * n, "write": 4 * n}
 
     if not args.no_cpu:
         crate, done, t_cpu, n_iter = cpu_leg(sims, args)

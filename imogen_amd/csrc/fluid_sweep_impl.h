@@ -46,6 +46,60 @@ __device__ inline long long xcd_tile(long long b, long long nblocks) {
     return x * q + (x < r ? x : r) + t;
 }
 
+// Ghost value of one shift call by one cell at a pencil end (the BC rule of that flux
+// object, with its tracker).  Out of line: only pencil-end threads need it, and keeping
+// the BC code out of every call site keeps the kernel inside the instruction cache.
+__device__ __noinline__ double fluid_ghost(int mode, int upper, int flip_sign, int zero_ghost,
+                                           int active_lin, int bc_inf, const double* ends, int N,
+                                           double* tracker) {
+    GhostSpec s;
+    s.mode = mode;
+    s.upper = upper;
+    s.flip_sign = flip_sign;
+    s.zero_ghost = zero_ghost;
+    s.active_linear = active_lin;
+    s.bc_infinity = bc_inf;
+    const Ends e = {ends[0], ends[1], ends[2], ends[3], ends[4], ends[5]};
+    double g0, g1;
+    ghost_values(s, e, N, 1, tracker, g0, g1);
+    return g0;
+}
+
+// Fade blend of a constant-shifted value (fade_shift.m:31-45) for a cell inside one of
+// the fade windows; out of line for the same reason.
+__device__ __noinline__ double fluid_fade(double r, int lower, int gi, int N, const double* ends,
+                                          const double* c, int f) {
+    const Ends e = {ends[0], ends[1], ends[2], ends[3], ends[4], ends[5]};
+    // constant-shifted end values: lower shift r[0]=a[0], r[N-1]=a[N-2];
+    // upper shift r[0]=a[1], r[N-1]=a[N-1]
+    const double v1 = lower ? e.a0 : end_at(e, 1, N);
+    const double vN = lower ? end_at(e, N - 2, N) : e.b2;
+    return fade_blend(r, gi, N, v1, vN, c, f);
+}
+
+// First assignment of a flux object (InitializedArray.m:43-47): cell gi of pencil fi
+// along d records its value in the object's edge stores of the other dimensions when it
+// lies on one of their faces.  Runs in a simulation's first sweep only.
+__device__ __noinline__ void fluid_init_face(double* edges, const long long* eoff, int modes,
+                                             int ndims, int d, int obj, int gi, long long fi,
+                                             int n0, int n1, int n2, double val) {
+    const int nn[3] = {n0, n1, n2};
+    int c0, c1, c2;
+    if (d == 0) { c0 = gi; c1 = int(fi % n1); c2 = int(fi / n1); }
+    else if (d == 1) { c0 = int(fi % n0); c1 = gi; c2 = int(fi / n0); }
+    else { c0 = int(fi % n0); c1 = int(fi / n0); c2 = gi; }
+    for (int dd = 0; dd < ndims; ++dd) {
+        if (dd == d) continue;
+        for (int s = 0; s < 2; ++s) {
+            int mode = mode_at(modes, s, dd);
+            if (mode != IMH_BC_TRANS && mode != IMH_BC_WALL) continue;
+            const int cdd = sel3i(dd, c0, c1, c2);
+            if (cdd != (s ? sel3i(dd, n0, n1, n2) - 1 : 0)) continue;  // off this face
+            edges[eoff[(obj * 2 + s) * 3 + dd] + face_index(nn, dd, c0, c1, c2)] = val;
+        }
+    }
+}
+
 template <typename T>
 __device__ inline T pencil_reduce_max(T v, const FluidArgs<T>& a, int tid, int px, T* s_red) {
     const int lane = tid & 63;
@@ -165,38 +219,20 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& a, long lon
     // ghost of one shift call by one cell, from the staged pencil ends of quantity
     // `which` in buffer `buf`; slot = tracker slot in s_trk
     auto ghost1 = [&](int upper, int v, int ft, int buf, int which, int slot) __attribute__((always_inline)) -> T {
-        const double* p = s_ends[buf][which][px];
-        const Ends e = {p[0], p[1], p[2], p[3], p[4], p[5]};
-        double g0, g1;
-        ghost_values(spec(upper, v, ft), e, N, 1, &s_trk[px][slot], g0, g1);
-        return T(g0);
+        const GhostSpec sp = spec(upper, v, ft);
+        return T(fluid_ghost(sp.mode, sp.upper, sp.flip_sign, sp.zero_ghost, sp.active_linear,
+                             sp.bc_infinity, s_ends[buf][which][px], N, &s_trk[px][slot]));
     };
 
     // First assignment of a flux object (InitializedArray.m:43-47): record its edges on
     // the faces of the other dimensions.
     auto init_faces = [&](int obj, const T (&vals)[KMAX]) __attribute__((always_inline)) {
-        if (!a.init_edges || kc == 0) return;
-        const long long fi = pen;
-        for (int dd = 0; dd < a.ndims; ++dd) {
-            if (dd == d) continue;
-            for (int s = 0; s < 2; ++s) {
-                int mode = mode_at(a.modes, s, dd);
-                if (mode != IMH_BC_TRANS && mode != IMH_BC_WALL) continue;
-                long long off = a.eoff[(obj * 2 + s) * 3 + dd];
+        if (!a.init_edges || kc == 0 || a.ndims < 2) return;
 #pragma unroll
-                for (int k = 0; k < KMAX; ++k) {
-                    if (k >= kc) continue;
-                    int gi = i0 + k;
-                    int c0, c1, c2;
-                    if (d == 0) { c0 = gi; c1 = int(fi % a.n1); c2 = int(fi / a.n1); }
-                    else if (d == 1) { c0 = int(fi % a.n0); c1 = gi; c2 = int(fi / a.n0); }
-                    else { c0 = int(fi % a.n0); c1 = int(fi / a.n0); c2 = gi; }
-                    const int cdd = sel3i(dd, c0, c1, c2);
-                    if (cdd != (s ? sel3i(dd, a.n0, a.n1, a.n2) - 1 : 0)) continue;  // off this face
-                    a.edges[off + face_index(nn, dd, c0, c1, c2)] = double(vals[k]);
-                }
-            }
-        }
+        for (int k = 0; k < KMAX; ++k)
+            if (k < kc)
+                fluid_init_face(a.edges, a.eoff, a.modes, a.ndims, d, obj, i0 + k, pen, a.n0,
+                                a.n1, a.n2, double(vals[k]));
     };
 
     // Stage the chunk boundary values and the pencil ends of two quantities.
@@ -243,15 +279,8 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& a, long lon
         T r;
         if (lower) r = (k > 0) ? A[k > 0 ? k - 1 : 0] : (has_left ? nb : ghost);
         else r = (k < kc - 1) ? A[k + 1 < KMAX ? k + 1 : k] : (has_right ? nb : ghost);
-        if (mode == IMH_BC_FADE && (gi < f || gi >= N - f)) {
-            // constant-shifted end values: lower shift r[0]=a[0], r[N-1]=a[N-2];
-            // upper shift r[0]=a[1], r[N-1]=a[N-1]
-            const double* p = s_ends[buf][which][px];
-            const Ends e = {p[0], p[1], p[2], p[3], p[4], p[5]};
-            T v1 = T(lower ? e.a0 : end_at(e, 1, N));
-            T vN = T(lower ? end_at(e, N - 2, N) : e.b2);
-            r = fade_blend(r, gi, N, v1, vN, a.fade, f);
-        }
+        if (mode == IMH_BC_FADE && (gi < f || gi >= N - f))
+            r = T(fluid_fade(double(r), lower, gi, N, s_ends[buf][which][px], a.fade, f));
         return r;
     };
 
@@ -467,15 +496,19 @@ template <typename T, int BS, int KMAX, bool MHD>
 __global__ void __launch_bounds__(BS) fluid_sweep_batch_kernel(const FluidArgs<T>* args,
                                                                const BatchTile* tiles,
                                                                int ntiles) {
-    if (!tiles) {
-        const long long t = args[0].xcd_remap ? xcd_tile(blockIdx.x, gridDim.x)
-                                               : (long long)blockIdx.x;
-        fluid_sweep_body<T, BS, KMAX, MHD>(args[0], t);
-        return;
-    }
+    // one call site of the body (it is large)
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const BatchTile bt = tiles[t];
-        fluid_sweep_body<T, BS, KMAX, MHD>(args[bt.arg], bt.tile);
+        int arg = 0;
+        long long tile;
+        if (tiles) {
+            const BatchTile bt = tiles[t];
+            arg = bt.arg;
+            tile = bt.tile;
+        } else {
+            tile = args[0].xcd_remap ? xcd_tile(blockIdx.x, gridDim.x) : (long long)blockIdx.x;
+        }
+        fluid_sweep_body<T, BS, KMAX, MHD>(args[arg], tile);
+        if (!tiles) break;
         __syncthreads();
     }
 }

@@ -44,24 +44,83 @@ __device__ inline Ends lds_ends(const T* row, int N) {
     return e;
 }
 
+// The BC code below runs only at pencil ends (or fade windows) and is kept out of line
+// so the kernels stay small enough for the instruction cache.
+
+// Ghost values of one shift call of an LDS pencil (m = 1 or 2 cells), written to out[].
+template <typename T>
+__device__ __noinline__ void mag_ghost_lds(int mode, int upper, int flip_sign, int zero_ghost,
+                                           int active_lin, int bc_inf, const T* row, int N,
+                                           int m, double* tracker, double* out) {
+    GhostSpec s;
+    s.mode = mode;
+    s.upper = upper;
+    s.flip_sign = flip_sign;
+    s.zero_ghost = zero_ghost;
+    s.active_linear = active_lin;
+    s.bc_infinity = bc_inf;
+    const Ends e = lds_ends(row, N);
+    double g0, g1;
+    ghost_values(s, e, N, m, tracker, g0, g1);
+    out[0] = g0;
+    if (m > 1) out[1] = g1;
+}
+
+template <typename T>
+__device__ __noinline__ T mag_fade_lds(T r, const T* row, int N, int i, int m, const double* fade,
+                                       int f) {
+    int up = m > 0;
+    int am = up ? m : -m;
+    T v1 = up ? row[2 + (am < N - 1 ? am : N - 1)] : row[2];
+    int jn = N - 1 - am;
+    T vN = up ? row[2 + N - 1] : row[2 + (jn > 0 ? jn : 0)];
+    return fade_blend(r, i, N, v1, vN, fade, f);
+}
+
 // shift(D, m) (m = +-1, +-2) of an LDS pencil at cell i, with ghost values g (the m
 // values ghost_values() returned for this call) and the fade window when mode is fade.
 template <typename T>
 __device__ inline T lds_shift(const T* row, int N, int i, int m, const double* g, int mode,
                               const double* fade, int f) {
     int up = m > 0;
-    int am = up ? m : -m;
     int idx = i + m;
     T r;
     if (idx >= 0 && idx <= N - 1) r = row[2 + idx];
     else r = T(up ? (idx - N == 0 ? g[0] : g[1]) : (i == 0 ? g[0] : g[1]));
-    if (mode == IMH_BC_FADE) {
-        T v1 = up ? row[2 + (am < N - 1 ? am : N - 1)] : row[2];
-        int jn = N - 1 - am;
-        T vN = up ? row[2 + N - 1] : row[2 + (jn > 0 ? jn : 0)];
-        r = fade_blend(r, i, N, v1, vN, fade, f);
-    }
+    if (mode == IMH_BC_FADE) r = mag_fade_lds(r, row, N, i, m, fade, f);
     return r;
+}
+
+// Ghost of a one-cell shift along dI of global array `arr` for the cell on the face of
+// the line starting at row0.
+template <typename T>
+__device__ __noinline__ double transverse_ghost(const T* arr, long long row0, int NI, long long sI,
+                                                int mode, int upper, int flip_sign,
+                                                int zero_ghost, int active_lin, int bc_inf,
+                                                double* tracker) {
+    GhostSpec s;
+    s.mode = mode;
+    s.upper = upper;
+    s.flip_sign = flip_sign;
+    s.zero_ghost = zero_ghost;
+    s.active_linear = active_lin;
+    s.bc_infinity = bc_inf;
+    auto lo = [&](int k) __attribute__((always_inline)) { return double(arr[row0 + (long long)(k < NI - 1 ? k : NI - 1) * sI]); };
+    auto hi = [&](int k) __attribute__((always_inline)) { int j = NI - 3 + k; return double(arr[row0 + (long long)(j > 0 ? j : 0) * sI]); };
+    Ends e = {lo(0), lo(1), lo(2), hi(0), hi(1), hi(2)};
+    double g0, g1;
+    ghost_values(s, e, NI, 1, tracker, g0, g1);
+    return g0;
+}
+
+template <typename T>
+__device__ __noinline__ T transverse_fade(T r, const T* arr, long long row0, int jI, int NI,
+                                          long long sI, int upper, const double* fade, int f) {
+    int i1 = 1 < NI - 1 ? 1 : NI - 1;
+    int jn = NI - 2 > 0 ? NI - 2 : 0;
+    T v1 = upper ? arr[row0 + i1 * sI] : arr[row0];
+    T vN = upper ? arr[row0 + (long long)(NI - 1) * sI] : arr[row0 + jn * sI];
+    return fade_blend(r, jI, NI, v1, vN, fade, f);
 }
 
 // Transverse shift along dimension dI of global array `arr` at the cell whose
@@ -72,26 +131,14 @@ __device__ inline T transverse_shift(const T* arr, long long cell, int jI, int N
                                      long long sI, bool enabled, const GhostSpec& s,
                                      double* tracker, const double* fade, int f) {
     if (!enabled) return arr[cell];
-    const long long row0 = cell - (long long)jI * sI;
     T r;
     bool ghost = s.upper ? (jI >= NI - 1) : (jI <= 0);
-    if (!ghost) {
-        r = arr[cell + (s.upper ? sI : -sI)];
-    } else {
-        auto lo = [&](int k) __attribute__((always_inline)) { return double(arr[row0 + (long long)(k < NI - 1 ? k : NI - 1) * sI]); };
-        auto hi = [&](int k) __attribute__((always_inline)) { int j = NI - 3 + k; return double(arr[row0 + (long long)(j > 0 ? j : 0) * sI]); };
-        Ends e = {lo(0), lo(1), lo(2), hi(0), hi(1), hi(2)};
-        double g0, g1;
-        ghost_values(s, e, NI, 1, tracker, g0, g1);
-        r = T(g0);
-    }
-    if (s.mode == IMH_BC_FADE) {
-        int i1 = 1 < NI - 1 ? 1 : NI - 1;
-        int jn = NI - 2 > 0 ? NI - 2 : 0;
-        T v1 = s.upper ? arr[row0 + i1 * sI] : arr[row0];
-        T vN = s.upper ? arr[row0 + (long long)(NI - 1) * sI] : arr[row0 + jn * sI];
-        r = fade_blend(r, jI, NI, v1, vN, fade, f);
-    }
+    if (!ghost) r = arr[cell + (s.upper ? sI : -sI)];
+    else r = T(transverse_ghost(arr, cell - (long long)jI * sI, NI, sI, s.mode, s.upper,
+                                s.flip_sign, s.zero_ghost, s.active_linear, s.bc_infinity,
+                                tracker));
+    if (s.mode == IMH_BC_FADE)
+        r = transverse_fade(r, arr, cell - (long long)jI * sI, jI, NI, sI, s.upper, fade, f);
     return r;
 }
 
@@ -100,6 +147,26 @@ __device__ inline long long coord_of(long long cell, const int n[3], int d) {
     const long long c1 = (cell / n[0]) % n[1];
     const long long c2 = cell / ((long long)n[0] * n[1]);
     return d == 0 ? c0 : (d == 1 ? c1 : c2);
+}
+
+// First assignment of an object (InitializedArray.m:43-47): `cell` records `val` in the
+// object's edge stores of the dims other than dX whose face it lies on.
+__device__ __noinline__ void mag_init_face(double* edges, const long long* eoff, int modes,
+                                           int ndims, int dX, int obj, long long cell, int n0,
+                                           int n1, int n2, double val) {
+    const int nn[3] = {n0, n1, n2};
+    const int c0 = int(coord_of(cell, nn, 0));
+    const int c1 = int(coord_of(cell, nn, 1));
+    const int c2 = int(coord_of(cell, nn, 2));
+    for (int dd = 0; dd < ndims; ++dd) {
+        if (dd == dX) continue;
+        for (int s = 0; s < 2; ++s) {
+            int mode = mode_at(modes, s, dd);
+            if (mode != IMH_BC_TRANS && mode != IMH_BC_WALL) continue;
+            if (sel3i(dd, c0, c1, c2) != (s ? sel3i(dd, n0, n1, n2) - 1 : 0)) continue;
+            edges[eoff[(obj * 2 + s) * 3 + dd] + face_index(nn, dd, c0, c1, c2)] = val;
+        }
+    }
 }
 
 }  // namespace
@@ -168,36 +235,21 @@ __device__ __forceinline__ void magnet_advect_body(const MagnetArgs<T>& a, long 
             if (!a.init_wmag) ewm_hi = a.edges[eidx(oWM, 1)];
         }
     }
-    auto spec = [&](int upper, int comp_is_dir, int zero_ghost) __attribute__((always_inline)) {
-        GhostSpec s;
-        s.mode = upper ? mhi : mlo;
-        s.upper = upper;
-        s.flip_sign = comp_is_dir;
-        s.zero_ghost = zero_ghost;
-        s.active_linear = a.active_linX;
-        s.bc_infinity = f;
-        return s;
+    // ghosts of one shift call along X (component I != X: no sign flips) into out[]
+    auto gh = [&](int upper, int zero_ghost, const T* row, int m, double* trk, double* out)
+        __attribute__((always_inline)) {
+        mag_ghost_lds<T>(upper ? mhi : mlo, upper, 0, zero_ghost, a.active_linX, f, row, N, m,
+                         trk, out);
     };
     // record the edges of a freshly assigned object on all faces (InitializedArray.m:43-47)
     auto init_faces = [&](int obj, const T (&vals)[KMAX]) __attribute__((always_inline)) {
-        for (int dd = 0; dd < a.ndims; ++dd) {
-            if (dd == dX) continue;
-            for (int s = 0; s < 2; ++s) {
-                int mode = mode_at(a.modes, s, dd);
-                if (mode != IMH_BC_TRANS && mode != IMH_BC_WALL) continue;
-                long long off = a.eoff[(obj * 2 + s) * 3 + dd];
+        if (a.ndims < 2) return;
 #pragma unroll
-                for (int k = 0; k < KMAX; ++k) {
-                    if (k >= kc) continue;
-                    long long cell = base + (long long)(i0 + k) * stride;
-                    const int c0 = int(coord_of(cell, nn, 0));
-                    const int c1 = int(coord_of(cell, nn, 1));
-                    const int c2 = int(coord_of(cell, nn, 2));
-                    if (sel3i(dd, c0, c1, c2) != (s ? sel3i(dd, a.n0, a.n1, a.n2) - 1 : 0)) continue;
-                    a.edges[off + face_index(nn, dd, c0, c1, c2)] = double(vals[k]);
-                }
-            }
-        }
+        for (int k = 0; k < KMAX; ++k)
+            if (k < kc)
+                mag_init_face(a.edges, a.eoff, a.modes, a.ndims, dX, obj,
+                              base + (long long)(i0 + k) * stride, a.n0, a.n1, a.n2,
+                              double(vals[k]));
     };
     auto put = [&](T* row, const T (&v)[KMAX]) __attribute__((always_inline)) {
 #pragma unroll
@@ -251,13 +303,9 @@ __device__ __forceinline__ void magnet_advect_body(const MagnetArgs<T>& a, long 
     put(row0, vg);
     __syncthreads();
     if (ghost_thread) {
-        double g0, g1;
-        const Ends e = lds_ends(row0, N);
-        if (a.init_vg) { evg_lo = e.a0; evg_hi = e.b2; }
-        ghost_values(spec(0, 0, 0), e, N, 1, &evg_lo, g0, g1);
-        s_g[px][0] = g0;
-        ghost_values(spec(1, 0, 0), e, N, 1, &evg_hi, g0, g1);
-        s_g[px][1] = g0;
+        if (a.init_vg) { evg_lo = double(row0[2]); evg_hi = double(row0[2 + N - 1]); }
+        gh(0, 0, row0, 1, &evg_lo, &s_g[px][0]);
+        gh(1, 0, row0, 1, &evg_hi, &s_g[px][1]);
     }
     __syncthreads();
 #pragma unroll
@@ -274,12 +322,7 @@ __device__ __forceinline__ void magnet_advect_body(const MagnetArgs<T>& a, long 
     T vf[KMAX];
     put(row1, vg);
     __syncthreads();
-    if (ghost_thread) {
-        double g0, g1;
-        const Ends e = lds_ends(row1, N);
-        ghost_values(spec(1, 0, 0), e, N, 1, &evg_hi, g0, g1);
-        s_g[px][2] = g0;
-    }
+    if (ghost_thread) gh(1, 0, row1, 1, &evg_hi, &s_g[px][2]);
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < KMAX; ++k) {
@@ -299,11 +342,8 @@ __device__ __forceinline__ void magnet_advect_body(const MagnetArgs<T>& a, long 
     put(row0, w);
     __syncthreads();
     if (ghost_thread) {
-        double g0, g1;
-        const Ends e = lds_ends(row0, N);
-        if (a.init_store) { esfr_lo = e.a0; esfr_hi = e.b2; }
-        ghost_values(spec(1, 0, 0), e, N, 1, &esfr_hi, g0, g1);  // FLUXR keeps the upper side
-        s_g[px][3] = g0;
+        if (a.init_store) { esfr_lo = double(row0[2]); esfr_hi = double(row0[2 + N - 1]); }
+        gh(1, 0, row0, 1, &esfr_hi, &s_g[px][3]);  // FLUXR keeps the upper side
     }
     __syncthreads();
 #pragma unroll
@@ -315,12 +355,7 @@ __device__ __forceinline__ void magnet_advect_body(const MagnetArgs<T>& a, long 
     }
     put(row1, w);
     __syncthreads();
-    if (ghost_thread) {
-        double g0, g1;
-        const Ends e = lds_ends(row1, N);
-        ghost_values(spec(0, 0, 1), e, N, 1, &esfr_lo, g0, g1);  // FLUXR: zero-mode lower ghost 0
-        s_g[px][4] = g0;
-    }
+    if (ghost_thread) gh(0, 1, row1, 1, &esfr_lo, &s_g[px][4]);  // FLUXR: zero-mode lower ghost 0
     __syncthreads();
     T st[KMAX];
 #pragma unroll
@@ -340,21 +375,13 @@ __device__ __forceinline__ void magnet_advect_body(const MagnetArgs<T>& a, long 
     put(row0, w);
     __syncthreads();
     if (ghost_thread) {
-        double g0, g1;
-        const Ends e = lds_ends(row0, N);
-        if (a.init_wmag) { ewm_lo = e.a0; ewm_hi = e.b2; }
+        if (a.init_wmag) { ewm_lo = double(row0[2]); ewm_hi = double(row0[2 + N - 1]); }
         // wMag is an InitializedArray (not a FluxArray): zero mode keeps constant ghosts
-        ghost_values(spec(1, 0, 0), e, N, 1, &ewm_hi, g0, g1);      // flux = ... shift(X,1)
-        s_g[px][0] = g0;
-        ghost_values(spec(1, 0, 0), e, N, 1, &ewm_hi, g0, g1);      // dFluxR: shift(X,1)
-        s_g[px][1] = g0;
-        ghost_values(spec(1, 0, 0), e, N, 2, &ewm_hi, g0, g1);      // dFluxR: shift(X,2)
-        s_g[px][2] = g0;
-        s_g[px][3] = g1;
-        ghost_values(spec(1, 0, 0), e, N, 1, &ewm_hi, g0, g1);      // dFluxL: shift(X,1)
-        s_g[px][4] = g0;
-        ghost_values(spec(0, 0, 0), e, N, 1, &ewm_lo, g0, g1);      // dFluxL: shift(X,-1)
-        s_g[px][5] = g0;
+        gh(1, 0, row0, 1, &ewm_hi, &s_g[px][0]);  // flux = ... shift(X,1)
+        gh(1, 0, row0, 1, &ewm_hi, &s_g[px][1]);  // dFluxR: shift(X,1)
+        gh(1, 0, row0, 2, &ewm_hi, &s_g[px][2]);  // dFluxR: shift(X,2) (slots 2, 3)
+        gh(1, 0, row0, 1, &ewm_hi, &s_g[px][4]);  // dFluxL: shift(X,1)
+        gh(0, 0, row0, 1, &ewm_lo, &s_g[px][5]);  // dFluxL: shift(X,-1)
     }
     __syncthreads();
     T F[KMAX];
@@ -391,12 +418,7 @@ __device__ __forceinline__ void magnet_advect_body(const MagnetArgs<T>& a, long 
     // ---- B_I = B_I - 2*fluxFactor .* (flux - flux.shift(X,-1)) -------------------------
     put(row1, F);
     __syncthreads();
-    if (ghost_thread) {
-        double g0, g1;
-        const Ends e = lds_ends(row1, N);
-        ghost_values(spec(0, 0, 1), e, N, 1, &efl_lo, g0, g1);  // 'flux' type: zero ghost 0
-        s_g[px][6] = g0;
-    }
+    if (ghost_thread) gh(0, 1, row1, 1, &efl_lo, &s_g[px][6]);  // 'flux' type: zero ghost 0
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < KMAX; ++k) {
@@ -494,16 +516,12 @@ __device__ __forceinline__ void magnet_ct_body(const MagnetArgs<T>& a, long long
         if (k < kc) row[2 + i0 + k] = G[k];
     __syncthreads();
     if (ghost_thread) {
-        double g0, g1;
-        const Ends e = lds_ends(row, N);
         double tr = 0;
         bool trk = (mlo == IMH_BC_TRANS || mlo == IMH_BC_WALL);
         long long ei = a.eoff[(oFL * 2 + 0) * 3 + dX] + pen;
         if (trk) tr = a.edges[ei];
-        GhostSpec s = {mlo, 0, 0, 1, a.active_linX, f};
-        ghost_values(s, e, N, 1, &tr, g0, g1);
+        mag_ghost_lds<T>(mlo, 0, 0, 1, a.active_linX, f, row, N, 1, &tr, &s_g[px][0]);
         if (trk) a.edges[ei] = tr;
-        s_g[px][0] = g0;
     }
     __syncthreads();
 #pragma unroll
@@ -549,17 +567,33 @@ __device__ __forceinline__ void cell_centered_body(const CellMagArgs<T>& a, long
 
 // Persistent batched forms: blocks walk a tile list spanning many simulations.
 // tiles == nullptr: the single simulation args[0], one block per tile.
+// The (argument block, tile) a block works on in round t of a persistent launch; with
+// tiles == nullptr the launch is one simulation, args[0], one block per tile.
+__device__ inline bool next_tile(const BatchTile* tiles, int ntiles, int t, int& arg,
+                                 long long& tile) {
+    if (t >= ntiles) return false;
+    if (!tiles) {
+        if (t != (int)blockIdx.x) return false;
+        arg = 0;
+        tile = t;
+        return true;
+    }
+    const BatchTile bt = tiles[t];
+    arg = bt.arg;
+    tile = bt.tile;
+    return true;
+}
+
+// Persistent batched forms: blocks walk a tile list spanning many simulations (one call
+// site of each body: they are large).
 template <typename T, int KMAX>
 __global__ void __launch_bounds__(MAG_BS) magnet_advect_batch_kernel(const MagnetArgs<T>* args,
                                                                      const BatchTile* tiles,
                                                                      int ntiles) {
-    if (!tiles) {
-        magnet_advect_body<T, KMAX>(args[0], blockIdx.x);
-        return;
-    }
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const BatchTile bt = tiles[t];
-        magnet_advect_body<T, KMAX>(args[bt.arg], bt.tile);
+    int arg;
+    long long tile;
+    for (int t = blockIdx.x; next_tile(tiles, ntiles, t, arg, tile); t += gridDim.x) {
+        magnet_advect_body<T, KMAX>(args[arg], tile);
         __syncthreads();
     }
 }
@@ -568,13 +602,10 @@ template <typename T, int KMAX>
 __global__ void __launch_bounds__(MAG_BS) magnet_ct_batch_kernel(const MagnetArgs<T>* args,
                                                                  const BatchTile* tiles,
                                                                  int ntiles) {
-    if (!tiles) {
-        magnet_ct_body<T, KMAX>(args[0], blockIdx.x);
-        return;
-    }
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const BatchTile bt = tiles[t];
-        magnet_ct_body<T, KMAX>(args[bt.arg], bt.tile);
+    int arg;
+    long long tile;
+    for (int t = blockIdx.x; next_tile(tiles, ntiles, t, arg, tile); t += gridDim.x) {
+        magnet_ct_body<T, KMAX>(args[arg], tile);
         __syncthreads();
     }
 }
@@ -583,14 +614,10 @@ template <typename T>
 __global__ void __launch_bounds__(256) cell_centered_batch_kernel(const CellMagArgs<T>* args,
                                                                   const BatchTile* tiles,
                                                                   int ntiles) {
-    if (!tiles) {
-        cell_centered_body<T>(args[0], blockIdx.x);
-        return;
-    }
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const BatchTile bt = tiles[t];
-        cell_centered_body<T>(args[bt.arg], bt.tile);
-    }
+    int arg;
+    long long tile;
+    for (int t = blockIdx.x; next_tile(tiles, ntiles, t, arg, tile); t += gridDim.x)
+        cell_centered_body<T>(args[arg], tile);
 }
 
 // ------------------------------------------------------------------------------------
