@@ -104,11 +104,17 @@ def roofline(prof, label):
     }
 
 
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
 def batched_leg(sims, args, cells):
     from imogen_amd.ensemble import Ensemble
     ens = Ensemble(sims, "fp32")
+    log("batched: uploaded")
     ens.step(args.warmup)          # first iteration of each simulation + graph capture
     ens.synchronize()
+    log("batched: warm")
     t0 = time.perf_counter()
     ens.step(args.steps)
     ens.synchronize()
@@ -226,24 +232,27 @@ def main():
         print("plain fp64: %.1f Mcell-updates/s; max L1 %.2e" % (prate, max(l1)),
               file=sys.stderr, flush=True)
 
+    log("profiling the batched step")
     prof = ens.profile_step(args.profile_steps)
     out["roofline"] = roofline(prof, "batched ensemble FP32, persistent launches")
     out["roofline"]["bytes_rule"] = BYTES_NOTE
     ens.close()
 
     if not args.no_large:
+        log("plain fp64 sedov 256^3")
         out["plain_roofline"] = large_plain_roofline(args)
 
     if args.copy_probe:
-        import torch
-        n = args.copy_probe * (1 << 20) // 4
-        src = torch.ones(n, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        out["copy_probe_bytes"] = {"read": 4 * n, "write": 4 * n}
+        import ctypes
+        nbytes = args.copy_probe << 20
+        ms = ctypes.c_double()
+        L.check(L.lib().imh_copy_probe(nbytes, 10, ctypes.byref(ms)))
+        out["copy_probe"] = {"bytes_read": nbytes, "bytes_written": nbytes,
+                             "ms": round(ms.value, 4),
+                             "GB/s": round(2.0 * nbytes / (ms.value * 1e-3) / 1e9, 1)}
 
     if not args.no_cpu:
+        log("cpu oracle sample")
         crate, done, t_cpu, n_iter = cpu_leg(sims, args)
         out["cpu_baseline"] = {
             "value": round(crate, 3), "unit": "Mcell-updates/s", "cores": 1, "kind": "port",

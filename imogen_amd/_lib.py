@@ -114,6 +114,7 @@ EXTRA = {
     "imh_batch_set_fluid_tile": (_I, [_P, _I, _I, _I, _I, _I]),
     "imh_profile_iterations": (_I, [_P, _I, _D, _D, ctypes.POINTER(_I)]),
     "imh_batch_profile_step": (_I, [_P, _I, _D, _D, ctypes.POINTER(_I)]),
+    "imh_copy_probe": (_I, [ctypes.c_size_t, _I, _D]),
 }
 
 

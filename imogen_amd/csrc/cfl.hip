@@ -6,6 +6,7 @@
 // direction (non-negative doubles order like their bit patterns).  dt is computed on the
 // device, so an iteration needs no host round trip and can be replayed as a hipGraph.
 #include "imh_common.h"
+#include "imh_host.h"
 #include "imh_kernels.h"
 
 namespace imh {
@@ -251,4 +252,57 @@ template hipError_t launch_conservation<double>(const double* const*, int, long 
 template hipError_t launch_conservation<float>(const float* const*, int, long long, double*, int,
                                                double*, hipStream_t);
 
+// ---- device copy probe ----------------------------------------------------------------
+// A plain streaming copy (16-byte vectors, grid-stride): the reference kernel of known
+// traffic for scaling HBM counters and the measured copy bandwidth beside the roofline.
+__global__ void __launch_bounds__(256) copy_probe_kernel(const float4* __restrict__ src,
+                                                         float4* __restrict__ dst, long long n) {
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n;
+         i += (long long)gridDim.x * 256)
+        dst[i] = src[i];
+}
+
 }  // namespace imh
+
+extern "C" int imh_copy_probe(size_t bytes, int reps, double* ms_per_copy) {
+    using namespace imh;
+    if (bytes < 16 || reps < 1 || !ms_per_copy) return fail(IMH_EINVAL, "bad argument");
+    const long long n = (long long)(bytes / 16);
+    void *a = nullptr, *b = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipStream_t st = nullptr;
+    hipError_t e = hipMalloc(&a, n * 16);
+    if (e == hipSuccess) e = hipMalloc(&b, n * 16);
+    if (e == hipSuccess) e = hipMemset(a, 0, n * 16);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    int nb = 0;
+    if (e == hipSuccess) {
+        hipDeviceProp_t prop;
+        e = hipGetDeviceProperties(&prop, 0);
+        nb = prop.multiProcessorCount * 8;
+    }
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(copy_probe_kernel, dim3(nb), dim3(256), 0, st, (const float4*)a,
+                           (float4*)b, n);  // warm-up
+        e = hipEventRecord(e0, st);
+    }
+    for (int r = 0; r < reps && e == hipSuccess; ++r) {
+        hipLaunchKernelGGL(copy_probe_kernel, dim3(nb), dim3(256), 0, st, (const float4*)a,
+                           (float4*)b, n);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    if (e != hipSuccess) return hip_fail(e, "copy probe");
+    *ms_per_copy = ms / reps;
+    return IMH_OK;
+}

@@ -50,7 +50,8 @@ __device__ inline long long xcd_tile(long long b, long long nblocks) {
 // object, with its tracker).  Out of line: only pencil-end threads need it, and keeping
 // the BC code out of every call site keeps the kernel inside the instruction cache.
 __device__ __noinline__ double fluid_ghost(int mode, int upper, int flip_sign, int zero_ghost,
-                                           int active_lin, int bc_inf, const double* ends, int N,
+                                           int active_lin, int bc_inf, double a0, double a1,
+                                           double a2, double b0, double b1, double b2, int N,
                                            double* tracker) {
     GhostSpec s;
     s.mode = mode;
@@ -59,21 +60,17 @@ __device__ __noinline__ double fluid_ghost(int mode, int upper, int flip_sign, i
     s.zero_ghost = zero_ghost;
     s.active_linear = active_lin;
     s.bc_infinity = bc_inf;
-    const Ends e = {ends[0], ends[1], ends[2], ends[3], ends[4], ends[5]};
+    const Ends e = {a0, a1, a2, b0, b1, b2};
     double g0, g1;
     ghost_values(s, e, N, 1, tracker, g0, g1);
     return g0;
 }
 
-// Fade blend of a constant-shifted value (fade_shift.m:31-45) for a cell inside one of
-// the fade windows; out of line for the same reason.
-__device__ __noinline__ double fluid_fade(double r, int lower, int gi, int N, const double* ends,
+// Fade blend (fade_shift.m:31-45) of a constant-shifted value r at cell gi inside one of
+// the fade windows; v1 / vN are the shifted array's first and last values.  Out of line
+// for the same reason.
+__device__ __noinline__ double fluid_fade(double r, int gi, int N, double v1, double vN,
                                           const double* c, int f) {
-    const Ends e = {ends[0], ends[1], ends[2], ends[3], ends[4], ends[5]};
-    // constant-shifted end values: lower shift r[0]=a[0], r[N-1]=a[N-2];
-    // upper shift r[0]=a[1], r[N-1]=a[N-1]
-    const double v1 = lower ? e.a0 : end_at(e, 1, N);
-    const double vN = lower ? end_at(e, N - 2, N) : e.b2;
     return fade_blend(r, gi, N, v1, vN, c, f);
 }
 
@@ -126,41 +123,134 @@ __device__ inline T pencil_reduce_max(T v, const FluidArgs<T>& a, int tid, int p
 
 }  // namespace
 
-// The sweep of one tile (PX pencils) of one simulation.  Called by the per-simulation
-// kernel (one block per tile) and by the batched persistent kernel (a block loops over
-// a tile list that mixes simulations of different sizes).
+// Arithmetic of the two precisions.  FP64 keeps every division and square root of the
+// .m text, in the same order, so it reproduces the CPU restatement; the FP32 bulk path
+// divides by multiplying with the hardware reciprocal and takes the FP32 square root,
+// while the pressure floor (E - KE, max(P, 0)) stays FP64.
+template <typename T>
+struct FMath;
+
+template <>
+struct FMath<double> {
+    static __device__ __forceinline__ double div(double x, double y, double) { return x / y; }
+    static __device__ __forceinline__ double rcp(double y) { return y; }  // divisor kept
+    // pressure.m 'totsnd': P and the fast speed c
+    static __device__ __forceinline__ void press(double rho, double px, double py, double pz,
+                                                 double E, double bx, double by, double bz,
+                                                 MagUse u, double g, double& P, double& c,
+                                                 double& inv) {
+        pressure_totsnd(rho, px, py, pz, E, bx, by, bz, u, g, P, c);
+        inv = rho;
+    }
+    static __device__ __forceinline__ double divrho(double x, double rho_or_inv) {
+        return x / rho_or_inv;
+    }
+    static __device__ __forceinline__ double vanleer(double dl, double dr) {
+        double t = fmax(dl * dr, 0.0) / (dl + dr);
+        return isnan(t) ? 0.0 : t;
+    }
+};
+
+template <>
+struct FMath<float> {
+    static __device__ __forceinline__ float div(float x, float, float r) { return x * r; }
+    static __device__ __forceinline__ float rcp(float y) { return __builtin_amdgcn_rcpf(y); }
+    static __device__ __forceinline__ void press(float rho, float px, float py, float pz, float E,
+                                                 float bx, float by, float bz, MagUse u, double g,
+                                                 float& P, float& c, float& inv) {
+        inv = __builtin_amdgcn_rcpf(rho);
+        const float v2 = (px * px + py * py + pz * pz) * inv * inv;
+        float b2 = 0.0f;
+        if (u.x) b2 += bx * bx;
+        if (u.y) b2 += by * by;
+        if (u.z) b2 += bz * bz;
+        const float gf = float(g);
+        // positivity-critical difference and floor in FP64
+        const double r = (g - 1.0) * (double(E) - 0.5 * double(rho * v2));
+        const float rf = float(r);
+        c = __builtin_amdgcn_sqrtf(fabsf((gf * (rf - (gf - 1.0f) * 0.5f * b2) + 2.0f * b2) * inv));
+        const double Pd = r + 0.5 * (2.0 - g) * double(b2);
+        P = float(Pd < 0.0 ? 0.0 : Pd);
+    }
+    static __device__ __forceinline__ float divrho(float x, float inv) { return x * inv; }
+    static __device__ __forceinline__ float vanleer(float dl, float dr) {
+        float t = fmaxf(dl * dr, 0.0f) * __builtin_amdgcn_rcpf(dl + dr);
+        return isnan(t) ? 0.0f : t;
+    }
+};
+
+template <typename T>
+__device__ __forceinline__ T limit(int lim, T dl, T dr) {
+    if (lim == IMH_LIM_VANLEER) return FMath<T>::vanleer(dl, dr);
+    if (lim == IMH_LIM_SUPERBEE) {
+        T cf = (fabs(dl) < fabs(dr)) ? T(1) : T(0);
+        return lim_minmod((T(1) + cf) * dl, (T(2) - cf) * dr);
+    }
+    return lim_minmod(dl, dr);
+}
+
+// Fields handled per LDS pass: the FR / FL rows of G fields of the block's pencils live in
+// LDS at once (at most about 48 KiB of rows).
+template <typename T, int BS, int KMAX>
+constexpr int fluid_group() {
+    constexpr int per_field = 2 * BS * KMAX * int(sizeof(T));
+    constexpr int g = (48 * 1024) / per_field;
+    return g < 1 ? 1 : (g > 5 ? 5 : g);
+}
+
+// The sweep of one tile (PX pencils) of one simulation.
+//
+// Cells are held in registers (K per thread).  Each phase of relaxingFluid.m works on a
+// group of G fields: the threads write that phase's FR / FL of their cells into LDS rows,
+// barrier; one lane per (pencil, field, shift call) evaluates the pencil-end ghost values
+// (BC rule + edge tracker) in parallel, barrier; every thread reads its cells' neighbours
+// from the rows (or the ghosts) and updates its registers.  The row layout puts cell k of
+// thread ty at (k*TL + ty), so consecutive lanes touch consecutive LDS words.
 template <typename T, int BS, int KMAX, bool MHD>
-__device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& a, long long tile) {
+__device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long long tile) {
+    const FluidArgs<T> a = ap;  // private copy: stores to the fields cannot alias it
     if (!a.force && !a.sim->active) return;
-    __shared__ T s_x[2][4][BS];
-    __shared__ double s_ends[2][2][PXMAX][6];
+    using M = FMath<T>;
+    constexpr int G = fluid_group<T, BS, KMAX>();
+    constexpr int CELLS = BS * KMAX;
+    __shared__ T s_row[2][G][CELLS];        // FR (0) and FL (1) rows of G fields
+    __shared__ double s_gh[PXMAX][5][4];    // ghosts: FR lower, FL lower, FR upper, FL upper
     __shared__ T s_red[2][(BS / 64) * PXMAX];
     // edge trackers of the block's pencils: [px][side*16 + role*5 + field], roles lower:
     // store.fluxR, fluxR, fluxL; upper: store.fluxL, fluxR, fluxL
     __shared__ double s_trk[PXMAX][32];
 
     const int tid = threadIdx.x;
-    const int px = a.xmajor ? tid / a.TL : tid % a.PX;
-    const int ty = a.xmajor ? tid % a.TL : tid / a.PX;
-    const int dstep = a.xmajor ? 1 : a.PX;  // thread distance between neighbouring chunks
-    const long long pen0 = tile * a.PX;
+    const int PX = a.PX, TL = a.TL;
+    const int px = a.xmajor ? tid / TL : tid % PX;
+    const int ty = a.xmajor ? tid % TL : tid / PX;
+    const long long pen0 = tile * PX;
     const long long pen = pen0 + px;
     const bool pen_ok = pen < a.npencil;
     const int d = a.d;
     const int nn[3] = {a.n0, a.n1, a.n2};
     const int N = sel3i(d, a.n0, a.n1, a.n2);
     const int K = a.K;
+    const int L = K * TL;                   // row slots per pencil
     const int i0 = ty * K;
     int kc = N - i0;
     kc = kc < 0 ? 0 : (kc > K ? K : kc);
     if (!pen_ok) kc = 0;
-    const bool has_left = ty > 0;
-    const bool has_right = i0 + K < N;
-    const bool own_first = pen_ok && ty == 0;
-    const bool own_last = pen_ok && kc > 0 && i0 + kc == N;
 
-    long long base = 0, stride = 1;
-    if (pen_ok) pencil_base(nn, d, pen, base, stride);
+    // LDS index of cell i of pencil p
+    auto idx = [&](int p, int i) __attribute__((always_inline)) -> int {
+        const int slot = (i % K) * TL + i / K;
+        return a.xmajor ? p * L + slot : slot * PX + p;
+    };
+    auto at_slot = [&](int slot) __attribute__((always_inline)) -> int {
+        return a.xmajor ? px * L + slot : slot * PX + px;
+    };
+    auto my_idx = [&](int k) __attribute__((always_inline)) -> int { return at_slot(k * TL + ty); };
+
+    // 32-bit cell offsets (a simulation holds fewer than 2^31 cells): one register each
+    long long base64 = 0, stride64 = 1;
+    if (pen_ok) pencil_base(nn, d, pen, base64, stride64);
+    const int base = int(base64), stride = int(stride64);
 
     const double dt = a.sim->dt;
     const double ff = dt / a.dx;          // fluxFactor = dTime ./ DGRID{X}
@@ -171,10 +261,11 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& a, long lon
     const int mlo = mode_at(a.modes, 0, d), mhi = mode_at(a.modes, 1, d);
     const bool trk_lo = (mlo == IMH_BC_TRANS || mlo == IMH_BC_WALL);
     const bool trk_hi = (mhi == IMH_BC_TRANS || mhi == IMH_BC_WALL);
+    const bool fade = (mlo == IMH_BC_FADE || mhi == IMH_BC_FADE);
 
     // ---- tracker prefetch (cooperative) ----------------------------------------------
     if ((trk_lo || trk_hi) && !a.init_edges) {
-        for (int e = tid; e < a.PX * 30; e += BS) {
+        for (int e = tid; e < PX * 30; e += BS) {
             int p = e / 30, r = e % 30;
             int side = r / 15, role = (r % 15) / 5, v = r % 5;
             long long pp = pen0 + p;
@@ -191,7 +282,7 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& a, long lon
 #pragma unroll
     for (int k = 0; k < KMAX; ++k) {
         if (k < kc) {
-            long long o = base + (long long)(i0 + k) * stride;
+            const int o = base + (i0 + k) * stride;
 #pragma unroll
             for (int v = 0; v < 5; ++v) q[v][k] = a.q[v][o];
 #pragma unroll
@@ -204,26 +295,6 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& a, long lon
         }
     }
 
-    // Ghost-cell spec of one side for an object of field v and flux type ft.
-    auto spec = [&](int upper, int v, int ft) __attribute__((always_inline)) {
-        GhostSpec s;
-        s.mode = upper ? mhi : mlo;
-        s.upper = upper;
-        s.flip_sign = (v == X);  // component of mom(X) equals the shift direction
-        // zero mode: a FLUXL object keeps its lower ghost, FLUXR its upper ghost
-        s.zero_ghost = upper ? (ft != FT_FLUXR) : (ft != FT_FLUXL);
-        s.active_linear = a.active_lin;
-        s.bc_infinity = f;
-        return s;
-    };
-    // ghost of one shift call by one cell, from the staged pencil ends of quantity
-    // `which` in buffer `buf`; slot = tracker slot in s_trk
-    auto ghost1 = [&](int upper, int v, int ft, int buf, int which, int slot) __attribute__((always_inline)) -> T {
-        const GhostSpec sp = spec(upper, v, ft);
-        return T(fluid_ghost(sp.mode, sp.upper, sp.flip_sign, sp.zero_ghost, sp.active_linear,
-                             sp.bc_infinity, s_ends[buf][which][px], N, &s_trk[px][slot]));
-    };
-
     // First assignment of a flux object (InitializedArray.m:43-47): record its edges on
     // the faces of the other dimensions.
     auto init_faces = [&](int obj, const T (&vals)[KMAX]) __attribute__((always_inline)) {
@@ -234,249 +305,237 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& a, long lon
                 fluid_init_face(a.edges, a.eoff, a.modes, a.ndims, d, obj, i0 + k, pen, a.n0,
                                 a.n1, a.n2, double(vals[k]));
     };
+    auto put = [&](int which, int gv, const T (&A)[KMAX]) __attribute__((always_inline)) {
+#pragma unroll
+        for (int k = 0; k < KMAX; ++k)
+            if (k < kc) s_row[which][gv][my_idx(k)] = A[k];
+    };
 
-    // Stage the chunk boundary values and the pencil ends of two quantities.
-    // Register arrays are indexed with compile-time k only (no scratch spills).
-    auto stage = [&](int buf, const T (&A)[KMAX], const T (&B)[KMAX]) __attribute__((always_inline)) {
-        if (kc > 0) {
-            s_x[buf][0][tid] = A[0];
-            s_x[buf][2][tid] = B[0];
-            T la = A[0], lb = B[0];
-#pragma unroll
-            for (int k = 1; k < KMAX; ++k)
-                if (k == kc - 1) { la = A[k]; lb = B[k]; }
-            s_x[buf][1][tid] = la;
-            s_x[buf][3][tid] = lb;
-#pragma unroll
-            for (int k = 0; k < KMAX; ++k) {
-                if (k < kc) {
-                    int gi = i0 + k;
-                    if (gi < 3) {
-                        s_ends[buf][0][px][gi] = double(A[k]);
-                        s_ends[buf][1][px][gi] = double(B[k]);
-                    }
-                    if (gi >= N - 3) {
-                        s_ends[buf][0][px][3 + gi - (N - 3)] = double(A[k]);
-                        s_ends[buf][1][px][3 + gi - (N - 3)] = double(B[k]);
-                    }
-                }
+    // Ghost values of one phase for the fields [v0, v0 + ng): kind 0 predictor
+    // (store.fluxR lower, store.fluxL upper), 1 corrector (fluxR and fluxL, both sides),
+    // 2 final (fluxR lower, fluxL upper).  One lane per (pencil, field, shift call).
+    auto ghost_phase = [&](int kind, int v0, int ng) __attribute__((always_inline)) {
+        const int J = kind == 1 ? 4 : 2;
+        for (int job = tid; job < PX * ng * J; job += BS) {
+            const int p = job / (ng * J);
+            const int r = job % (ng * J);
+            const int gv = r / J;
+            const int v = v0 + gv;
+            const int j = (J == 4) ? (r % J) : ((r % J) ? 3 : 0);  // 0 FR lo, 1 FL lo, 2 FR hi, 3 FL hi
+            if (pen0 + p >= a.npencil) continue;
+            const int which = j & 1;           // 0 fluxR, 1 fluxL
+            const int upper = j >> 1;
+            const int role = kind == 0 ? 0 : 1 + which;
+            const int slot = upper * 16 + role * 5 + v;
+            const T* row = s_row[which][gv];
+            const double e0 = double(row[idx(p, 0)]), e1 = double(row[idx(p, N > 1 ? 1 : 0)]);
+            const double e2 = double(row[idx(p, N > 2 ? 2 : N - 1)]);
+            const double e3 = double(row[idx(p, N >= 3 ? N - 3 : 0)]);
+            const double e4 = double(row[idx(p, N >= 2 ? N - 2 : 0)]);
+            const double e5 = double(row[idx(p, N - 1)]);
+            if (a.init_edges && kind < 2) s_trk[p][slot] = upper ? e5 : e0;
+            const int mode = upper ? mhi : mlo;
+            double gval;
+            if (mode == IMH_BC_CIRC) gval = upper ? e0 : e5;
+            else if (mode == IMH_BC_CONST || mode == IMH_BC_FADE) gval = upper ? e5 : e0;
+            else {
+                const int ft = which ? FT_FLUXL : FT_FLUXR;
+                const int zero_ghost = upper ? (ft != FT_FLUXR) : (ft != FT_FLUXL);
+                gval = fluid_ghost(mode, upper, v == X, zero_ghost, a.active_lin, f, e0, e1, e2,
+                                   e3, e4, e5, N, &s_trk[p][slot]);
+            }
+            s_gh[p][v][j] = gval;
+        }
+    };
+
+    // shift(X,-1) (lower) / shift(X,+1) of row (which, gv) of field v at cell i0+k
+    auto shifted = [&](bool lower, int which, int gv, int v, int k) __attribute__((always_inline)) -> T {
+        const int i = i0 + k;
+        const T* row = s_row[which][gv];
+        T r;
+        // the neighbour is this thread's cell k -+ 1, or the last / first cell of the
+        // neighbouring thread's chunk
+        if (lower) r = (i > 0) ? row[at_slot(k > 0 ? (k - 1) * TL + ty : (K - 1) * TL + ty - 1)]
+                               : T(s_gh[px][v][which]);
+        else r = (i < N - 1) ? row[at_slot(k + 1 < K ? (k + 1) * TL + ty : ty + 1)]
+                             : T(s_gh[px][v][2 + which]);
+        if (fade) {
+            const int mode = lower ? mlo : mhi;
+            if (mode == IMH_BC_FADE && (i < f || i >= N - f)) {
+                // constant-shifted end values: lower r[0]=a[0], r[N-1]=a[N-2];
+                // upper r[0]=a[1], r[N-1]=a[N-1]
+                const double v1 = double(row[idx(px, lower ? 0 : 1)]);
+                const double vN = double(row[idx(px, lower ? N - 2 : N - 1)]);
+                r = T(fluid_fade(double(r), i, N, v1, vN, a.fade, f));
             }
         }
-    };
-    auto last_of = [&](const T (&A)[KMAX]) __attribute__((always_inline)) {
-        T r = A[0];
-#pragma unroll
-        for (int k = 1; k < KMAX; ++k)
-            if (k == kc - 1) r = A[k];
         return r;
     };
 
-    // Value of shift(X,-1) (lower=true) or shift(X,+1) at chunk cell k (k compile-time).
-    //   nb: the neighbouring chunk's boundary value; ghost: the BC ghost.
-    auto shifted = [&](bool lower, const T (&A)[KMAX], int k, T nb, T ghost, int mode,
-                       int buf, int which) __attribute__((always_inline)) -> T {
-        int gi = i0 + k;
-        T r;
-        if (lower) r = (k > 0) ? A[k > 0 ? k - 1 : 0] : (has_left ? nb : ghost);
-        else r = (k < kc - 1) ? A[k + 1 < KMAX ? k + 1 : k] : (has_right ? nb : ghost);
-        if (mode == IMH_BC_FADE && (gi < f || gi >= N - f))
-            r = T(fluid_fade(double(r), lower, gi, N, s_ends[buf][which][px], a.fade, f));
-        return r;
-    };
-
-    int buf = 0;
-    T P[KMAX], U[KMAX];
-
-    // ================= predictor: W and C from U^n (wFluidFlux.m) ====================
-    T lmax = T(NAN);
-#pragma unroll
-    for (int k = 0; k < KMAX; ++k) {
-        P[k] = T(0);
-        U[k] = T(0);
-        if (k < kc) {
-            double Pd, cd;
-            pressure_totsnd(double(q[0][k]), double(q[1][k]), double(q[2][k]), double(q[3][k]),
-                            double(q[4][k]), double(bm[0][k]), double(bm[1][k]), double(bm[2][k]),
-                            a.use, g, Pd, cd);
-            P[k] = T(Pd);
-            U[k] = sel3(d, q[1][k], q[2][k], q[3][k]) / q[0][k];
-            T s = fabs(U[k]) + T(cd);
-            lmax = (k == 0) ? s : fmax(lmax, s);
-        }
-    }
-    const T C = pencil_reduce_max(lmax, a, tid, px, s_red[0]);
-
-    T qh[5][KMAX];
-    const T hfC = T(hf) * C;
-    auto predictor_field = [&](auto VC) __attribute__((always_inline)) {
-        constexpr int v = decltype(VC)::value;
-        T FR[KMAX], FL[KMAX];
+    T P[KMAX], U[KMAX], IR[KMAX];
+    // pressure, velocity and the freezing-speed candidate of every cell of state Q
+    auto cell_speeds = [&](const T (&Q)[5][KMAX]) __attribute__((always_inline)) -> T {
+        T lmax = T(NAN);
 #pragma unroll
         for (int k = 0; k < KMAX; ++k) {
-            T w;
+            P[k] = T(0);
+            U[k] = T(0);
+            IR[k] = T(1);
+            if (k < kc) {
+                T Pk, ck, ir;
+                M::press(Q[0][k], Q[1][k], Q[2][k], Q[3][k], Q[4][k], bm[0][k], bm[1][k],
+                         bm[2][k], a.use, g, Pk, ck, ir);
+                P[k] = Pk;
+                IR[k] = ir;
+                U[k] = M::divrho(sel3(d, Q[1][k], Q[2][k], Q[3][k]), ir);
+                T sp = fabs(U[k]) + ck;
+                lmax = (k == 0) ? sp : fmax(lmax, sp);
+            }
+        }
+        return lmax;
+    };
+
+    // wFluidFlux.m: FR / FL of field v from state Q, freezing speed C
+    auto fluxes = [&](auto VC, const T (&Q)[5][KMAX], T C, T rC, T (&FR)[KMAX], T (&FL)[KMAX])
+        __attribute__((always_inline)) {
+        constexpr int v = decltype(VC)::value;
+#pragma unroll
+        for (int k = 0; k < KMAX; ++k) {
             const T bX = sel3(d, bm[0][k], bm[1][k], bm[2][k]);
+            T w;
             if constexpr (v == 0) {
-                w = sel3(d, q[1][k], q[2][k], q[3][k]) / C;
+                w = M::div(sel3(d, Q[1][k], Q[2][k], Q[3][k]), C, rC);
             } else if constexpr (v == 4) {
-                T bdotp = bm[0][k] * q[1][k] + bm[1][k] * q[2][k] + bm[2][k] * q[3][k];
-                T we = U[k] * (q[4][k] + P[k]) - bX * bdotp / q[0][k];
-                w = we / C;
+                T bdotp = bm[0][k] * Q[1][k] + bm[1][k] * Q[2][k] + bm[2][k] * Q[3][k];
+                T we = U[k] * (Q[4][k] + P[k]) - M::divrho(bX * bdotp, IR[k]);
+                w = M::div(we, C, rC);
             } else {
                 T dv = (v == X) ? T(1) : T(0);
-                w = (U[k] * q[v][k] + P[k] * dv - bX * bm[v - 1][k]) / C;
+                w = M::div(U[k] * Q[v][k] + P[k] * dv - bX * bm[v - 1][k], C, rC);
             }
-            FR[k] = T(0.5) * (q[v][k] + w);
-            FL[k] = T(0.5) * (q[v][k] - w);
+            FR[k] = T(0.5) * (Q[v][k] + w);
+            FL[k] = T(0.5) * (Q[v][k] - w);
         }
-        init_faces(OBJ_SFR + v, FR);
-        init_faces(OBJ_SFL + v, FL);
-        stage(buf, FR, FL);
-        if (a.init_edges) {
-            if (own_first) s_trk[px][0 * 16 + 0 * 5 + v] = double(FR[0]);
-            if (own_last) s_trk[px][1 * 16 + 0 * 5 + v] = double(last_of(FL));
-        }
-        __syncthreads();
-        T gR = T(0), gL = T(0);
-        if (own_first) gR = ghost1(0, v, FT_FLUXR, buf, 0, 0 * 16 + 0 * 5 + v);
-        if (own_last) gL = ghost1(1, v, FT_FLUXL, buf, 1, 1 * 16 + 0 * 5 + v);
-        const T nbR = has_left ? s_x[buf][1][tid - dstep] : T(0);
-        const T nbL = has_right ? s_x[buf][2][tid + dstep] : T(0);
-#pragma unroll
-        for (int k = 0; k < KMAX; ++k) {
-            qh[v][k] = T(1);
-            if (k < kc) {
-                T sR = shifted(true, FR, k, nbR, gR, mlo, buf, 0);
-                T sL = shifted(false, FL, k, nbL, gL, mhi, buf, 1);
-                qh[v][k] = q[v][k] - hfC * (FR[k] - sR + FL[k] - sL);
-            }
-        }
-        buf ^= 1;
     };
-    predictor_field(FieldIdx<4>{});
-    predictor_field(FieldIdx<1>{});
-    predictor_field(FieldIdx<2>{});
-    predictor_field(FieldIdx<3>{});
-    predictor_field(FieldIdx<0>{});
+    // run fn(FieldIdx<v>, gv) for the fields of group [v0, v0 + G)
+    auto for_group = [&](int v0, auto fn) __attribute__((always_inline)) {
+        auto one = [&](auto VC) __attribute__((always_inline)) {
+            constexpr int v = decltype(VC)::value;
+            if (v >= v0 && v < v0 + G) fn(VC, v - v0);
+        };
+        one(FieldIdx<0>{});
+        one(FieldIdx<1>{});
+        one(FieldIdx<2>{});
+        one(FieldIdx<3>{});
+        one(FieldIdx<4>{});
+    };
+
+    // ================= predictor: W and C from U^n (wFluidFlux.m) ====================
+    const T C = pencil_reduce_max(cell_speeds(q), a, tid, px, s_red[0]);
+    const T rC = M::rcp(C);
+    const T hfC = T(hf) * C;
+    T qh[5][KMAX];
+#pragma unroll
+    for (int v0 = 0; v0 < 5; v0 += G) {
+        const int ng = (5 - v0) < G ? (5 - v0) : G;
+        if (v0 > 0) __syncthreads();  // the previous group's rows have been read
+        for_group(v0, [&](auto VC, int gv) __attribute__((always_inline)) {
+            constexpr int v = decltype(VC)::value;
+            T FR[KMAX], FL[KMAX];
+            fluxes(VC, q, C, rC, FR, FL);
+            init_faces(OBJ_SFR + v, FR);
+            init_faces(OBJ_SFL + v, FL);
+            put(0, gv, FR);
+            put(1, gv, FL);
+        });
+        __syncthreads();
+        ghost_phase(0, v0, ng);
+        __syncthreads();
+        for_group(v0, [&](auto VC, int gv) __attribute__((always_inline)) {
+            constexpr int v = decltype(VC)::value;
+#pragma unroll
+            for (int k = 0; k < KMAX; ++k) {
+                qh[v][k] = T(1);
+                if (k < kc) {
+                    const T fr = s_row[0][gv][my_idx(k)], fl = s_row[1][gv][my_idx(k)];
+                    const T sR = shifted(true, 0, gv, v, k);
+                    const T sL = shifted(false, 1, gv, v, k);
+                    qh[v][k] = q[v][k] - hfC * (fr - sR + fl - sL);
+                }
+            }
+        });
+    }
     // mass.store.array = max(mass.store.array, MINMASS)
 #pragma unroll
     for (int k = 0; k < KMAX; ++k) qh[0][k] = fmax(qh[0][k], T(a.minmass));
 
     // ================= corrector: W' and C' from U^(n+1/2) ===========================
-    lmax = T(NAN);
-#pragma unroll
-    for (int k = 0; k < KMAX; ++k) {
-        if (k < kc) {
-            double Pd, cd;
-            pressure_totsnd(double(qh[0][k]), double(qh[1][k]), double(qh[2][k]),
-                            double(qh[3][k]), double(qh[4][k]), double(bm[0][k]),
-                            double(bm[1][k]), double(bm[2][k]), a.use, g, Pd, cd);
-            P[k] = T(Pd);
-            U[k] = sel3(d, qh[1][k], qh[2][k], qh[3][k]) / qh[0][k];
-            T s = fabs(U[k]) + T(cd);
-            lmax = (k == 0) ? s : fmax(lmax, s);
-        }
-    }
-    const T C2 = pencil_reduce_max(lmax, a, tid, px, s_red[1]);
+    const T C2 = pencil_reduce_max(cell_speeds(qh), a, tid, px, s_red[1]);
+    const T rC2 = M::rcp(C2);
     const T ffC = T(ff) * C2;
     const int lim = a.limiter;
-
-    auto corrector_field = [&](auto VC) __attribute__((always_inline)) {
-        constexpr int v = decltype(VC)::value;
-        T FR[KMAX], FL[KMAX];
 #pragma unroll
-        for (int k = 0; k < KMAX; ++k) {
-            T w;
-            const T bX = sel3(d, bm[0][k], bm[1][k], bm[2][k]);
-            if constexpr (v == 0) {
-                w = sel3(d, qh[1][k], qh[2][k], qh[3][k]) / C2;
-            } else if constexpr (v == 4) {
-                T bdotp = bm[0][k] * qh[1][k] + bm[1][k] * qh[2][k] + bm[2][k] * qh[3][k];
-                T we = U[k] * (qh[4][k] + P[k]) - bX * bdotp / qh[0][k];
-                w = we / C2;
-            } else {
-                T dv = (v == X) ? T(1) : T(0);
-                w = (U[k] * qh[v][k] + P[k] * dv - bX * bm[v - 1][k]) / C2;
-            }
-            FR[k] = T(0.5) * (qh[v][k] + w);
-            FL[k] = T(0.5) * (qh[v][k] - w);
-        }
-        init_faces(OBJ_FR + v, FR);
-        init_faces(OBJ_FL + v, FL);
-        stage(buf, FR, FL);
-        if (a.init_edges) {
-            if (own_first) {
-                s_trk[px][0 * 16 + 1 * 5 + v] = double(FR[0]);
-                s_trk[px][0 * 16 + 2 * 5 + v] = double(FL[0]);
-            }
-            if (own_last) {
-                s_trk[px][1 * 16 + 1 * 5 + v] = double(last_of(FR));
-                s_trk[px][1 * 16 + 2 * 5 + v] = double(last_of(FL));
-            }
-        }
+    for (int v0 = 0; v0 < 5; v0 += G) {
+        const int ng = (5 - v0) < G ? (5 - v0) : G;
+        if (v0 > 0) __syncthreads();
+        for_group(v0, [&](auto VC, int gv) __attribute__((always_inline)) {
+            constexpr int v = decltype(VC)::value;
+            T FR[KMAX], FL[KMAX];
+            fluxes(VC, qh, C2, rC2, FR, FL);
+            init_faces(OBJ_FR + v, FR);
+            init_faces(OBJ_FL + v, FL);
+            put(0, gv, FR);
+            put(1, gv, FL);
+        });
         __syncthreads();
-        T FR2[KMAX], FL2[KMAX];
-        {
-            // ghosts: fluxR lower (1st nudge), fluxR upper, fluxL lower, fluxL upper (1st)
-            T gRlo = T(0), gRhi = T(0), gLlo = T(0), gLhi = T(0);
-            if (own_first) {
-                gRlo = ghost1(0, v, FT_FLUXR, buf, 0, 0 * 16 + 1 * 5 + v);
-                gLlo = ghost1(0, v, FT_FLUXL, buf, 1, 0 * 16 + 2 * 5 + v);
-            }
-            if (own_last) {
-                gRhi = ghost1(1, v, FT_FLUXR, buf, 0, 1 * 16 + 1 * 5 + v);
-                gLhi = ghost1(1, v, FT_FLUXL, buf, 1, 1 * 16 + 2 * 5 + v);
-            }
-            const T nbRl = has_left ? s_x[buf][1][tid - dstep] : T(0);   // FR of cell i0-1
-            const T nbRr = has_right ? s_x[buf][0][tid + dstep] : T(0);  // FR of cell i0+kc
-            const T nbLl = has_left ? s_x[buf][3][tid - dstep] : T(0);
-            const T nbLr = has_right ? s_x[buf][2][tid + dstep] : T(0);
+        ghost_phase(1, v0, ng);
+        __syncthreads();
+        T FR2[G][KMAX], FL2[G][KMAX];
+        for_group(v0, [&](auto VC, int gv) __attribute__((always_inline)) {
+            constexpr int v = decltype(VC)::value;
 #pragma unroll
             for (int k = 0; k < KMAX; ++k) {
-                FR2[k] = FL2[k] = T(0);
+                FR2[gv][k] = FL2[gv][k] = T(0);
                 if (k < kc) {
-                    T rm = shifted(true, FR, k, nbRl, gRlo, mlo, buf, 0);
-                    T rp = shifted(false, FR, k, nbRr, gRhi, mhi, buf, 0);
-                    T lm = shifted(true, FL, k, nbLl, gLlo, mlo, buf, 1);
-                    T lp = shifted(false, FL, k, nbLr, gLhi, mhi, buf, 1);
-                    FR2[k] = FR[k] + lim_delta(lim, FR[k] - rm, rp - FR[k]);
-                    FL2[k] = FL[k] + lim_delta(lim, lm - FL[k], FL[k] - lp);
+                    const T fr = s_row[0][gv][my_idx(k)], fl = s_row[1][gv][my_idx(k)];
+                    const T rm = shifted(true, 0, gv, v, k);
+                    const T rp = shifted(false, 0, gv, v, k);
+                    const T lm = shifted(true, 1, gv, v, k);
+                    const T lp = shifted(false, 1, gv, v, k);
+                    FR2[gv][k] = fr + limit(lim, fr - rm, rp - fr);
+                    FL2[gv][k] = fl + limit(lim, lm - fl, fl - lp);
                 }
             }
-        }
-        buf ^= 1;
-        stage(buf, FR2, FL2);
+        });
+        __syncthreads();  // every thread has read the FR / FL rows
+        for_group(v0, [&](auto VC, int gv) __attribute__((always_inline)) {
+            put(0, gv, FR2[gv]);
+            put(1, gv, FL2[gv]);
+        });
         __syncthreads();
-        {
-            T g2R = T(0), g2L = T(0);
-            if (own_first) g2R = ghost1(0, v, FT_FLUXR, buf, 0, 0 * 16 + 1 * 5 + v);
-            if (own_last) g2L = ghost1(1, v, FT_FLUXL, buf, 1, 1 * 16 + 2 * 5 + v);
-            const T nb2R = has_left ? s_x[buf][1][tid - dstep] : T(0);
-            const T nb2L = has_right ? s_x[buf][2][tid + dstep] : T(0);
+        ghost_phase(2, v0, ng);
+        __syncthreads();
+        for_group(v0, [&](auto VC, int gv) __attribute__((always_inline)) {
+            constexpr int v = decltype(VC)::value;
 #pragma unroll
             for (int k = 0; k < KMAX; ++k) {
                 if (k < kc) {
-                    T sR = shifted(true, FR2, k, nb2R, g2R, mlo, buf, 0);
-                    T sL = shifted(false, FL2, k, nb2L, g2L, mhi, buf, 1);
-                    const long long o = base + (long long)(i0 + k) * stride;
-                    // U^n is re-read here (it is L2-resident) rather than kept live
-                    T qn = a.q[v][o] - ffC * (FR2[k] - sR + FL2[k] - sL);
+                    const T sR = shifted(true, 0, gv, v, k);
+                    const T sL = shifted(false, 1, gv, v, k);
+                    const int o = base + (i0 + k) * stride;
+                    // U^n is re-read here (it is cache-resident) rather than kept live
+                    T qn = a.q[v][o] - ffC * (FR2[gv][k] - sR + FL2[gv][k] - sL);
                     if (v == 0) qn = fmax(qn, T(a.minmass));  // mass.array = max(mass, MINMASS)
                     a.q[v][o] = qn;
                 }
             }
-        }
-        buf ^= 1;
-    };
-    corrector_field(FieldIdx<4>{});
-    corrector_field(FieldIdx<1>{});
-    corrector_field(FieldIdx<2>{});
-    corrector_field(FieldIdx<3>{});
-    corrector_field(FieldIdx<0>{});
+        });
+    }
 
     // ---- write the edge trackers back -----------------------------------------------
     if (trk_lo || trk_hi) {
         __syncthreads();
-        for (int e = tid; e < a.PX * 30; e += BS) {
+        for (int e = tid; e < PX * 30; e += BS) {
             int p = e / 30, r = e % 30;
             int side = r / 15, role = (r % 15) / 5, v = r % 5;
             long long pp = pen0 + p;

@@ -346,6 +346,12 @@ int imh_create(const imh_config* cfg, imh_ctx** out) {
     for (int d = 0; d < 3; ++d) c->n[d] = cfg->grid[d];
     c->ndims = cfg->grid[2] > 1 ? 3 : 2;
     c->ncell = (long long)c->n[0] * c->n[1] * c->n[2];
+    if (c->ncell >= (1LL << 31)) {
+        const long long nc = c->ncell;
+        delete c;
+        return fail(IMH_ENOTSUP, "a simulation of %lld cells (the kernels index cells with 32 "
+                    "bits; split it into grids below 2^31 cells)", nc);
+    }
     for (int d = 0; d < 3; ++d) c->nface[d] = c->ncell / c->n[d];
     for (int d = 0; d < 3; ++d) {
         int L = d;  // Edges.ACTIVE(D) with linear index D = d+1 into the 2x3 array

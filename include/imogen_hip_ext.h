@@ -41,6 +41,10 @@ int imh_profile_iterations(imh_ctx *ctx, int iterations, double ms[IMH_NKIND],
 int imh_batch_profile_step(imh_batch *b, int iterations, double ms[IMH_NKIND],
                            double bytes[IMH_NKIND], int count[IMH_NKIND]);
 
+/* Time a plain device-to-device copy of `bytes` (16-byte vectors, grid-stride) over
+ * `reps` launches; *ms_per_copy gets the mean time.  Reads and writes bytes each. */
+int imh_copy_probe(size_t bytes, int reps, double *ms_per_copy);
+
 #ifdef __cplusplus
 }
 #endif
