@@ -38,6 +38,7 @@ struct Launch {
     int step;       // finalize: 1 or 2
     int bs, kmax;   // fluid / magnet variant
     bool mhd;
+    bool general;   // fluid: the kernel form with fade boundaries / first assignments
     size_t smem;
     long long off;  // into the tile array
     int ntiles;
@@ -258,10 +259,10 @@ static int build_plans(imh_batch* b) {
             if (oi == 1 && b->any_step_mode) cfl_block(2);
             for (int t = 0; t < 3; ++t)
                 for (int o = 0; o < 2; ++o) {
-                    // fluid groups keyed by (bs, kmax, mhd)
+                    // fluid groups keyed by (bs, kmax, mhd, general)
                     struct G {
                         int bs, kmax;
-                        bool mhd;
+                        bool mhd, general;
                         double bytes;
                         std::vector<std::pair<int, BatchTile>> tl;
                     };
@@ -281,11 +282,16 @@ static int build_plans(imh_batch* b) {
                         const bool fluid_op = (order > 0) ? (o == 0) : (o == 1);
                         if (fluid_op && c->cfg.fluid_active) {
                             const FluidTile& ft = c->ftile[X - 1];
+                            const int dX = X - 1;
+                            const bool gen = c->cfg.bc[0][dX] == IMH_BC_FADE ||
+                                             c->cfg.bc[1][dX] == IMH_BC_FADE;
                             G* g = nullptr;
                             for (G& gg : groups)
-                                if (gg.bs == ft.bs && gg.kmax == ft.kmax && gg.mhd == c->mhd_kernel) g = &gg;
+                                if (gg.bs == ft.bs && gg.kmax == ft.kmax && gg.mhd == c->mhd_kernel &&
+                                    gg.general == gen)
+                                    g = &gg;
                             if (!g) {
-                                groups.push_back({ft.bs, ft.kmax, c->mhd_kernel, 0.0, {}});
+                                groups.push_back({ft.bs, ft.kmax, c->mhd_kernel, gen, 0.0, {}});
                                 g = &groups.back();
                             }
                             g->bytes += double(10 + (c->mhd_kernel ? 3 : 0)) * c->ncell * es;
@@ -332,6 +338,7 @@ static int build_plans(imh_batch* b) {
                         f.bs = g.bs;
                         f.kmax = g.kmax;
                         f.mhd = g.mhd;
+                        f.general = g.general;
                         f.bytes = g.bytes;
                         std::vector<BatchTile> tl = sorted(g.tl);
                         add_launch(p, f, tl);
@@ -392,7 +399,7 @@ static hipError_t run_launch(imh_batch* b, const Launch& L) {
     case L_FINALIZE: return launch_time_finalize(b->sims_dev, b->pars_dev, n, L.step, b->st);
     case L_FLUID:
         return launch_fluid_batch<T>((const FluidArgs<T>*)b->fargs, tl, L.ntiles, nb, L.bs, L.kmax,
-                                     L.mhd, b->st);
+                                     L.mhd, L.general, b->st);
     case L_MAG_ADV:
         return launch_magnet_batch<T>(0, (const MagnetArgs<T>*)b->margs, tl, L.ntiles, nb, L.kmax,
                                       L.smem, b->st);

@@ -189,32 +189,37 @@ __device__ __forceinline__ T limit(int lim, T dl, T dr) {
     return lim_minmod(dl, dr);
 }
 
-// Fields handled per LDS pass: the FR / FL rows of G fields of the block's pencils live in
-// LDS at once (at most about 48 KiB of rows).
-template <typename T, int BS, int KMAX>
+// Fields handled per exchange phase: their FR / FL live in registers across the phase's
+// barriers, so FP32 chunks of 8 cells (and FP64 of 4) take them in smaller groups.
+template <typename T, int KMAX>
 constexpr int fluid_group() {
-    constexpr int per_field = 2 * BS * KMAX * int(sizeof(T));
-    constexpr int g = (48 * 1024) / per_field;
-    return g < 1 ? 1 : (g > 5 ? 5 : g);
+    return (int(sizeof(T)) * KMAX <= 16) ? 5 : (int(sizeof(T)) * KMAX <= 32 ? 2 : 1);
 }
 
 // The sweep of one tile (PX pencils) of one simulation.
 //
-// Cells are held in registers (K per thread).  Each phase of relaxingFluid.m works on a
-// group of G fields: the threads write that phase's FR / FL of their cells into LDS rows,
-// barrier; one lane per (pencil, field, shift call) evaluates the pencil-end ghost values
-// (BC rule + edge tracker) in parallel, barrier; every thread reads its cells' neighbours
-// from the rows (or the ghosts) and updates its registers.  The row layout puts cell k of
-// thread ty at (k*TL + ty), so consecutive lanes touch consecutive LDS words.
-template <typename T, int BS, int KMAX, bool MHD>
+// A pencil is split over TL threads, each keeping KMAX consecutive cells in registers
+// (cells past the pencil end are padding: loaded clamped, computed, never stored or
+// counted).  Each phase of relaxingFluid.m works on a group of fields: compute FR / FL
+// in registers, stage each chunk's first and last values (for the neighbouring threads)
+// and the pencil ends (for the BC) in LDS, barrier, evaluate all pencil-end ghosts of
+// the group in parallel (one lane per pencil, field and shift call), barrier, update.
+//
+// GEN = false is the common case and keeps the kernel free of calls (calls cost
+// registers): no fade boundary and no first assignment of the flux objects.  GEN = true
+// handles those too, with their code out of line.
+template <typename T, int BS, int KMAX, bool MHD, bool GEN>
 __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long long tile) {
     const FluidArgs<T> a = ap;  // private copy: stores to the fields cannot alias it
     if (!a.force && !a.sim->active) return;
     using M = FMath<T>;
-    constexpr int G = fluid_group<T, BS, KMAX>();
-    constexpr int CELLS = BS * KMAX;
-    __shared__ T s_row[2][G][CELLS];        // FR (0) and FL (1) rows of G fields
-    __shared__ double s_gh[PXMAX][5][4];    // ghosts: FR lower, FL lower, FR upper, FL upper
+    constexpr int G = fluid_group<T, KMAX>();
+    constexpr int K = KMAX;
+    // chunk boundary values of the group: [field*4 + q], q = 0 first FR, 1 last FR,
+    // 2 first FL, 3 last FL
+    __shared__ T s_x[4 * G][BS];
+    __shared__ double s_ends[2][G][PXMAX][6];   // FR / FL pencil ends (a[0..2], a[N-3..N-1])
+    __shared__ double s_gh[PXMAX][G][4];        // ghosts: FR lower, FL lower, FR upper, FL upper
     __shared__ T s_red[2][(BS / 64) * PXMAX];
     // edge trackers of the block's pencils: [px][side*16 + role*5 + field], roles lower:
     // store.fluxR, fluxR, fluxL; upper: store.fluxL, fluxR, fluxL
@@ -224,33 +229,26 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
     const int PX = a.PX, TL = a.TL;
     const int px = a.xmajor ? tid / TL : tid % PX;
     const int ty = a.xmajor ? tid % TL : tid / PX;
+    const int dstep = a.xmajor ? 1 : PX;
     const long long pen0 = tile * PX;
-    const long long pen = pen0 + px;
-    const bool pen_ok = pen < a.npencil;
+    const bool pen_ok = pen0 + px < a.npencil;
+    const long long pen = pen_ok ? pen0 + px : a.npencil - 1;  // padding pencils read the last one
     const int d = a.d;
     const int nn[3] = {a.n0, a.n1, a.n2};
     const int N = sel3i(d, a.n0, a.n1, a.n2);
-    const int K = a.K;
-    const int L = K * TL;                   // row slots per pencil
     const int i0 = ty * K;
-    int kc = N - i0;
-    kc = kc < 0 ? 0 : (kc > K ? K : kc);
-    if (!pen_ok) kc = 0;
-
-    // LDS index of cell i of pencil p
-    auto idx = [&](int p, int i) __attribute__((always_inline)) -> int {
-        const int slot = (i % K) * TL + i / K;
-        return a.xmajor ? p * L + slot : slot * PX + p;
-    };
-    auto at_slot = [&](int slot) __attribute__((always_inline)) -> int {
-        return a.xmajor ? px * L + slot : slot * PX + px;
-    };
-    auto my_idx = [&](int k) __attribute__((always_inline)) -> int { return at_slot(k * TL + ty); };
-
-    // 32-bit cell offsets (a simulation holds fewer than 2^31 cells): one register each
+    const bool has_left = ty > 0;
+    const int tl_ = tid - dstep < 0 ? 0 : tid - dstep;      // neighbour threads (clamped)
+    const int tr_ = tid + dstep >= BS ? BS - 1 : tid + dstep;
+    // 32-bit cell offsets (a simulation holds fewer than 2^31 cells)
     long long base64 = 0, stride64 = 1;
-    if (pen_ok) pencil_base(nn, d, pen, base64, stride64);
-    const int base = int(base64), stride = int(stride64);
+    pencil_base(nn, d, pen, base64, stride64);
+    const unsigned base = unsigned(base64), stride = unsigned(stride64);
+    auto off = [&](int k) __attribute__((always_inline)) -> unsigned {
+        const int i = i0 + k < N ? i0 + k : N - 1;
+        return base + unsigned(i) * stride;
+    };
+    auto valid = [&](int k) __attribute__((always_inline)) -> bool { return pen_ok && i0 + k < N; };
 
     const double dt = a.sim->dt;
     const double ff = dt / a.dx;          // fluxFactor = dTime ./ DGRID{X}
@@ -261,7 +259,8 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
     const int mlo = mode_at(a.modes, 0, d), mhi = mode_at(a.modes, 1, d);
     const bool trk_lo = (mlo == IMH_BC_TRANS || mlo == IMH_BC_WALL);
     const bool trk_hi = (mhi == IMH_BC_TRANS || mhi == IMH_BC_WALL);
-    const bool fade = (mlo == IMH_BC_FADE || mhi == IMH_BC_FADE);
+    const bool fade = GEN && (mlo == IMH_BC_FADE || mhi == IMH_BC_FADE);
+    const bool owns_end = pen_ok && (i0 < 3 || i0 + K > N - 3) && i0 < N;
 
     // ---- tracker prefetch (cooperative) ----------------------------------------------
     if ((trk_lo || trk_hi) && !a.init_edges) {
@@ -276,131 +275,136 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
         }
     }
 
-    // ---- load the pencil chunk ------------------------------------------------------
-    T q[5][KMAX];
-    T bm[3][KMAX];
+    // ---- load the chunk (clamped at the pencil end: no branches) ----------------------
+    T q[5][K];
+    T bm[3][K];
 #pragma unroll
-    for (int k = 0; k < KMAX; ++k) {
-        if (k < kc) {
-            const int o = base + (i0 + k) * stride;
+    for (int k = 0; k < K; ++k) {
+        const unsigned o = off(k);
 #pragma unroll
-            for (int v = 0; v < 5; ++v) q[v][k] = a.q[v][o];
+        for (int v = 0; v < 5; ++v) q[v][k] = a.q[v][o];
 #pragma unroll
-            for (int v = 0; v < 3; ++v) bm[v][k] = MHD ? a.bc[v][o] : T(0);
-        } else {
-#pragma unroll
-            for (int v = 0; v < 5; ++v) q[v][k] = T(1);
-#pragma unroll
-            for (int v = 0; v < 3; ++v) bm[v][k] = T(0);
-        }
+        for (int v = 0; v < 3; ++v) bm[v][k] = MHD ? a.bc[v][o] : T(0);
     }
 
     // First assignment of a flux object (InitializedArray.m:43-47): record its edges on
     // the faces of the other dimensions.
-    auto init_faces = [&](int obj, const T (&vals)[KMAX]) __attribute__((always_inline)) {
-        if (!a.init_edges || kc == 0 || a.ndims < 2) return;
+    auto init_faces = [&](int obj, const T (&vals)[K]) __attribute__((always_inline)) {
+        if constexpr (!GEN) return;
+        if (!a.init_edges || a.ndims < 2) return;
 #pragma unroll
-        for (int k = 0; k < KMAX; ++k)
-            if (k < kc)
+        for (int k = 0; k < K; ++k)
+            if (valid(k))
                 fluid_init_face(a.edges, a.eoff, a.modes, a.ndims, d, obj, i0 + k, pen, a.n0,
                                 a.n1, a.n2, double(vals[k]));
     };
-    auto put = [&](int which, int gv, const T (&A)[KMAX]) __attribute__((always_inline)) {
+    // pencil ends of a row held by this thread's cells
+    auto put_ends = [&](int which, int gv, const T (&A)[K]) __attribute__((always_inline)) {
+        if (!owns_end) return;
 #pragma unroll
-        for (int k = 0; k < KMAX; ++k)
-            if (k < kc) s_row[which][gv][my_idx(k)] = A[k];
+        for (int k = 0; k < K; ++k) {
+            const int gi = i0 + k;
+            if (gi < 3 && gi < N) s_ends[which][gv][px][gi] = double(A[k]);
+            if (gi >= N - 3 && gi < N) s_ends[which][gv][px][3 + gi - (N - 3)] = double(A[k]);
+        }
     };
 
-    // Ghost values of one phase for the fields [v0, v0 + ng): kind 0 predictor
-    // (store.fluxR lower, store.fluxL upper), 1 corrector (fluxR and fluxL, both sides),
-    // 2 final (fluxR lower, fluxL upper).  One lane per (pencil, field, shift call).
+    // Ghosts of one phase for the group [v0, v0 + ng): kind 0 predictor (store.fluxR
+    // lower, store.fluxL upper), 1 corrector (fluxR and fluxL, both sides), 2 final
+    // (fluxR lower, fluxL upper).
     auto ghost_phase = [&](int kind, int v0, int ng) __attribute__((always_inline)) {
         const int J = kind == 1 ? 4 : 2;
         for (int job = tid; job < PX * ng * J; job += BS) {
             const int p = job / (ng * J);
-            const int r = job % (ng * J);
+            const int r = job - p * (ng * J);
             const int gv = r / J;
             const int v = v0 + gv;
-            const int j = (J == 4) ? (r % J) : ((r % J) ? 3 : 0);  // 0 FR lo, 1 FL lo, 2 FR hi, 3 FL hi
+            const int jj = r - gv * J;
+            const int j = (J == 4) ? jj : (jj ? 3 : 0);  // 0 FR lo, 1 FL lo, 2 FR hi, 3 FL hi
             if (pen0 + p >= a.npencil) continue;
             const int which = j & 1;           // 0 fluxR, 1 fluxL
             const int upper = j >> 1;
             const int role = kind == 0 ? 0 : 1 + which;
             const int slot = upper * 16 + role * 5 + v;
-            const T* row = s_row[which][gv];
-            const double e0 = double(row[idx(p, 0)]), e1 = double(row[idx(p, N > 1 ? 1 : 0)]);
-            const double e2 = double(row[idx(p, N > 2 ? 2 : N - 1)]);
-            const double e3 = double(row[idx(p, N >= 3 ? N - 3 : 0)]);
-            const double e4 = double(row[idx(p, N >= 2 ? N - 2 : 0)]);
-            const double e5 = double(row[idx(p, N - 1)]);
-            if (a.init_edges && kind < 2) s_trk[p][slot] = upper ? e5 : e0;
+            const double* e = s_ends[which][gv][p];
+            if (GEN && a.init_edges && kind < 2) s_trk[p][slot] = upper ? e[5] : e[0];
             const int mode = upper ? mhi : mlo;
             double gval;
-            if (mode == IMH_BC_CIRC) gval = upper ? e0 : e5;
-            else if (mode == IMH_BC_CONST || mode == IMH_BC_FADE) gval = upper ? e5 : e0;
+            if (mode == IMH_BC_CIRC) gval = upper ? e[0] : e[5];
+            else if (mode == IMH_BC_CONST || mode == IMH_BC_FADE) gval = upper ? e[5] : e[0];
             else {
                 const int ft = which ? FT_FLUXL : FT_FLUXR;
                 const int zero_ghost = upper ? (ft != FT_FLUXR) : (ft != FT_FLUXL);
-                gval = fluid_ghost(mode, upper, v == X, zero_ghost, a.active_lin, f, e0, e1, e2,
-                                   e3, e4, e5, N, &s_trk[p][slot]);
+                if constexpr (GEN) {
+                    gval = fluid_ghost(mode, upper, v == X, zero_ghost, a.active_lin, f, e[0],
+                                       e[1], e[2], e[3], e[4], e[5], N, &s_trk[p][slot]);
+                } else {
+                    GhostSpec sp;
+                    sp.mode = mode;
+                    sp.upper = upper;
+                    sp.flip_sign = (v == X);
+                    sp.zero_ghost = zero_ghost;
+                    sp.active_linear = a.active_lin;
+                    sp.bc_infinity = f;
+                    const Ends en = {e[0], e[1], e[2], e[3], e[4], e[5]};
+                    double g1;
+                    ghost_values(sp, en, N, 1, &s_trk[p][slot], gval, g1);
+                }
             }
-            s_gh[p][v][j] = gval;
+            s_gh[p][gv][j] = gval;
         }
     };
 
-    // shift(X,-1) (lower) / shift(X,+1) of row (which, gv) of field v at cell i0+k
-    auto shifted = [&](bool lower, int which, int gv, int v, int k) __attribute__((always_inline)) -> T {
-        const int i = i0 + k;
-        const T* row = s_row[which][gv];
-        T r;
-        // the neighbour is this thread's cell k -+ 1, or the last / first cell of the
-        // neighbouring thread's chunk
-        if (lower) r = (i > 0) ? row[at_slot(k > 0 ? (k - 1) * TL + ty : (K - 1) * TL + ty - 1)]
-                               : T(s_gh[px][v][which]);
-        else r = (i < N - 1) ? row[at_slot(k + 1 < K ? (k + 1) * TL + ty : ty + 1)]
-                             : T(s_gh[px][v][2 + which]);
-        if (fade) {
-            const int mode = lower ? mlo : mhi;
-            if (mode == IMH_BC_FADE && (i < f || i >= N - f)) {
-                // constant-shifted end values: lower r[0]=a[0], r[N-1]=a[N-2];
-                // upper r[0]=a[1], r[N-1]=a[N-1]
-                const double v1 = double(row[idx(px, lower ? 0 : 1)]);
-                const double vN = double(row[idx(px, lower ? N - 2 : N - 1)]);
-                r = T(fluid_fade(double(r), i, N, v1, vN, a.fade, f));
-            }
+    // fade_shift.m blend of a shifted value at cell i (lower: shift(X,-1))
+    auto fade_fix = [&](bool lower, int which, int gv, int i, T r) __attribute__((always_inline)) -> T {
+        if constexpr (!GEN) return r;
+        const int mode = lower ? mlo : mhi;
+        if (mode == IMH_BC_FADE && (i < f || i >= N - f) && i < N) {
+            const double* e = s_ends[which][gv][px];
+            // constant-shifted end values: lower r[0]=a[0], r[N-1]=a[N-2];
+            // upper r[0]=a[1], r[N-1]=a[N-1]
+            const double v1 = lower ? e[0] : e[1];
+            const double vN = lower ? e[4] : e[5];
+            r = T(fluid_fade(double(r), i, N, v1, vN, a.fade, f));
         }
         return r;
     };
+    // lower / upper shifted values of a register row A at chunk cell k, from the
+    // neighbouring chunk's boundary value nb or the pencil-end ghost
+    auto lower_of = [&](const T (&A)[K], int k, T nb, T ghost) __attribute__((always_inline)) -> T {
+        if (k > 0) return A[k > 0 ? k - 1 : 0];
+        return has_left ? nb : ghost;
+    };
+    auto upper_of = [&](const T (&A)[K], int k, T nb, T ghost) __attribute__((always_inline)) -> T {
+        const int i = i0 + k;
+        const T in = (k + 1 < K) ? A[k + 1 < K ? k + 1 : k] : nb;
+        return i < N - 1 ? in : ghost;
+    };
 
-    T P[KMAX], U[KMAX], IR[KMAX];
+    T P[K], U[K], IR[K];
     // pressure, velocity and the freezing-speed candidate of every cell of state Q
-    auto cell_speeds = [&](const T (&Q)[5][KMAX]) __attribute__((always_inline)) -> T {
+    auto cell_speeds = [&](const T (&Q)[5][K]) __attribute__((always_inline)) -> T {
         T lmax = T(NAN);
 #pragma unroll
-        for (int k = 0; k < KMAX; ++k) {
-            P[k] = T(0);
-            U[k] = T(0);
-            IR[k] = T(1);
-            if (k < kc) {
-                T Pk, ck, ir;
-                M::press(Q[0][k], Q[1][k], Q[2][k], Q[3][k], Q[4][k], bm[0][k], bm[1][k],
-                         bm[2][k], a.use, g, Pk, ck, ir);
-                P[k] = Pk;
-                IR[k] = ir;
-                U[k] = M::divrho(sel3(d, Q[1][k], Q[2][k], Q[3][k]), ir);
-                T sp = fabs(U[k]) + ck;
-                lmax = (k == 0) ? sp : fmax(lmax, sp);
-            }
+        for (int k = 0; k < K; ++k) {
+            T Pk, ck, ir;
+            M::press(Q[0][k], Q[1][k], Q[2][k], Q[3][k], Q[4][k], bm[0][k], bm[1][k], bm[2][k],
+                     a.use, g, Pk, ck, ir);
+            P[k] = Pk;
+            IR[k] = ir;
+            U[k] = M::divrho(sel3(d, Q[1][k], Q[2][k], Q[3][k]), ir);
+            const T sp = fabs(U[k]) + ck;
+            lmax = fmax(lmax, valid(k) ? sp : T(NAN));  // NaN-ignoring, like MATLAB max
         }
         return lmax;
     };
 
-    // wFluidFlux.m: FR / FL of field v from state Q, freezing speed C
-    auto fluxes = [&](auto VC, const T (&Q)[5][KMAX], T C, T rC, T (&FR)[KMAX], T (&FL)[KMAX])
+    // wFluidFlux.m: FR / FL of field v from state Q and freezing speed C
+    auto fluxes = [&](auto VC, const T (&Q)[5][K], T C, T rC, T (&FR)[K], T (&FL)[K])
         __attribute__((always_inline)) {
         constexpr int v = decltype(VC)::value;
 #pragma unroll
-        for (int k = 0; k < KMAX; ++k) {
+        for (int k = 0; k < K; ++k) {
             const T bX = sel3(d, bm[0][k], bm[1][k], bm[2][k]);
             T w;
             if constexpr (v == 0) {
@@ -417,11 +421,12 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
             FL[k] = T(0.5) * (Q[v][k] - w);
         }
     };
-    // run fn(FieldIdx<v>, gv) for the fields of group [v0, v0 + G)
-    auto for_group = [&](int v0, auto fn) __attribute__((always_inline)) {
+    // fn(FieldIdx<v>, gv) for the fields of group [V0, V0 + G)
+    auto for_group = [&](auto V0C, auto fn) __attribute__((always_inline)) {
+        constexpr int V0 = decltype(V0C)::value;
         auto one = [&](auto VC) __attribute__((always_inline)) {
             constexpr int v = decltype(VC)::value;
-            if (v >= v0 && v < v0 + G) fn(VC, v - v0);
+            if constexpr (v >= V0 && v < V0 + G) fn(VC, v - V0);
         };
         one(FieldIdx<0>{});
         one(FieldIdx<1>{});
@@ -429,108 +434,150 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
         one(FieldIdx<3>{});
         one(FieldIdx<4>{});
     };
+    // fn(FieldIdx<v0>) for every group start v0 = 0, G, 2G, ...
+    auto for_groups = [&](auto fn) __attribute__((always_inline)) {
+        fn(FieldIdx<0>{});
+        if constexpr (G < 5) fn(FieldIdx<(G < 5 ? G : 0)>{});
+        if constexpr (2 * G < 5) fn(FieldIdx<(2 * G < 5 ? 2 * G : 0)>{});
+        if constexpr (3 * G < 5) fn(FieldIdx<(3 * G < 5 ? 3 * G : 0)>{});
+        if constexpr (4 * G < 5) fn(FieldIdx<(4 * G < 5 ? 4 * G : 0)>{});
+    };
 
     // ================= predictor: W and C from U^n (wFluidFlux.m) ====================
     const T C = pencil_reduce_max(cell_speeds(q), a, tid, px, s_red[0]);
     const T rC = M::rcp(C);
     const T hfC = T(hf) * C;
-    T qh[5][KMAX];
-#pragma unroll
-    for (int v0 = 0; v0 < 5; v0 += G) {
-        const int ng = (5 - v0) < G ? (5 - v0) : G;
-        if (v0 > 0) __syncthreads();  // the previous group's rows have been read
-        for_group(v0, [&](auto VC, int gv) __attribute__((always_inline)) {
+    T qh[5][K];
+    for_groups([&](auto V0C) __attribute__((always_inline)) {
+        constexpr int V0 = decltype(V0C)::value;
+        constexpr int NG = (5 - V0) < G ? (5 - V0) : G;
+        if constexpr (V0 > 0) __syncthreads();  // the previous group's values have been read
+        T FR[G][K], FL[G][K];
+        for_group(V0C, [&](auto VC, int gv) __attribute__((always_inline)) {
             constexpr int v = decltype(VC)::value;
-            T FR[KMAX], FL[KMAX];
-            fluxes(VC, q, C, rC, FR, FL);
-            init_faces(OBJ_SFR + v, FR);
-            init_faces(OBJ_SFL + v, FL);
-            put(0, gv, FR);
-            put(1, gv, FL);
+            fluxes(VC, q, C, rC, FR[v - V0], FL[v - V0]);
+            init_faces(OBJ_SFR + v, FR[v - V0]);
+            init_faces(OBJ_SFL + v, FL[v - V0]);
+            s_x[(v - V0) * 4 + 1][tid] = FR[v - V0][K - 1];
+            s_x[(v - V0) * 4 + 2][tid] = FL[v - V0][0];
+            put_ends(0, v - V0, FR[v - V0]);
+            put_ends(1, v - V0, FL[v - V0]);
         });
         __syncthreads();
-        ghost_phase(0, v0, ng);
+        ghost_phase(0, V0, NG);
         __syncthreads();
-        for_group(v0, [&](auto VC, int gv) __attribute__((always_inline)) {
+        for_group(V0C, [&](auto VC, int gv) __attribute__((always_inline)) {
             constexpr int v = decltype(VC)::value;
+            constexpr int gg = v - V0;
+            const T nbR = s_x[gg * 4 + 1][tl_];
+            const T nbL = s_x[gg * 4 + 2][tr_];
+            const T gR = T(s_gh[px][gg][0]), gL = T(s_gh[px][gg][3]);
 #pragma unroll
-            for (int k = 0; k < KMAX; ++k) {
-                qh[v][k] = T(1);
-                if (k < kc) {
-                    const T fr = s_row[0][gv][my_idx(k)], fl = s_row[1][gv][my_idx(k)];
-                    const T sR = shifted(true, 0, gv, v, k);
-                    const T sL = shifted(false, 1, gv, v, k);
-                    qh[v][k] = q[v][k] - hfC * (fr - sR + fl - sL);
+            for (int k = 0; k < K; ++k) {
+                T sR = lower_of(FR[gg], k, nbR, gR);
+                T sL = upper_of(FL[gg], k, nbL, gL);
+                if (fade) {
+                    sR = fade_fix(true, 0, gg, i0 + k, sR);
+                    sL = fade_fix(false, 1, gg, i0 + k, sL);
                 }
+                qh[v][k] = q[v][k] - hfC * (FR[gg][k] - sR + FL[gg][k] - sL);
             }
         });
-    }
+    });
     // mass.store.array = max(mass.store.array, MINMASS)
 #pragma unroll
-    for (int k = 0; k < KMAX; ++k) qh[0][k] = fmax(qh[0][k], T(a.minmass));
+    for (int k = 0; k < K; ++k) qh[0][k] = fmax(qh[0][k], T(a.minmass));
 
     // ================= corrector: W' and C' from U^(n+1/2) ===========================
     const T C2 = pencil_reduce_max(cell_speeds(qh), a, tid, px, s_red[1]);
     const T rC2 = M::rcp(C2);
     const T ffC = T(ff) * C2;
     const int lim = a.limiter;
-#pragma unroll
-    for (int v0 = 0; v0 < 5; v0 += G) {
-        const int ng = (5 - v0) < G ? (5 - v0) : G;
-        if (v0 > 0) __syncthreads();
-        for_group(v0, [&](auto VC, int gv) __attribute__((always_inline)) {
+    for_groups([&](auto V0C) __attribute__((always_inline)) {
+        constexpr int V0 = decltype(V0C)::value;
+        constexpr int NG = (5 - V0) < G ? (5 - V0) : G;
+        if constexpr (V0 > 0) __syncthreads();
+        T FR[G][K], FL[G][K];
+        for_group(V0C, [&](auto VC, int gv) __attribute__((always_inline)) {
             constexpr int v = decltype(VC)::value;
-            T FR[KMAX], FL[KMAX];
-            fluxes(VC, qh, C2, rC2, FR, FL);
-            init_faces(OBJ_FR + v, FR);
-            init_faces(OBJ_FL + v, FL);
-            put(0, gv, FR);
-            put(1, gv, FL);
+            constexpr int gg = v - V0;
+            fluxes(VC, qh, C2, rC2, FR[gg], FL[gg]);
+            init_faces(OBJ_FR + v, FR[gg]);
+            init_faces(OBJ_FL + v, FL[gg]);
+            s_x[gg * 4 + 0][tid] = FR[gg][0];
+            s_x[gg * 4 + 1][tid] = FR[gg][K - 1];
+            s_x[gg * 4 + 2][tid] = FL[gg][0];
+            s_x[gg * 4 + 3][tid] = FL[gg][K - 1];
+            put_ends(0, gg, FR[gg]);
+            put_ends(1, gg, FL[gg]);
         });
         __syncthreads();
-        ghost_phase(1, v0, ng);
+        ghost_phase(1, V0, NG);
         __syncthreads();
-        T FR2[G][KMAX], FL2[G][KMAX];
-        for_group(v0, [&](auto VC, int gv) __attribute__((always_inline)) {
+        // limited fluxes, in place (each cell's update uses the values before it)
+        for_group(V0C, [&](auto VC, int gv) __attribute__((always_inline)) {
             constexpr int v = decltype(VC)::value;
+            constexpr int gg = v - V0;
+            const T nbRl = s_x[gg * 4 + 1][tl_], nbRr = s_x[gg * 4 + 0][tr_];
+            const T nbLl = s_x[gg * 4 + 3][tl_], nbLr = s_x[gg * 4 + 2][tr_];
+            const T gRlo = T(s_gh[px][gg][0]), gLlo = T(s_gh[px][gg][1]);
+            const T gRhi = T(s_gh[px][gg][2]), gLhi = T(s_gh[px][gg][3]);
+            T R2[K], L2[K];
 #pragma unroll
-            for (int k = 0; k < KMAX; ++k) {
-                FR2[gv][k] = FL2[gv][k] = T(0);
-                if (k < kc) {
-                    const T fr = s_row[0][gv][my_idx(k)], fl = s_row[1][gv][my_idx(k)];
-                    const T rm = shifted(true, 0, gv, v, k);
-                    const T rp = shifted(false, 0, gv, v, k);
-                    const T lm = shifted(true, 1, gv, v, k);
-                    const T lp = shifted(false, 1, gv, v, k);
-                    FR2[gv][k] = fr + limit(lim, fr - rm, rp - fr);
-                    FL2[gv][k] = fl + limit(lim, lm - fl, fl - lp);
+            for (int k = 0; k < K; ++k) {
+                T rm = lower_of(FR[gg], k, nbRl, gRlo), rp = upper_of(FR[gg], k, nbRr, gRhi);
+                T lm = lower_of(FL[gg], k, nbLl, gLlo), lp = upper_of(FL[gg], k, nbLr, gLhi);
+                if (fade) {
+                    rm = fade_fix(true, 0, gg, i0 + k, rm);
+                    rp = fade_fix(false, 0, gg, i0 + k, rp);
+                    lm = fade_fix(true, 1, gg, i0 + k, lm);
+                    lp = fade_fix(false, 1, gg, i0 + k, lp);
                 }
+                const T fr = FR[gg][k], fl = FL[gg][k];
+                R2[k] = fr + limit(lim, fr - rm, rp - fr);
+                L2[k] = fl + limit(lim, lm - fl, fl - lp);
+            }
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                FR[gg][k] = R2[k];
+                FL[gg][k] = L2[k];
             }
         });
-        __syncthreads();  // every thread has read the FR / FL rows
-        for_group(v0, [&](auto VC, int gv) __attribute__((always_inline)) {
-            put(0, gv, FR2[gv]);
-            put(1, gv, FL2[gv]);
+        __syncthreads();  // every thread has read this phase's boundary values and ends
+        for_group(V0C, [&](auto VC, int gv) __attribute__((always_inline)) {
+            constexpr int gg = decltype(VC)::value - V0;
+            s_x[gg * 4 + 1][tid] = FR[gg][K - 1];
+            s_x[gg * 4 + 2][tid] = FL[gg][0];
+            put_ends(0, gg, FR[gg]);
+            put_ends(1, gg, FL[gg]);
         });
         __syncthreads();
-        ghost_phase(2, v0, ng);
+        ghost_phase(2, V0, NG);
         __syncthreads();
-        for_group(v0, [&](auto VC, int gv) __attribute__((always_inline)) {
+        for_group(V0C, [&](auto VC, int gv) __attribute__((always_inline)) {
             constexpr int v = decltype(VC)::value;
+            constexpr int gg = v - V0;
+            const T nb2R = s_x[gg * 4 + 1][tl_];
+            const T nb2L = s_x[gg * 4 + 2][tr_];
+            const T g2R = T(s_gh[px][gg][0]), g2L = T(s_gh[px][gg][3]);
 #pragma unroll
-            for (int k = 0; k < KMAX; ++k) {
-                if (k < kc) {
-                    const T sR = shifted(true, 0, gv, v, k);
-                    const T sL = shifted(false, 1, gv, v, k);
-                    const int o = base + (i0 + k) * stride;
+            for (int k = 0; k < K; ++k) {
+                T sR = lower_of(FR[gg], k, nb2R, g2R);
+                T sL = upper_of(FL[gg], k, nb2L, g2L);
+                if (fade) {
+                    sR = fade_fix(true, 0, gg, i0 + k, sR);
+                    sL = fade_fix(false, 1, gg, i0 + k, sL);
+                }
+                if (valid(k)) {
+                    const unsigned o = off(k);
                     // U^n is re-read here (it is cache-resident) rather than kept live
-                    T qn = a.q[v][o] - ffC * (FR2[gv][k] - sR + FL2[gv][k] - sL);
+                    T qn = a.q[v][o] - ffC * (FR[gg][k] - sR + FL[gg][k] - sL);
                     if (v == 0) qn = fmax(qn, T(a.minmass));  // mass.array = max(mass, MINMASS)
                     a.q[v][o] = qn;
                 }
             }
         });
-    }
+    });
 
     // ---- write the edge trackers back -----------------------------------------------
     if (trk_lo || trk_hi) {
@@ -551,7 +598,7 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
 // tile list (largest tiles first), so hundreds of uneven simulations share one launch.
 // With tiles == nullptr it is the sweep of the single simulation args[0], one block per
 // tile, with an XCD-aware block-to-tile map (neighbouring pencils share an L2).
-template <typename T, int BS, int KMAX, bool MHD>
+template <typename T, int BS, int KMAX, bool MHD, bool GEN>
 __global__ void __launch_bounds__(BS) fluid_sweep_batch_kernel(const FluidArgs<T>* args,
                                                                const BatchTile* tiles,
                                                                int ntiles) {
@@ -566,7 +613,7 @@ __global__ void __launch_bounds__(BS) fluid_sweep_batch_kernel(const FluidArgs<T
         } else {
             tile = args[0].xcd_remap ? xcd_tile(blockIdx.x, gridDim.x) : (long long)blockIdx.x;
         }
-        fluid_sweep_body<T, BS, KMAX, MHD>(args[arg], tile);
+        fluid_sweep_body<T, BS, KMAX, MHD, GEN>(args[arg], tile);
         if (!tiles) break;
         __syncthreads();
     }
@@ -585,15 +632,18 @@ using FluidT = float;
 #endif
 
 hipError_t IMH_FLUID_LAUNCHER(IMH_FLUID_ESZ, IMH_FLUID_BS, IMH_FLUID_KMAX)(
-    const void* args, const BatchTile* tiles, int ntiles, int nblocks, bool mhd, hipStream_t st) {
+    const void* args, const BatchTile* tiles, int ntiles, int nblocks, bool mhd, bool general,
+    hipStream_t st) {
     const FluidArgs<FluidT>* a = (const FluidArgs<FluidT>*)args;
-    if (mhd)
-        hipLaunchKernelGGL((fluid_sweep_batch_kernel<FluidT, IMH_FLUID_BS, IMH_FLUID_KMAX, true>),
-                           dim3(nblocks), dim3(IMH_FLUID_BS), 0, st, a, tiles, ntiles);
-    else
-        hipLaunchKernelGGL((fluid_sweep_batch_kernel<FluidT, IMH_FLUID_BS, IMH_FLUID_KMAX, false>),
-                           dim3(nblocks), dim3(IMH_FLUID_BS), 0, st, a, tiles, ntiles);
-    return hipGetLastError();
+#define IMH_FL(M, G)                                                                          \
+    if (mhd == M && general == G) {                                                           \
+        hipLaunchKernelGGL((fluid_sweep_batch_kernel<FluidT, IMH_FLUID_BS, IMH_FLUID_KMAX, M, G>), \
+                           dim3(nblocks), dim3(IMH_FLUID_BS), 0, st, a, tiles, ntiles);          \
+        return hipGetLastError();                                                             \
+    }
+    IMH_FL(false, false) IMH_FL(true, false) IMH_FL(false, true) IMH_FL(true, true)
+#undef IMH_FL
+    return hipErrorInvalidValue;
 }
 
 }  // namespace imh

@@ -648,7 +648,7 @@ static int fluid_sweep_t(imh_ctx* c, int d, int force) {
     const int nt = int((a.npencil + t.PX - 1) / t.PX);
     ProfScope ps(c, IMH_K_FLUID, double(10 + (c->mhd_kernel ? 3 : 0)) * c->ncell * sizeof(T));
     hipError_t e = launch_fluid_batch((const FluidArgs<T>*)da, nullptr, nt, nt, t.bs, t.kmax,
-                                      c->mhd_kernel, c->st);
+                                      c->mhd_kernel, fluid_needs_general(a), c->st);
     if (e != hipSuccess) return hip_fail(e, "fluid sweep launch");
     c->fluid_init = true;
     return IMH_OK;

@@ -56,9 +56,18 @@ struct BatchTile {
     long long tile;
 };
 
+// general: the launch may include a fade boundary or the first sweep of a simulation
+// (the kernel form that handles them out of line); otherwise the lean form.
 template <typename T>
 hipError_t launch_fluid_batch(const FluidArgs<T>* args, const BatchTile* tiles, int ntiles,
-                              int nblocks, int bs, int kmax, bool mhd, hipStream_t st);
+                              int nblocks, int bs, int kmax, bool mhd, bool general,
+                              hipStream_t st);
+// whether a sweep of these arguments needs the general kernel form
+template <typename T>
+inline bool fluid_needs_general(const FluidArgs<T>& a) {
+    return a.init_edges || mode_at(a.modes, 0, a.d) == IMH_BC_FADE ||
+           mode_at(a.modes, 1, a.d) == IMH_BC_FADE;
+}
 
 // Compiled fluid sweep variants X(element size, block size, KMAX): FP64 keeps K <= KMAX
 // cells per thread in registers with KMAX 1..4 (8 for pencils of 2049..4096 cells, with
@@ -71,7 +80,7 @@ hipError_t launch_fluid_batch(const FluidArgs<T>* args, const BatchTile* tiles, 
 #define IMH_FLUID_LAUNCHER(E, B, K) IMH_FLUID_LAUNCHER_(E, B, K)
 #define IMH_FLUID_DECLARE(E, B, K)                                                         \
     hipError_t IMH_FLUID_LAUNCHER(E, B, K)(const void* args, const BatchTile* tiles, int ntiles, \
-                                           int nblocks, bool mhd, hipStream_t st);
+                                           int nblocks, bool mhd, bool general, hipStream_t st);
 IMH_FLUID_VARIANTS(IMH_FLUID_DECLARE)
 bool fluid_variant_compiled(int esz, int bs, int kmax);
 
