@@ -260,7 +260,19 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
     const bool trk_lo = (mlo == IMH_BC_TRANS || mlo == IMH_BC_WALL);
     const bool trk_hi = (mhi == IMH_BC_TRANS || mhi == IMH_BC_WALL);
     const bool fade = GEN && (mlo == IMH_BC_FADE || mhi == IMH_BC_FADE);
-    const bool owns_end = pen_ok && (i0 < 3 || i0 + K > N - 3) && i0 < N;
+    // circular and constant boundaries need only a[0] and a[N-1] of a row: the first value
+    // of the first chunk and the last valid value of the last chunk
+    const bool simple_bc = (mlo == IMH_BC_CIRC || mlo == IMH_BC_CONST) &&
+                           (mhi == IMH_BC_CIRC || mhi == IMH_BC_CONST);
+    const bool owns_end = !simple_bc && pen_ok && (i0 < 3 || i0 + K > N - 3) && i0 < N;
+    const int ty_last = (N - 1) / K;
+    const int kl = (N - i0 >= K) ? K - 1 : (N - i0 >= 1 ? N - i0 - 1 : 0);  // last valid cell
+    auto lastv = [&](const T (&A)[K]) __attribute__((always_inline)) -> T {
+        T r = A[0];
+#pragma unroll
+        for (int k = 1; k < K; ++k) r = (k == kl) ? A[k] : r;
+        return r;
+    };
 
     // ---- tracker prefetch (cooperative) ----------------------------------------------
     if ((trk_lo || trk_hi) && !a.init_edges) {
@@ -300,7 +312,7 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
     };
     // pencil ends of a row held by this thread's cells
     auto put_ends = [&](int which, int gv, const T (&A)[K]) __attribute__((always_inline)) {
-        if (!owns_end) return;
+        if (simple_bc || !owns_end) return;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const int gi = i0 + k;
@@ -324,11 +336,19 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
             if (pen0 + p >= a.npencil) continue;
             const int which = j & 1;           // 0 fluxR, 1 fluxL
             const int upper = j >> 1;
+            const int mode = upper ? mhi : mlo;
+            if (simple_bc) {
+                // circ: lower ghost a[N-1], upper a[0]; const: lower a[0], upper a[N-1]
+                const bool first = (mode == IMH_BC_CIRC) == (upper == 1);
+                const int t = first ? (a.xmajor ? p * TL : p)
+                                    : (a.xmajor ? p * TL + ty_last : ty_last * PX + p);
+                s_gh[p][gv][j] = double(s_x[gv * 4 + which * 2 + (first ? 0 : 1)][t]);
+                continue;
+            }
             const int role = kind == 0 ? 0 : 1 + which;
             const int slot = upper * 16 + role * 5 + v;
             const double* e = s_ends[which][gv][p];
             if (GEN && a.init_edges && kind < 2) s_trk[p][slot] = upper ? e[5] : e[0];
-            const int mode = upper ? mhi : mlo;
             double gval;
             if (mode == IMH_BC_CIRC) gval = upper ? e[0] : e[5];
             else if (mode == IMH_BC_CONST || mode == IMH_BC_FADE) gval = upper ? e[5] : e[0];
@@ -458,8 +478,10 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
             fluxes(VC, q, C, rC, FR[v - V0], FL[v - V0]);
             init_faces(OBJ_SFR + v, FR[v - V0]);
             init_faces(OBJ_SFL + v, FL[v - V0]);
-            s_x[(v - V0) * 4 + 1][tid] = FR[v - V0][K - 1];
+            s_x[(v - V0) * 4 + 0][tid] = FR[v - V0][0];
+            s_x[(v - V0) * 4 + 1][tid] = lastv(FR[v - V0]);
             s_x[(v - V0) * 4 + 2][tid] = FL[v - V0][0];
+            s_x[(v - V0) * 4 + 3][tid] = lastv(FL[v - V0]);
             put_ends(0, v - V0, FR[v - V0]);
             put_ends(1, v - V0, FL[v - V0]);
         });
@@ -505,9 +527,9 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
             init_faces(OBJ_FR + v, FR[gg]);
             init_faces(OBJ_FL + v, FL[gg]);
             s_x[gg * 4 + 0][tid] = FR[gg][0];
-            s_x[gg * 4 + 1][tid] = FR[gg][K - 1];
+            s_x[gg * 4 + 1][tid] = lastv(FR[gg]);
             s_x[gg * 4 + 2][tid] = FL[gg][0];
-            s_x[gg * 4 + 3][tid] = FL[gg][K - 1];
+            s_x[gg * 4 + 3][tid] = lastv(FL[gg]);
             put_ends(0, gg, FR[gg]);
             put_ends(1, gg, FL[gg]);
         });
@@ -546,8 +568,10 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
         __syncthreads();  // every thread has read this phase's boundary values and ends
         for_group(V0C, [&](auto VC, int gv) __attribute__((always_inline)) {
             constexpr int gg = decltype(VC)::value - V0;
-            s_x[gg * 4 + 1][tid] = FR[gg][K - 1];
+            s_x[gg * 4 + 0][tid] = FR[gg][0];
+            s_x[gg * 4 + 1][tid] = lastv(FR[gg]);
             s_x[gg * 4 + 2][tid] = FL[gg][0];
+            s_x[gg * 4 + 3][tid] = lastv(FL[gg]);
             put_ends(0, gg, FR[gg]);
             put_ends(1, gg, FL[gg]);
         });
