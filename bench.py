@@ -54,6 +54,9 @@ def parse():
                     help="also run one device copy of this many MiB (the reference kernel "
                          "for scaling HBM counters)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving stream")
+    ap.add_argument("--serve-requests", type=int, default=48)
+    ap.add_argument("--serve-rates", default="10,40,160", help="Poisson arrival rates (req/s)")
     ap.add_argument("--profile-steps", type=int, default=6)
     return ap.parse_args()
 
@@ -171,6 +174,58 @@ def cpu_leg(sims, args):
     return cu / t_used / 1e6, done, t_used, n_iter
 
 
+SERVE_2D = [64, 128, 256, 512, 1024]
+SERVE_3D = [64, 96, 128, 192]
+
+
+def serve_requests(n, seed):
+    """Solve requests of the serving stream: Sod, Orszag-Tang and Kelvin-Helmholtz on
+    64^2..1024^2 and Sedov on 64^3..192^3, each run to a fixed end time, seeded ICs."""
+    from imogen_amd import problems
+    rng = np.random.default_rng(seed)
+    reqs, kinds = [], []
+    for i in range(n):
+        kind = rng.choice(["sod", "ot", "kh", "sedov"], p=[0.3, 0.3, 0.3, 0.1])
+        big = 10 ** 8
+        if kind == "sedov":
+            m = int(rng.choice(SERVE_3D, p=[0.4, 0.3, 0.2, 0.1]))
+            ini, st = problems.sedov(m, time_max=5e-3, iter_max=big)
+        else:
+            m = int(rng.choice(SERVE_2D, p=[0.3, 0.3, 0.2, 0.15, 0.05]))
+            if kind == "sod":
+                ini, st = problems.sod(m, ny=m, bc="const", iter_max=big, time_max=0.2)
+            elif kind == "ot":
+                ini, st = problems.orszag_tang(m, iter_max=big)
+                ini["timeMax"] = 0.5
+            else:
+                ini, st = problems.kelvin_helmholtz(m, seed=int(rng.integers(1 << 30)),
+                                                    iter_max=big)
+                ini["timeMax"] = 0.2
+        reqs.append((ini, st))
+        kinds.append("%s%d" % (kind, m))
+    return reqs, kinds
+
+
+def serving_leg(args):
+    """The serving stream at each arrival rate: latency p50 / p99 and Mcell-updates/s."""
+    from imogen_amd.serve import Server, run_stream
+    out = []
+    for k, rate in enumerate(float(r) for r in args.serve_rates.split(",")):
+        reqs, kinds = serve_requests(args.serve_requests, args.seed * 7919 + k)
+        rng = np.random.default_rng(args.seed * 104729 + k)
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, len(reqs)))
+        srv = Server("fp32", max_active=256)
+        rep = run_stream(srv, reqs, arrivals, iterations_per_round=4)
+        srv.close()
+        out.append({"rate_per_s": rate, "requests": len(reqs), "failed": len(rep["failed"]),
+                    "p50_s": round(rep["p50_s"], 4), "p99_s": round(rep["p99_s"], 4),
+                    "wall_s": round(rep["wall_s"], 3),
+                    "mcell_updates_per_s": round(rep["mcell_updates_per_s"], 1)})
+        log("serve rate %.0f/s: p50 %.3f s p99 %.3f s, %.0f Mcell-updates/s" % (
+            rate, rep["p50_s"], rep["p99_s"], rep["mcell_updates_per_s"]))
+    return out
+
+
 def large_plain_roofline(args):
     import imogen_amd
     from imogen_amd import problems
@@ -241,6 +296,15 @@ def main():
     if not args.no_large:
         log("plain fp64 sedov 256^3")
         out["plain_roofline"] = large_plain_roofline(args)
+
+    if not args.no_serve:
+        log("serving stream")
+        out["serving"] = {
+            "workload": "Poisson arrivals of Sod / Orszag-Tang / KH on 64^2..1024^2 and "
+                        "Sedov on 64^3..192^3, each to a fixed end time; FP32 bulk, "
+                        "HBM-budget admission, continuous batching",
+            "latency": "completion - arrival, host wall clock",
+            "runs": serving_leg(args)}
 
     if args.copy_probe:
         import ctypes
