@@ -97,23 +97,25 @@ __device__ __noinline__ void fluid_init_face(double* edges, const long long* eof
     }
 }
 
+// Max over each pencil of the block (NaN-ignoring): wave shuffles, then one LDS pass.
+// Takes the tile geometry by value: a reference to the argument block here would keep
+// the block's private copy in scratch memory.
 template <typename T>
-__device__ inline T pencil_reduce_max(T v, const FluidArgs<T>& a, int tid, int px, T* s_red) {
-    const int lane = tid & 63;
+__device__ __forceinline__ T pencil_reduce_max(T v, int xmajor, int TL, int PX, int tid, int px,
+                                               T* s_red) {
     const int wave = tid >> 6;
-    if (a.xmajor) {
-        int g = a.TL < 64 ? a.TL : 64;
+    if (xmajor) {
+        int g = TL < 64 ? TL : 64;
         for (int off = g >> 1; off >= 1; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
     } else {
-        for (int off = a.PX; off < 64; off <<= 1) v = fmax(v, __shfl_xor(v, off, 64));
+        for (int off = PX; off < 64; off <<= 1) v = fmax(v, __shfl_xor(v, off, 64));
     }
-    (void)lane;
     s_red[wave * PXMAX + px] = v;
     __syncthreads();
     T m = v;
     const int nw = blockDim.x >> 6;
-    if (a.xmajor) {
-        int w0 = (px * a.TL) >> 6, w1 = (px * a.TL + a.TL - 1) >> 6;
+    if (xmajor) {
+        int w0 = (px * TL) >> 6, w1 = (px * TL + TL - 1) >> 6;
         for (int w = w0; w <= w1; ++w) m = fmax(m, s_red[w * PXMAX + px]);
     } else {
         for (int w = 0; w < nw; ++w) m = fmax(m, s_red[w * PXMAX + px]);
@@ -261,17 +263,16 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
     const bool trk_hi = (mhi == IMH_BC_TRANS || mhi == IMH_BC_WALL);
     const bool fade = GEN && (mlo == IMH_BC_FADE || mhi == IMH_BC_FADE);
     // circular and constant boundaries need only a[0] and a[N-1] of a row: the first value
-    // of the first chunk and the last valid value of the last chunk
+    // of the first chunk and the last value of the last chunk
     const bool simple_bc = (mlo == IMH_BC_CIRC || mlo == IMH_BC_CONST) &&
                            (mhi == IMH_BC_CIRC || mhi == IMH_BC_CONST);
     const bool owns_end = !simple_bc && pen_ok && (i0 < 3 || i0 + K > N - 3) && i0 < N;
     const int ty_last = (N - 1) / K;
-    const int kl = (N - i0 >= K) ? K - 1 : (N - i0 >= 1 ? N - i0 - 1 : 0);  // last valid cell
-    auto lastv = [&](const T (&A)[K]) __attribute__((always_inline)) -> T {
-        T r = A[0];
+    // padding cells past the pencil end repeat the last cell (loads are clamped, and the
+    // half step and limited fluxes are copied forward), so a chunk's last value is a[N-1]
+    auto pad_fill = [&](T (&A)[K]) __attribute__((always_inline)) {
 #pragma unroll
-        for (int k = 1; k < K; ++k) r = (k == kl) ? A[k] : r;
-        return r;
+        for (int k = 1; k < K; ++k) A[k] = (i0 + k < N) ? A[k] : A[k - 1];
     };
 
     // ---- tracker prefetch (cooperative) ----------------------------------------------
@@ -464,7 +465,7 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
     };
 
     // ================= predictor: W and C from U^n (wFluidFlux.m) ====================
-    const T C = pencil_reduce_max(cell_speeds(q), a, tid, px, s_red[0]);
+    const T C = pencil_reduce_max(cell_speeds(q), a.xmajor, TL, PX, tid, px, s_red[0]);
     const T rC = M::rcp(C);
     const T hfC = T(hf) * C;
     T qh[5][K];
@@ -479,9 +480,9 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
             init_faces(OBJ_SFR + v, FR[v - V0]);
             init_faces(OBJ_SFL + v, FL[v - V0]);
             s_x[(v - V0) * 4 + 0][tid] = FR[v - V0][0];
-            s_x[(v - V0) * 4 + 1][tid] = lastv(FR[v - V0]);
+            s_x[(v - V0) * 4 + 1][tid] = FR[v - V0][K - 1];
             s_x[(v - V0) * 4 + 2][tid] = FL[v - V0][0];
-            s_x[(v - V0) * 4 + 3][tid] = lastv(FL[v - V0]);
+            s_x[(v - V0) * 4 + 3][tid] = FL[v - V0][K - 1];
             put_ends(0, v - V0, FR[v - V0]);
             put_ends(1, v - V0, FL[v - V0]);
         });
@@ -509,9 +510,11 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
     // mass.store.array = max(mass.store.array, MINMASS)
 #pragma unroll
     for (int k = 0; k < K; ++k) qh[0][k] = fmax(qh[0][k], T(a.minmass));
+#pragma unroll
+    for (int v = 0; v < 5; ++v) pad_fill(qh[v]);
 
     // ================= corrector: W' and C' from U^(n+1/2) ===========================
-    const T C2 = pencil_reduce_max(cell_speeds(qh), a, tid, px, s_red[1]);
+    const T C2 = pencil_reduce_max(cell_speeds(qh), a.xmajor, TL, PX, tid, px, s_red[1]);
     const T rC2 = M::rcp(C2);
     const T ffC = T(ff) * C2;
     const int lim = a.limiter;
@@ -527,9 +530,9 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
             init_faces(OBJ_FR + v, FR[gg]);
             init_faces(OBJ_FL + v, FL[gg]);
             s_x[gg * 4 + 0][tid] = FR[gg][0];
-            s_x[gg * 4 + 1][tid] = lastv(FR[gg]);
+            s_x[gg * 4 + 1][tid] = FR[gg][K - 1];
             s_x[gg * 4 + 2][tid] = FL[gg][0];
-            s_x[gg * 4 + 3][tid] = lastv(FL[gg]);
+            s_x[gg * 4 + 3][tid] = FL[gg][K - 1];
             put_ends(0, gg, FR[gg]);
             put_ends(1, gg, FL[gg]);
         });
@@ -559,6 +562,8 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
                 R2[k] = fr + limit(lim, fr - rm, rp - fr);
                 L2[k] = fl + limit(lim, lm - fl, fl - lp);
             }
+            pad_fill(R2);
+            pad_fill(L2);
 #pragma unroll
             for (int k = 0; k < K; ++k) {
                 FR[gg][k] = R2[k];
@@ -569,9 +574,9 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
         for_group(V0C, [&](auto VC, int gv) __attribute__((always_inline)) {
             constexpr int gg = decltype(VC)::value - V0;
             s_x[gg * 4 + 0][tid] = FR[gg][0];
-            s_x[gg * 4 + 1][tid] = lastv(FR[gg]);
+            s_x[gg * 4 + 1][tid] = FR[gg][K - 1];
             s_x[gg * 4 + 2][tid] = FL[gg][0];
-            s_x[gg * 4 + 3][tid] = lastv(FL[gg]);
+            s_x[gg * 4 + 3][tid] = FL[gg][K - 1];
             put_ends(0, gg, FR[gg]);
             put_ends(1, gg, FL[gg]);
         });
