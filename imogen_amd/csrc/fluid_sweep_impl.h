@@ -191,6 +191,56 @@ __device__ __forceinline__ T limit(int lim, T dl, T dr) {
     return lim_minmod(dl, dr);
 }
 
+// K consecutive cells as 16-byte (or one 8-byte) vector accesses; p is aligned to
+// K * sizeof(T) bytes.
+template <typename T, int K>
+__device__ __forceinline__ void chunk_load(const T* p, T (&out)[K]) {
+    constexpr int B = K * int(sizeof(T));
+    if constexpr (B % 16 == 0) {
+        constexpr int L = 16 / int(sizeof(T));
+        typedef T V __attribute__((ext_vector_type(L)));
+#pragma unroll
+        for (int j = 0; j < K / L; ++j) {
+            const V x = *reinterpret_cast<const V*>(p + j * L);
+#pragma unroll
+            for (int l = 0; l < L; ++l) out[j * L + l] = x[l];
+        }
+    } else if constexpr (B == 8 && K == 2) {
+        typedef T V __attribute__((ext_vector_type(2)));
+        const V x = *reinterpret_cast<const V*>(p);
+        out[0] = x[0];
+        out[1] = x[1];
+    } else {
+#pragma unroll
+        for (int k = 0; k < K; ++k) out[k] = p[k];
+    }
+}
+
+template <typename T, int K>
+__device__ __forceinline__ void chunk_store(T* p, const T (&in)[K]) {
+    constexpr int B = K * int(sizeof(T));
+    if constexpr (B % 16 == 0) {
+        constexpr int L = 16 / int(sizeof(T));
+        typedef T V __attribute__((ext_vector_type(L)));
+#pragma unroll
+        for (int j = 0; j < K / L; ++j) {
+            V x;
+#pragma unroll
+            for (int l = 0; l < L; ++l) x[l] = in[j * L + l];
+            *reinterpret_cast<V*>(p + j * L) = x;
+        }
+    } else if constexpr (B == 8 && K == 2) {
+        typedef T V __attribute__((ext_vector_type(2)));
+        V x;
+        x[0] = in[0];
+        x[1] = in[1];
+        *reinterpret_cast<V*>(p) = x;
+    } else {
+#pragma unroll
+        for (int k = 0; k < K; ++k) p[k] = in[k];
+    }
+}
+
 // Fields handled per exchange phase: their FR / FL live in registers across the phase's
 // barriers, so FP32 chunks of 8 cells (and FP64 of 4) take them in smaller groups.
 template <typename T, int KMAX>
@@ -289,15 +339,33 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
     }
 
     // ---- load the chunk (clamped at the pencil end: no branches) ----------------------
+    // x pencils whose length is a multiple of K move whole chunks as vectors; threads past
+    // the pencil end then read the last chunk (their values are never used)
+    const bool vec = a.xmajor && (N % K == 0);
+    const unsigned vbase = base + unsigned(i0 < N ? i0 : N - K);
     T q[5][K];
     T bm[3][K];
+    if (vec) {
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-        const unsigned o = off(k);
+        for (int v = 0; v < 5; ++v) chunk_load<T, K>(a.q[v] + vbase, q[v]);
+        if constexpr (MHD) {
 #pragma unroll
-        for (int v = 0; v < 5; ++v) q[v][k] = a.q[v][o];
+            for (int v = 0; v < 3; ++v) chunk_load<T, K>(a.bc[v] + vbase, bm[v]);
+        } else {
 #pragma unroll
-        for (int v = 0; v < 3; ++v) bm[v][k] = MHD ? a.bc[v][o] : T(0);
+            for (int v = 0; v < 3; ++v)
+#pragma unroll
+                for (int k = 0; k < K; ++k) bm[v][k] = T(0);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const unsigned o = off(k);
+#pragma unroll
+            for (int v = 0; v < 5; ++v) q[v][k] = a.q[v][o];
+#pragma unroll
+            for (int v = 0; v < 3; ++v) bm[v][k] = MHD ? a.bc[v][o] : T(0);
+        }
     }
 
     // First assignment of a flux object (InitializedArray.m:43-47): record its edges on
@@ -589,6 +657,7 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
             const T nb2R = s_x[gg * 4 + 1][tl_];
             const T nb2L = s_x[gg * 4 + 2][tr_];
             const T g2R = T(s_gh[px][gg][0]), g2L = T(s_gh[px][gg][3]);
+            T dq[K];
 #pragma unroll
             for (int k = 0; k < K; ++k) {
                 T sR = lower_of(FR[gg], k, nb2R, g2R);
@@ -597,12 +666,29 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
                     sR = fade_fix(true, 0, gg, i0 + k, sR);
                     sL = fade_fix(false, 1, gg, i0 + k, sL);
                 }
-                if (valid(k)) {
-                    const unsigned o = off(k);
-                    // U^n is re-read here (it is cache-resident) rather than kept live
-                    T qn = a.q[v][o] - ffC * (FR[gg][k] - sR + FL[gg][k] - sL);
-                    if (v == 0) qn = fmax(qn, T(a.minmass));  // mass.array = max(mass, MINMASS)
-                    a.q[v][o] = qn;
+                dq[k] = ffC * (FR[gg][k] - sR + FL[gg][k] - sL);
+            }
+            // U^n is re-read here (it is cache-resident) rather than kept live
+            if (vec) {
+                if (pen_ok && i0 < N) {
+                    T un[K];
+                    chunk_load<T, K>(a.q[v] + vbase, un);
+#pragma unroll
+                    for (int k = 0; k < K; ++k) {
+                        un[k] = un[k] - dq[k];
+                        if (v == 0) un[k] = fmax(un[k], T(a.minmass));  // mass = max(mass, MINMASS)
+                    }
+                    chunk_store<T, K>(a.q[v] + vbase, un);
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    if (valid(k)) {
+                        const unsigned o = off(k);
+                        T qn = a.q[v][o] - dq[k];
+                        if (v == 0) qn = fmax(qn, T(a.minmass));  // mass.array = max(mass, MINMASS)
+                        a.q[v][o] = qn;
+                    }
                 }
             }
         });
