@@ -55,6 +55,8 @@ def parse():
                          "for scaling HBM counters)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving stream")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="use the default tiles instead of the tuned table")
     ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--serve-rates", default="10,40,160", help="Poisson arrival rates (req/s)")
     ap.add_argument("--profile-steps", type=int, default=6)
@@ -112,9 +114,18 @@ def log(msg):
 
 
 def batched_leg(sims, args, cells):
+    from imogen_amd import autotune
     from imogen_amd.ensemble import Ensemble
+    tuned = 0
+    if not args.no_autotune:
+        table = autotune.load_table()
+        table = autotune.tune(sims, "fp32", table=table, log=log)
+        autotune.save_table(table, os.path.join(HERE, "gpurun_out", "tuned_tiles.json")
+                            if os.path.isdir(os.path.join(HERE, "gpurun_out")) else autotune.TABLE)
     ens = Ensemble(sims, "fp32")
-    log("batched: uploaded")
+    if not args.no_autotune:
+        tuned = autotune.apply(ens, sims, table, "fp32")
+    log("batched: uploaded, %d sweeps with tuned tiles" % tuned)
     ens.step(args.warmup)          # first iteration of each simulation + graph capture
     ens.synchronize()
     log("batched: warm")
@@ -254,6 +265,7 @@ def main():
           flush=True)
 
     ens, rate, t_batched = batched_leg(sims, args, cells)
+    tiles = "autotuned (imogen_amd/autotune.py)" if not args.no_autotune else "default"
     print("batched fp32: %.1f Mcell-updates/s (%.3f s)" % (rate, t_batched), file=sys.stderr,
           flush=True)
     rho32 = [ens.download(i, "mass") for i in range(len(sims))]
@@ -270,7 +282,8 @@ def main():
                                "seed %d" % (args.riemann, args.kh, KH_SIDES, args.seed),
                    "simulations": len(sims), "cells": int(cells.sum()),
                    "cell_updates_per_step": int(2 * cells.sum()),
-                   "parallelism": "single GPU, one persistent launch per kernel kind"},
+                   "parallelism": "single GPU, one persistent launch per kernel kind",
+                   "tiles": tiles},
     }
 
     if not args.no_plain:

@@ -24,8 +24,8 @@ COMMON = [
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
 
-SOURCES = ["magnet_dispatch.hip", "cfl.hip", "fluid_dispatch.cpp", "imh_api.cpp", "batch.cpp",
-           "serve.cpp"]
+SOURCES = ["magnet_dispatch.hip", "cfl.hip", "fluid_dispatch.cpp", "fluid_column.hip",
+           "imh_api.cpp", "batch.cpp", "serve.cpp"]
 VARIANT_SRC = "fluid_sweep_variant.hip"
 MAG_VARIANT_SRC = "magnet_sweep_variant.hip"
 

@@ -285,17 +285,21 @@ static int build_plans(imh_batch* b) {
                             const int dX = X - 1;
                             const bool gen = c->cfg.bc[0][dX] == IMH_BC_FADE ||
                                              c->cfg.bc[1][dX] == IMH_BC_FADE;
+                            // bs 0: the column march (one thread per pencil)
+                            const bool col = ft.column && !gen;
+                            const int gbs = col ? 0 : ft.bs, gkm = col ? 0 : ft.kmax;
                             G* g = nullptr;
                             for (G& gg : groups)
-                                if (gg.bs == ft.bs && gg.kmax == ft.kmax && gg.mhd == c->mhd_kernel &&
+                                if (gg.bs == gbs && gg.kmax == gkm && gg.mhd == c->mhd_kernel &&
                                     gg.general == gen)
                                     g = &gg;
                             if (!g) {
-                                groups.push_back({ft.bs, ft.kmax, c->mhd_kernel, gen, 0.0, {}});
+                                groups.push_back({gbs, gkm, c->mhd_kernel, gen, 0.0, {}});
                                 g = &groups.back();
                             }
                             g->bytes += double(10 + (c->mhd_kernel ? 3 : 0)) * c->ncell * es;
-                            long long ntile = (c->nface[X - 1] + ft.PX - 1) / ft.PX;
+                            const long long per = col ? fluid_column_pencils() : ft.PX;
+                            long long ntile = (c->nface[X - 1] + per - 1) / per;
                             for (long long tt = 0; tt < ntile; ++tt)
                                 g->tl.push_back({c->n[X - 1], {i * 3 + (X - 1), 0, tt}});
                         } else if (!fluid_op && c->cfg.magnet_active) {
@@ -398,6 +402,9 @@ static hipError_t run_launch(imh_batch* b, const Launch& L) {
     case L_CFL: return launch_cfl_batch<T>((const CflArgs<T>*)b->cflargs, tl, L.ntiles, nb, b->st);
     case L_FINALIZE: return launch_time_finalize(b->sims_dev, b->pars_dev, n, L.step, b->st);
     case L_FLUID:
+        if (L.bs == 0)
+            return launch_fluid_column<T>((const FluidArgs<T>*)b->fargs, tl, L.ntiles, nb, L.mhd,
+                                          b->st);
         return launch_fluid_batch<T>((const FluidArgs<T>*)b->fargs, tl, L.ntiles, nb, L.bs, L.kmax,
                                      L.mhd, L.general, b->st);
     case L_MAG_ADV:

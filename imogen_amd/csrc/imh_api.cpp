@@ -113,6 +113,7 @@ bool fluid_variant_exists(int esz, int bs, int kmax) { return fluid_variant_comp
 FluidTile choose_fluid_tile(int N, int d, int esz, const TileOverride* ov) {
     FluidTile t;
     t.xmajor = (d == 0);
+    t.column = 0;
     if (ov && ov->valid) {
         t.bs = ov->bs;
         t.kmax = ov->kmax;
@@ -153,6 +154,14 @@ FluidTile choose_fluid_tile(int N, int d, int esz, const TileOverride* ov) {
     t.K = (N + t.TL - 1) / t.TL;
     if (t.K < 1) t.K = 1;
     return t;
+}
+
+// The column march applies along y / z when both boundaries are circular or constant.
+int column_possible(const imh_config* cfg, int d) {
+    if (d == 0) return 0;
+    for (int s = 0; s < 2; ++s)
+        if (cfg->bc[s][d] != IMH_BC_CIRC && cfg->bc[s][d] != IMH_BC_CONST) return 0;
+    return 1;
 }
 
 MagTile choose_mag_tile(int N) {
@@ -410,6 +419,9 @@ int imh_create(const imh_config* cfg, imh_ctx** out) {
     CHK(hipMemcpy(c->par, &c->host_par, sizeof(SimParams), hipMemcpyHostToDevice), "par upload");
     for (int d = 0; d < 3; ++d) {
         c->ftile[d] = choose_fluid_tile(c->n[d], d, (int)es, nullptr);
+        // the column march has one thread per pencil: it pays where a sweep has many
+        // pencils (3-D grids); the autotuner can switch it on for a dimension
+        c->ftile[d].column = column_possible(cfg, d) && c->nface[d] >= (1 << 16);
         c->mtile[d] = choose_mag_tile(c->n[d]);
         if (c->n[d] >= 3 && c->ftile[d].K > c->ftile[d].kmax) {
             int maxn = c->ftile[d].kmax * 512;
@@ -645,10 +657,16 @@ static int fluid_sweep_t(imh_ctx* c, int d, int force) {
     const FluidTile& t = c->ftile[d];
     const void* da = imh_stage_args(c, &a, sizeof(a));
     if (!da) return IMH_EHIP;
-    const int nt = int((a.npencil + t.PX - 1) / t.PX);
     ProfScope ps(c, IMH_K_FLUID, double(10 + (c->mhd_kernel ? 3 : 0)) * c->ncell * sizeof(T));
-    hipError_t e = launch_fluid_batch((const FluidArgs<T>*)da, nullptr, nt, nt, t.bs, t.kmax,
-                                      c->mhd_kernel, fluid_needs_general(a), c->st);
+    hipError_t e;
+    if (t.column && !fluid_needs_general(a)) {
+        const int nt = int((a.npencil + fluid_column_pencils() - 1) / fluid_column_pencils());
+        e = launch_fluid_column((const FluidArgs<T>*)da, nullptr, nt, nt, c->mhd_kernel, c->st);
+    } else {
+        const int nt = int((a.npencil + t.PX - 1) / t.PX);
+        e = launch_fluid_batch((const FluidArgs<T>*)da, nullptr, nt, nt, t.bs, t.kmax,
+                               c->mhd_kernel, fluid_needs_general(a), c->st);
+    }
     if (e != hipSuccess) return hip_fail(e, "fluid sweep launch");
     c->fluid_init = true;
     return IMH_OK;
@@ -1014,6 +1032,17 @@ int imh_memory_estimate(const imh_config* cfg, size_t* bytes) {
 // Tuning hook (used by the autotuner): override the fluid tile of one sweep dim.
 int imh_set_fluid_tile(imh_ctx* c, int X, int bs, int kmax, int px) {
     if (!c || X < 1 || X > 3) return fail(IMH_EINVAL, "bad argument");
+    if (bs == 0) {  // the column march
+        if (!column_possible(&c->cfg, X - 1))
+            return fail(IMH_EINVAL, "the column march needs circular or constant boundaries "
+                        "along y or z");
+        c->ftile[X - 1].column = 1;
+        for (int i = 0; i < 6; ++i) {
+            if (c->graph[i]) hipGraphExecDestroy(c->graph[i]);
+            c->graph[i] = nullptr;
+        }
+        return IMH_OK;
+    }
     TileOverride ov = {1, bs, kmax, px};
     FluidTile t = choose_fluid_tile(c->n[X - 1], X - 1, (int)esize(c), &ov);
     if (t.K > t.kmax || !fluid_variant_exists((int)esize(c), t.bs, t.kmax) || (t.bs % t.PX) != 0 ||
@@ -1030,7 +1059,7 @@ int imh_set_fluid_tile(imh_ctx* c, int X, int bs, int kmax, int px) {
 
 int imh_get_fluid_tile(imh_ctx* c, int X, int* bs, int* kmax, int* px) {
     if (!c || X < 1 || X > 3) return fail(IMH_EINVAL, "bad argument");
-    *bs = c->ftile[X - 1].bs;
+    *bs = c->ftile[X - 1].column ? 0 : c->ftile[X - 1].bs;  // 0: the column march
     *kmax = c->ftile[X - 1].kmax;
     *px = c->ftile[X - 1].PX;
     return IMH_OK;

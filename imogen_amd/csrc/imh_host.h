@@ -30,6 +30,7 @@ int prof_collect(Profiler* p, double* ms, double* bytes, int* count);
 
 struct FluidTile {
     int bs, kmax, PX, TL, K, xmajor;
+    int column;  // sweep by the column march (fluid_column.hip) when the launch is lean
 };
 
 struct MagTile {
@@ -48,6 +49,7 @@ void sweep_order(int iteration, int order, int perms_order, int dirs[3], int mi[
 FluidTile choose_fluid_tile(int N, int d, int esz, const TileOverride* ov);
 bool fluid_variant_exists(int esz, int bs, int kmax);
 MagTile choose_mag_tile(int N);
+int column_possible(const imh_config* cfg, int d);
 
 }  // namespace imh
 

@@ -62,6 +62,13 @@ template <typename T>
 hipError_t launch_fluid_batch(const FluidArgs<T>* args, const BatchTile* tiles, int ntiles,
                               int nblocks, int bs, int kmax, bool mhd, bool general,
                               hipStream_t st);
+// The column march for sweeps along y / z with circular or constant boundaries: one
+// thread per pencil, fluid_column_pencils() pencils per tile.
+template <typename T>
+hipError_t launch_fluid_column(const FluidArgs<T>* args, const BatchTile* tiles, int ntiles,
+                               int nblocks, bool mhd, hipStream_t st);
+int fluid_column_pencils();
+
 // whether a sweep of these arguments needs the general kernel form
 template <typename T>
 inline bool fluid_needs_general(const FluidArgs<T>& a) {
