@@ -395,7 +395,8 @@ template <typename T>
 static hipError_t run_launch(imh_batch* b, const Launch& L) {
     const int n = b->n;
     const BatchTile* tl = b->tiles + L.off;
-    const int nb = std::max(1, std::min(L.ntiles, b->ncu * 4));
+    // a multiple of 8 blocks: the XCD-range schedule (xcd_round_tile)
+    const int nb = ((std::max(1, std::min(L.ntiles, b->ncu * 4)) + 7) / 8) * 8;
     switch (L.kind) {
     case L_BEGIN: return launch_begin_iteration(b->sims_dev, b->pars_dev, n, 0, b->st);
     case L_CFL_RESET: return launch_cfl_reset(b->sims_dev, n, b->st);

@@ -63,7 +63,9 @@ constexpr long long CFL_CHUNK = 256 * 32;  // cells per tile of a batched launch
 template <typename T>
 __global__ void __launch_bounds__(256) cfl_batch_kernel(const CflArgs<T>* args,
                                                         const BatchTile* tiles, int ntiles) {
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (int r = 0;; ++r) {
+        const int t = tiles ? xcd_round_tile(r, ntiles) : int(blockIdx.x + r * gridDim.x);
+        if (t < 0 || t >= ntiles) break;
         const BatchTile bt = tiles ? tiles[t] : BatchTile{0, 0, (long long)t};
         const CflArgs<T>& a = args[bt.arg];
         long long c0 = bt.tile * a.chunk;

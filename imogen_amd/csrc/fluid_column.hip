@@ -294,18 +294,20 @@ template <typename T, bool MHD>
 __global__ void __launch_bounds__(COL_BS) fluid_column_kernel(const FluidArgs<T>* args,
                                                               const BatchTile* tiles,
                                                               int ntiles) {
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (int r = 0;; ++r) {
         int arg = 0;
-        long long tile = t;
+        long long tile;
         if (tiles) {
+            const int t = xcd_round_tile(r, ntiles);
+            if (t < 0) break;
             const BatchTile bt = tiles[t];
             arg = bt.arg;
             tile = bt.tile;
-        } else if (t != (int)blockIdx.x) {
-            break;
+        } else {
+            if (r > 0 || (int)blockIdx.x >= ntiles) break;
+            tile = blockIdx.x;
         }
         fluid_column_body<T, MHD>(args[arg], tile);
-        if (!tiles) break;
     }
 }
 

@@ -647,18 +647,20 @@ __global__ void __launch_bounds__(BS) fluid_sweep_batch_kernel(const FluidArgs<T
                                                                const BatchTile* tiles,
                                                                int ntiles) {
     // one call site of the body (it is large)
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (int r = 0;; ++r) {
         int arg = 0;
         long long tile;
         if (tiles) {
+            const int t = xcd_round_tile(r, ntiles);
+            if (t < 0) break;
             const BatchTile bt = tiles[t];
             arg = bt.arg;
             tile = bt.tile;
         } else {
+            if (r > 0 || (int)blockIdx.x >= ntiles) break;
             tile = args[0].xcd_remap ? xcd_tile(blockIdx.x, gridDim.x) : (long long)blockIdx.x;
         }
         fluid_sweep_body<T, BS, KMAX, MHD, GEN>(args[arg], tile);
-        if (!tiles) break;
         __syncthreads();
     }
 }

@@ -56,6 +56,25 @@ struct BatchTile {
     long long tile;
 };
 
+// Schedule of a persistent launch over a tile list: the list is cut into 8 contiguous
+// ranges, one per XCD (blocks are dispatched round-robin over the XCDs, block b runs on
+// XCD b % 8), and the blocks of an XCD walk its range together, so tiles that neighbour
+// in memory (adjacent pencils of one simulation) are in flight on the same L2.  Returns
+// the tile of round r of this block, or -1 when the block is done.  A grid that is not a
+// multiple of 8 falls back to the grid-stride order.
+__device__ __forceinline__ int xcd_round_tile(int r, int ntiles) {
+    const int G = gridDim.x, b = blockIdx.x;
+    if (G % 8 != 0) {
+        const int t = b + r * G;
+        return t < ntiles ? t : -1;
+    }
+    const int x = b & 7, j = b >> 3, per = G >> 3;
+    const int lo = int((long long)ntiles * x / 8);
+    const int hi = int((long long)ntiles * (x + 1) / 8);
+    const int t = lo + j + r * per;
+    return t < hi ? t : -1;
+}
+
 // general: the launch may include a fade boundary or the first sweep of a simulation
 // (the kernel form that handles them out of line); otherwise the lean form.
 template <typename T>

@@ -569,15 +569,16 @@ __device__ __forceinline__ void cell_centered_body(const CellMagArgs<T>& a, long
 // tiles == nullptr: the single simulation args[0], one block per tile.
 // The (argument block, tile) a block works on in round t of a persistent launch; with
 // tiles == nullptr the launch is one simulation, args[0], one block per tile.
-__device__ inline bool next_tile(const BatchTile* tiles, int ntiles, int t, int& arg,
+__device__ inline bool next_tile(const BatchTile* tiles, int ntiles, int r, int& arg,
                                  long long& tile) {
-    if (t >= ntiles) return false;
     if (!tiles) {
-        if (t != (int)blockIdx.x) return false;
+        if (r > 0 || (int)blockIdx.x >= ntiles) return false;
         arg = 0;
-        tile = t;
+        tile = blockIdx.x;
         return true;
     }
+    const int t = xcd_round_tile(r, ntiles);
+    if (t < 0) return false;
     const BatchTile bt = tiles[t];
     arg = bt.arg;
     tile = bt.tile;
@@ -592,7 +593,7 @@ __global__ void __launch_bounds__(MAG_BS) magnet_advect_batch_kernel(const Magne
                                                                      int ntiles) {
     int arg;
     long long tile;
-    for (int t = blockIdx.x; next_tile(tiles, ntiles, t, arg, tile); t += gridDim.x) {
+    for (int r = 0; next_tile(tiles, ntiles, r, arg, tile); ++r) {
         magnet_advect_body<T, KMAX>(args[arg], tile);
         __syncthreads();
     }
@@ -604,7 +605,7 @@ __global__ void __launch_bounds__(MAG_BS) magnet_ct_batch_kernel(const MagnetArg
                                                                  int ntiles) {
     int arg;
     long long tile;
-    for (int t = blockIdx.x; next_tile(tiles, ntiles, t, arg, tile); t += gridDim.x) {
+    for (int r = 0; next_tile(tiles, ntiles, r, arg, tile); ++r) {
         magnet_ct_body<T, KMAX>(args[arg], tile);
         __syncthreads();
     }
@@ -616,7 +617,7 @@ __global__ void __launch_bounds__(256) cell_centered_batch_kernel(const CellMagA
                                                                   int ntiles) {
     int arg;
     long long tile;
-    for (int t = blockIdx.x; next_tile(tiles, ntiles, t, arg, tile); t += gridDim.x)
+    for (int r = 0; next_tile(tiles, ntiles, r, arg, tile); ++r)
         cell_centered_body<T>(args[arg], tile);
 }
 
