@@ -12,7 +12,7 @@ Printed as one JSON line:
   value            batched FP32-bulk Mcell-updates/s over the K timed steps
   plain_fp64       the same ensemble through the plain FP64 path, one simulation after
                    the other (imh_run), and the ratio batched / plain
-  error            L1 / L-inf of the FP32-bulk density against the FP64 path after the
+  error            L1 / L-inf of every FP32-bulk field against the FP64 path after the
                    same number of iterations (per simulation, max and median), and
                    against the CPU oracle on the cpu_baseline sample
   roofline         the dominant kernel of the batched step, timed with HIP events
@@ -21,6 +21,12 @@ Printed as one JSON line:
   plain_roofline   the same for the plain FP64 path on a 256^3 hydro grid
   cpu_baseline     the CPU oracle (oracle/, C FP64 restatement), single thread, on a
                    bounded sample of the ensemble
+
+roofline.traffic is the HBM traffic per launch of the dominant kernel measured by two
+rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) over this same workload, read from
+profiles/fluid_sweep_traffic.json (null when that file has no entry for the workload).
+`python bench.py --pmc-traffic FETCH.csv WRITE.csv BENCH.log` computes those entries from
+the counter_collection files of the two passes and the bench output of either pass.
 """
 import argparse
 import json
@@ -60,7 +66,130 @@ def parse():
     ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--serve-rates", default="10,40,160", help="Poisson arrival rates (req/s)")
     ap.add_argument("--profile-steps", type=int, default=6)
+    ap.add_argument("--pmc-traffic", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "BENCH_LOG"),
+                    help="offline: HBM bytes per launch from two rocprofv3 --pmc passes")
+    ap.add_argument("--trace-summary", nargs=2, metavar=("KERNEL_TRACE_CSV", "BENCH_LOG"),
+                    help="offline: the profiled passes of a bench run made under "
+                         "rocprofv3 --kernel-trace, as markdown tables")
     return ap.parse_args()
+
+
+def _bench_json(path):
+    out = None
+    for line in open(path):
+        if line.startswith("{"):
+            out = json.loads(line)
+    if out is None:
+        raise SystemExit("no bench JSON line in %s" % path)
+    return out
+
+
+def trace_summary(trace_csv, bench_log):
+    """Markdown tables of the profiled passes in a rocprofv3 kernel trace of a bench run:
+    the batched step (the last `roofline.launches` float fluid launches and every launch
+    between the iteration start before the first of them and the next double-precision
+    launch) and the plain 256^3 grid (the last `plain_roofline.launches` double fluid
+    launches and what runs between them), with the average fluid launch duration beside
+    the one bench.py measured with HIP events."""
+    import collections
+    import csv
+    out = _bench_json(bench_log)
+    rows = sorted(csv.DictReader(open(trace_csv)), key=lambda r: int(r["Start_Timestamp"]))
+
+    def short(n):
+        return n.split("(")[0].replace("void ", "").replace("imh::", "")
+
+    def table(sel, title, ref):
+        agg = collections.defaultdict(lambda: [0, 0])
+        for r in sel:
+            a = agg[short(r["Kernel_Name"])]
+            a[0] += 1
+            a[1] += int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+        tot = sum(v[1] for v in agg.values())
+        print("### %s\n" % title)
+        print("| kernel | launches | total us | avg us | share |\n|---|---|---|---|---|")
+        for k, (n, ns) in sorted(agg.items(), key=lambda t: -t[1][1]):
+            print("| %s | %d | %.1f | %.1f | %.1f%% |" % (k, n, ns / 1e3, ns / n / 1e3,
+                                                         100.0 * ns / tot))
+        fl = [(n, ns) for k, (n, ns) in agg.items() if k.startswith("fluid_sweep")]
+        n, ns = sum(x[0] for x in fl), sum(x[1] for x in fl)
+        print("\nfluid sweep: %d launches, average %.2f us per launch in the trace; "
+              "bench.py (HIP events): %.2f us over %d launches\n" % (
+                  n, ns / n / 1e3, ref["avg_launch_us"], ref["launches"]))
+
+    def is_fluid(r, prec):
+        return "fluid_sweep" in r["Kernel_Name"] and ("<%s" % prec) in r["Kernel_Name"]
+
+    n = out["roofline"]["launches"]
+    fl = [i for i, r in enumerate(rows) if is_fluid(r, "float")][-n:]
+    lo, hi = fl[0], fl[-1] + 1
+    while lo > 0 and "begin_iteration" not in rows[lo]["Kernel_Name"]:
+        lo -= 1
+    while hi < len(rows) and "<double" not in rows[hi]["Kernel_Name"] \
+            and "copy_probe" not in rows[hi]["Kernel_Name"] and not is_fluid(rows[hi], "float"):
+        hi += 1
+    table(rows[lo:hi], "batched step, profiled pass: " + out["config"]["workload"],
+          out["roofline"])
+    if "plain_roofline" in out:
+        n = out["plain_roofline"]["launches"]
+        fl = [i for i, r in enumerate(rows) if is_fluid(r, "double")][-n:]
+        table(rows[fl[0]:fl[-1] + 1], "plain path, profiled pass: " +
+              out["plain_roofline"]["workload"], out["plain_roofline"])
+    return 0
+
+
+TRAFFIC_FILE = os.path.join(HERE, "profiles", "fluid_sweep_traffic.json")
+
+
+def measured_traffic(workload):
+    """HBM bytes per launch of the dominant kernel for `workload`, if measured."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as f:
+        e = json.load(f).get(workload)
+    return None if e is None else int(e["hbm_bytes_per_launch"])
+
+
+def pmc_traffic(fetch_csv, write_csv, bench_log):
+    """HBM bytes per launch of the fluid sweep in the profiled passes of a bench run made
+    under `rocprofv3 --pmc FETCH_SIZE` and `--pmc WRITE_SIZE` (run with --copy-probe MIB).
+
+    FETCH_SIZE / WRITE_SIZE (KiB per dispatch) are scaled by what they read for the copy
+    probe, which reads and writes a known number of bytes.  The profiled pass of the
+    batched step is the last `roofline.launches` float fluid launches; that of the plain
+    256^3 grid the last `plain_roofline.launches` double ones."""
+    import csv
+    out = _bench_json(bench_log)
+    probe = out["copy_probe"]["bytes_read"]
+
+    def load(path, counter):
+        rows = [r for r in csv.DictReader(open(path)) if r["Counter_Name"] == counter]
+        rows.sort(key=lambda r: int(r["Dispatch_Id"]))
+        return rows
+
+    res = {}
+    for path, counter in ((fetch_csv, "FETCH_SIZE"), (write_csv, "WRITE_SIZE")):
+        rows = load(path, counter)
+        cp = [r for r in rows if "copy_probe" in r["Kernel_Name"]]
+        scale = probe / (float(cp[-1]["Counter_Value"]) * 1024.0)
+        for key, prec, n in (("roofline", "float", out["roofline"]["launches"]),
+                             ("plain_roofline", "double",
+                              out.get("plain_roofline", {}).get("launches", 0))):
+            if not n:
+                continue
+            fl = [r for r in rows if "fluid_sweep" in r["Kernel_Name"]
+                  and ("<%s" % prec) in r["Kernel_Name"]][-n:]
+            kib = sum(float(r["Counter_Value"]) for r in fl)
+            e = res.setdefault(out[key]["workload"] if key == "plain_roofline"
+                               else out["config"]["workload"], {"launches": n})
+            e[counter.lower() + "_scale"] = round(scale, 4)
+            e[counter.lower() + "_bytes_per_launch"] = int(kib * 1024.0 * scale / n)
+            e["alg_bytes_per_launch"] = out[key]["alg_bytes_per_launch"]
+    for e in res.values():
+        e["hbm_bytes_per_launch"] = e["fetch_size_bytes_per_launch"] + e["write_size_bytes_per_launch"]
+        e["traffic_over_alg"] = round(e["hbm_bytes_per_launch"] / e["alg_bytes_per_launch"], 3)
+    print(json.dumps(res, indent=1, sort_keys=True))
+    return 0
 
 
 def ensemble(seed, n_riemann, n_kh):
@@ -89,6 +218,34 @@ def rel_l1(a, ref):
 
 def rel_linf(a, ref):
     return float(np.max(np.abs(a - ref)) / np.max(np.abs(ref)))
+
+
+FIELDS = ["mass", "momx", "momy", "momz", "ener", "magx", "magy", "magz"]
+
+
+def field_errors(pairs):
+    """Normwise relative L1 and L-inf error of every field over (result, reference) pairs
+    of per-simulation {field: array}: the largest and the median over the simulations in
+    which the reference field is not identically zero; where it is, the largest |result|
+    is reported as `abs_where_ref_zero`."""
+    out = {}
+    for f in FIELDS:
+        l1, li, az = [], [], 0.0
+        for a, b in pairs:
+            x, y = a[f], b[f]
+            if np.any(y != 0):
+                l1.append(rel_l1(x, y))
+                li.append(rel_linf(x, y))
+            else:
+                az = max(az, float(np.max(np.abs(x))))
+        e = {"sims": len(l1)}
+        if l1:
+            e.update(l1_max=float("%.3e" % max(l1)), l1_median=float("%.3e" % np.median(l1)),
+                     linf_max=float("%.3e" % max(li)))
+        if len(l1) < len(pairs):
+            e["abs_where_ref_zero"] = float("%.3e" % az)
+        out[f] = e
+    return out
 
 
 def roofline(prof, label):
@@ -120,8 +277,10 @@ def batched_leg(sims, args, cells):
     if not args.no_autotune:
         table = autotune.load_table()
         table = autotune.tune(sims, "fp32", table=table, log=log)
-        autotune.save_table(table, os.path.join(HERE, "gpurun_out", "tuned_tiles.json")
-                            if os.path.isdir(os.path.join(HERE, "gpurun_out")) else autotune.TABLE)
+        gout = os.path.join(HERE, "gpurun_out")
+        autotune.save_table(table, os.path.join(gout, "tuned_tiles.json")
+                            if os.path.isdir(gout) and not os.environ.get("IMH_TILE_TABLE")
+                            else autotune.TABLE)
     ens = Ensemble(sims, "fp32")
     if not args.no_autotune:
         tuned = autotune.apply(ens, sims, table, "fp32")
@@ -152,7 +311,7 @@ def plain_leg(sims, args):
         run.synchronize()
         total_t += time.perf_counter() - t0
         total_cu += 2.0 * ncells(ini) * args.steps
-        rho.append(run.download("mass"))
+        rho.append({f: run.download(f) for f in FIELDS})
         run.close()
     return total_cu / total_t / 1e6, rho, total_t
 
@@ -178,7 +337,7 @@ def cpu_leg(sims, args):
             orc.iteration()
         t_used += time.perf_counter() - t0
         cu += 2.0 * ncells(ini) * n_iter
-        done.append((i, orc.get("mass")))
+        done.append((i, {f: orc.get(f) for f in FIELDS}))
         orc.close()
         if t_used >= args.cpu_seconds:
             break
@@ -252,6 +411,10 @@ def large_plain_roofline(args):
 
 def main():
     args = parse()
+    if args.pmc_traffic:
+        return pmc_traffic(*args.pmc_traffic)
+    if args.trace_summary:
+        return trace_summary(*args.trace_summary)
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (--gpus %d "
               "requested); run with --gpus 1" % args.gpus, file=sys.stderr)
@@ -268,7 +431,7 @@ def main():
     tiles = "autotuned (imogen_amd/autotune.py)" if not args.no_autotune else "default"
     print("batched fp32: %.1f Mcell-updates/s (%.3f s)" % (rate, t_batched), file=sys.stderr,
           flush=True)
-    rho32 = [ens.download(i, "mass") for i in range(len(sims))]
+    res32 = [{f: ens.download(i, f) for f in FIELDS} for i in range(len(sims))]
 
     out = {
         "metric": "Mcell-updates/s batched FP32 vs plain FP64 @L1<=1e-4; p50/p99 solve latency",
@@ -287,16 +450,18 @@ def main():
     }
 
     if not args.no_plain:
-        prate, rho64, t_plain = plain_leg(sims, args)
-        l1 = [rel_l1(a, b) for a, b in zip(rho32, rho64)]
-        li = [rel_linf(a, b) for a, b in zip(rho32, rho64)]
+        prate, res64, t_plain = plain_leg(sims, args)
+        l1 = [rel_l1(a["mass"], b["mass"]) for a, b in zip(res32, res64)]
+        li = [rel_linf(a["mass"], b["mass"]) for a, b in zip(res32, res64)]
         out["plain_fp64"] = {"value": round(prate, 2), "unit": "Mcell-updates/s",
                              "seconds": round(t_plain, 3),
                              "ratio_batched_over_plain": round(rate / prate, 2)}
         out["error"] = {"density_l1_vs_fp64_max": float("%.3e" % max(l1)),
                         "density_l1_vs_fp64_median": float("%.3e" % np.median(l1)),
                         "density_linf_vs_fp64_max": float("%.3e" % max(li)),
-                        "iterations": args.warmup + 1 + args.steps}
+                        "iterations": args.warmup + 1 + args.steps,
+                        "fields_vs_fp64": field_errors(list(zip(res32, res64)))}
+        del res64
         print("plain fp64: %.1f Mcell-updates/s; max L1 %.2e" % (prate, max(l1)),
               file=sys.stderr, flush=True)
 
@@ -304,11 +469,13 @@ def main():
     prof = ens.profile_step(args.profile_steps)
     out["roofline"] = roofline(prof, "batched ensemble FP32, persistent launches")
     out["roofline"]["bytes_rule"] = BYTES_NOTE
+    out["roofline"]["traffic"] = measured_traffic(out["config"]["workload"])
     ens.close()
 
     if not args.no_large:
         log("plain fp64 sedov 256^3")
         out["plain_roofline"] = large_plain_roofline(args)
+        out["plain_roofline"]["traffic"] = measured_traffic(out["plain_roofline"]["workload"])
 
     if not args.no_serve:
         log("serving stream")
@@ -336,9 +503,10 @@ def main():
             "sample": "%d of the ensemble's simulations (%d cells), %d iterations each, "
                       "%.1f s on one host core" % (len(done), sum(ncells(sims[i][0]) for i, _ in done),
                                                    n_iter, t_cpu)}
-        l1o = [rel_l1(rho32[i], r) for i, r in done]
+        l1o = [rel_l1(res32[i]["mass"], r["mass"]) for i, r in done]
         out.setdefault("error", {})["density_l1_vs_oracle_max"] = float("%.3e" % max(l1o))
         out["error"]["oracle_sample"] = len(done)
+        out["error"]["fields_vs_oracle"] = field_errors([(res32[i], r) for i, r in done])
     print(json.dumps(out))
     return 0
 

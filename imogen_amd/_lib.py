@@ -9,7 +9,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libimogen_hip.so")
+# IMH_LIB_PATH selects another build of the library (one made with build.py --out)
+LIB_PATH = os.environ.get("IMH_LIB_PATH") or os.path.join(_HERE, "libimogen_hip.so")
 
 IMH_OK = 0
 STATUS = {-1: "IMH_EINVAL", -2: "IMH_EHIP", -3: "IMH_ENOMEM", -4: "IMH_ECFL",

@@ -16,7 +16,9 @@ from . import build as _build
 from .config import from_ini
 from .ensemble import Ensemble
 
-TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned_tiles.json")
+# IMH_TILE_TABLE selects another table (e.g. for a build with other kernel knobs)
+TABLE = os.environ.get("IMH_TILE_TABLE") or os.path.join(
+    os.path.dirname(os.path.abspath(__file__)), "tuned_tiles.json")
 
 
 def _pow2(n):

@@ -48,12 +48,16 @@ def magnet_variants():
 def units():
     """(source, object name, extra flags) of every translation unit."""
     out = [(s, os.path.splitext(s)[0], []) for s in SOURCES]
+    # FP64 units keep -ffp-contract=off (the FP64 kernels reproduce the oracle's rounding);
+    # FP32-bulk units may fuse multiply-adds
+    fp32 = ["-ffp-contract=fast"]
     for e, b, k in fluid_variants():
         out.append((VARIANT_SRC, "fluid_sweep_v%d_%d_%d" % (e, b, k),
-                    ["-DIMH_FLUID_ESZ=%d" % e, "-DIMH_FLUID_BS=%d" % b, "-DIMH_FLUID_KMAX=%d" % k]))
+                    ["-DIMH_FLUID_ESZ=%d" % e, "-DIMH_FLUID_BS=%d" % b, "-DIMH_FLUID_KMAX=%d" % k]
+                    + (fp32 if e == 4 else [])))
     for e, k in magnet_variants():
         out.append((MAG_VARIANT_SRC, "magnet_sweep_v%d_%d" % (e, k),
-                    ["-DIMH_MAG_ESZ=%d" % e, "-DIMH_MAG_KMAX=%d" % k]))
+                    ["-DIMH_MAG_ESZ=%d" % e, "-DIMH_MAG_KMAX=%d" % k] + (fp32 if e == 4 else [])))
     return out
 
 
@@ -64,10 +68,11 @@ def _deps_mtime():
     return max(os.path.getmtime(p) for p in paths)
 
 
-def _compile(unit, verbose=False, incremental=False):
+def _compile(unit, verbose=False, incremental=False, defines=(), build_dir=None):
     src, name, flags = unit
     path = os.path.join(CSRC, src)
-    obj = os.path.join(BUILD, name + ".o")
+    obj = os.path.join(build_dir or BUILD, name + ".o")
+    flags = list(flags) + ["-D" + d for d in defines]
     if incremental and os.path.exists(obj) and os.path.getmtime(obj) >= _deps_mtime():
         return obj, ""
     cmd = [HIPCC, "-x", "hip", "--offload-arch=" + ARCH] + COMMON + flags + ["-c", path, "-o", obj]
@@ -79,29 +84,43 @@ def _compile(unit, verbose=False, incremental=False):
     return obj, r.stderr
 
 
-def build(verbose=False, jobs=None, incremental=False):
+def build(verbose=False, jobs=None, incremental=False, out=None, defines=()):
     """Compile every translation unit and link libimogen_hip.so.  Recompiles everything
-    unless `incremental` (then only objects older than some source or header)."""
-    os.makedirs(BUILD, exist_ok=True)
+    unless `incremental` (then only objects older than some source or header).
+
+    `out` links the library elsewhere (loaded when IMH_LIB_PATH names it) and `defines`
+    (NAME=VALUE) set the compile-time tuning knobs of the kernels, e.g.
+    IMH_FLUID_WAVES_PER_EU or IMH_FLUID_GROUP (fluid_sweep_impl.h)."""
+    lib_path = os.path.abspath(out) if out else LIB
+    build_dir = BUILD if not out else os.path.join(
+        BUILD, "alt_" + os.path.splitext(os.path.basename(lib_path))[0])
+    os.makedirs(build_dir, exist_ok=True)
     us = units()
     # the long fluid variants first, so they overlap with the short units
     us.sort(key=lambda u: 0 if u[0] in (VARIANT_SRC, MAG_VARIANT_SRC) else 1)
     jobs = jobs or min(len(us), max(1, (os.cpu_count() or 2)))
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        results = list(ex.map(lambda u: _compile(u, verbose, incremental), us))
+        results = list(ex.map(lambda u: _compile(u, verbose, incremental, defines, build_dir), us))
     objs = []
     for obj, warn in results:
         objs.append(obj)
         if warn.strip() and verbose:
             print(warn, file=sys.stderr)
-    tmp = LIB + ".tmp"
+    tmp = lib_path + ".tmp"
     cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n%s\n%s" % (r.stdout, r.stderr))
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, lib_path)
+    return lib_path
 
 
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv, incremental="--incremental" in sys.argv))
+    import argparse
+    ap = argparse.ArgumentParser(description="build libimogen_hip.so for " + ARCH)
+    ap.add_argument("-v", action="store_true", help="print the compiler commands")
+    ap.add_argument("--incremental", action="store_true")
+    ap.add_argument("--out", help="library path (default: in-tree libimogen_hip.so)")
+    ap.add_argument("-D", dest="defines", action="append", default=[], metavar="NAME=VALUE")
+    a = ap.parse_args()
+    print(build(verbose=a.v, incremental=a.incremental, out=a.out, defines=a.defines))

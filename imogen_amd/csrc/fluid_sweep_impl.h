@@ -172,10 +172,26 @@ __device__ __forceinline__ void chunk_store(T* p, const T (&in)[K]) {
 
 // Fields handled per exchange phase: their FR / FL live in registers across the phase's
 // barriers, so FP32 chunks of 8 cells (and FP64 of 4) take them in smaller groups.
+// IMH_FLUID_GROUP (1, 2 or 5) overrides the choice at build time.
+#ifndef IMH_FLUID_GROUP
+#define IMH_FLUID_GROUP 0
+#endif
 template <typename T, int KMAX>
 constexpr int fluid_group() {
+    if (IMH_FLUID_GROUP > 0) return IMH_FLUID_GROUP;
     return (int(sizeof(T)) * KMAX <= 16) ? 5 : (int(sizeof(T)) * KMAX <= 32 ? 2 : 1);
 }
+
+// Waves per SIMD the register allocator aims for (0: as many registers as the block size
+// allows); a build-time knob.
+#ifndef IMH_FLUID_WAVES_PER_EU
+#define IMH_FLUID_WAVES_PER_EU 0
+#endif
+#if IMH_FLUID_WAVES_PER_EU > 0
+#define IMH_FLUID_OCCUPANCY __attribute__((amdgpu_waves_per_eu(IMH_FLUID_WAVES_PER_EU)))
+#else
+#define IMH_FLUID_OCCUPANCY
+#endif
 
 // The sweep of one tile (PX pencils) of one simulation.
 //
@@ -643,7 +659,7 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
 // With tiles == nullptr it is the sweep of the single simulation args[0], one block per
 // tile, with an XCD-aware block-to-tile map (neighbouring pencils share an L2).
 template <typename T, int BS, int KMAX, bool MHD, bool GEN>
-__global__ void __launch_bounds__(BS) fluid_sweep_batch_kernel(const FluidArgs<T>* args,
+__global__ void __launch_bounds__(BS) IMH_FLUID_OCCUPANCY fluid_sweep_batch_kernel(const FluidArgs<T>* args,
                                                                const BatchTile* tiles,
                                                                int ntiles) {
     // one call site of the body (it is large)
