@@ -135,17 +135,24 @@ FluidTile choose_fluid_tile(int N, int d, int esz, const TileOverride* ov) {
             if (tl < 8) tl = 8;
         }
         t.PX = t.bs / (tl < t.bs ? tl : t.bs);
-        // pencils longer than BS * KMAX of the preferred variant: one pencil per 512-thread
-        // block with the smallest compiled KMAX that holds it
-        const int need = (N + 511) / 512;
-        if (need > t.kmax) {
-            for (int k = need; k <= 64; ++k)
-                if (fluid_variant_exists(esz, 512, k)) {
-                    t.bs = 512;
-                    t.kmax = k;
-                    t.PX = 1;
-                    break;
-                }
+        // pencils longer than BS * KMAX of the preferred variant: one pencil per block,
+        // the compiled variant with the fewest cells per thread that holds it (the
+        // smaller block on a tie)
+        if ((N + t.bs - 1) / t.bs > t.kmax) {
+            int best_k = 1 << 30;
+            for (int bs : {512, 1024}) {
+                const int need = (N + bs - 1) / bs;
+                for (int k = need; k <= 64; ++k)
+                    if (fluid_variant_exists(esz, bs, k)) {
+                        if (k < best_k) {
+                            best_k = k;
+                            t.bs = bs;
+                            t.kmax = k;
+                            t.PX = 1;
+                        }
+                        break;
+                    }
+            }
         }
     }
     if (t.PX > 32) t.PX = 32;
@@ -424,7 +431,7 @@ int imh_create(const imh_config* cfg, imh_ctx** out) {
         c->ftile[d].column = column_possible(cfg, d) && c->nface[d] >= (1 << 16);
         c->mtile[d] = choose_mag_tile(c->n[d]);
         if (c->n[d] >= 3 && c->ftile[d].K > c->ftile[d].kmax) {
-            int maxn = c->ftile[d].kmax * 512;
+            int maxn = c->ftile[d].kmax * c->ftile[d].bs;
             ctx_free(c);
             return fail(IMH_ENOTSUP, "pencils of %d cells along dim %d exceed the longest "
                         "compiled sweep tile (%d cells at this precision)", cfg->grid[d], d + 1, maxn);

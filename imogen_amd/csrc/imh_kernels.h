@@ -101,7 +101,7 @@ inline bool fluid_needs_general(const FluidArgs<T>& a) {
 // variant is its own translation unit.
 #define IMH_FLUID_VARIANTS(X) \
     X(8, 256, 1) X(8, 256, 2) X(8, 512, 2) X(8, 512, 4) X(8, 512, 8) \
-    X(4, 256, 2) X(4, 256, 4) X(4, 512, 4) X(4, 512, 8)
+    X(4, 256, 2) X(4, 256, 4) X(4, 512, 4) X(4, 512, 8) X(4, 1024, 4)
 #define IMH_FLUID_LAUNCHER_(E, B, K) launch_fluid_v##E##_##B##_##K
 #define IMH_FLUID_LAUNCHER(E, B, K) IMH_FLUID_LAUNCHER_(E, B, K)
 #define IMH_FLUID_DECLARE(E, B, K)                                                         \
