@@ -107,12 +107,40 @@ def tune(sims, precision="fp32", copies_cells=2_000_000, steps=2, table=None, lo
     return table
 
 
-def apply(ens, sims, table, precision="fp32"):
+def _shared_single_pencil_tiles(sims, table, precision):
+    """Single-pencil sweeps (1-D simulations: one pencil per simulation along x) are
+    latency-bound: a launch over them lasts as long as its longest pencil, and every
+    distinct tile is a launch of its own.  So all single-pencil classes of a family
+    (precision, dimension, MHD, boundary kind) take the tuned tile of the family's
+    longest class present, and run as one launch.  Returns {class key: tile}."""
+    longest = {}
+    for ini, _ in sims:
+        for key, X, n, _, nface in sweep_classes(ini, precision):
+            if nface != 1 or key not in table:
+                continue
+            fam = key.split("_n")[0] + "_" + key.split("_", 3)[3].split("_", 1)[1]
+            if fam not in longest or n > longest[fam][0]:
+                longest[fam] = (n, key)
+    out = {}
+    for ini, _ in sims:
+        for key, X, n, _, nface in sweep_classes(ini, precision):
+            if nface != 1:
+                continue
+            fam = key.split("_n")[0] + "_" + key.split("_", 3)[3].split("_", 1)[1]
+            if fam in longest:
+                t = table[longest[fam][1]]
+                if int(t[0]) * int(t[1]) >= n:
+                    out[key] = t
+    return out
+
+
+def apply(ens, sims, table, precision="fp32", share_single_pencil=True):
     """Set the tuned tile of every sweep of every simulation of an Ensemble."""
+    shared = _shared_single_pencil_tiles(sims, table, precision) if share_single_pencil else {}
     n = 0
     for i, (ini, _) in enumerate(sims):
         for key, X, _, _, _ in sweep_classes(ini, precision):
-            t = table.get(key)
+            t = shared.get(key, table.get(key))
             if t is None:
                 continue
             ens.set_fluid_tile(i, X, int(t[0]), int(t[1]), int(t[2]))
