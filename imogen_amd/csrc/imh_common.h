@@ -329,19 +329,33 @@ __host__ __device__ inline T lim_delta(int lim, T dl, T dr) {
 // conventions d=0: f = j + ny*k, d=1: f = i + nx*k, d=2: f = i + nx*j.
 __host__ __device__ inline void pencil_base(const int n[3], int d, long long f, long long& base,
                                             long long& stride) {
-    long long nx = n[0], ny = n[1];
+    // a simulation holds fewer than 2^31 cells: 32-bit divisions (64-bit ones are long
+    // instruction sequences on the GPU)
+    const unsigned nx = unsigned(n[0]), ny = unsigned(n[1]), uf = unsigned(f);
     if (d == 0) {
-        long long j = f % ny, k = f / ny;
-        base = nx * (j + ny * k);
+        const unsigned j = uf % ny, k = uf / ny;
+        base = (long long)nx * (j + (long long)ny * k);
         stride = 1;
     } else if (d == 1) {
-        long long i = f % nx, k = f / nx;
-        base = i + nx * ny * k;
+        const unsigned i = uf % nx, k = uf / nx;
+        base = i + (long long)nx * ny * k;
         stride = nx;
     } else {
-        long long i = f % nx, j = f / nx;
-        base = i + nx * j;
-        stride = nx * ny;
+        const unsigned i = uf % nx, j = uf / nx;
+        base = i + (long long)nx * j;
+        stride = (long long)nx * ny;
+    }
+}
+
+// Coordinates (x, y, z) of the first cell of pencil f along d.
+__host__ __device__ inline void pencil_coords(const int n[3], int d, long long f, int c[3]) {
+    const unsigned nx = unsigned(n[0]), ny = unsigned(n[1]), uf = unsigned(f);
+    if (d == 0) {
+        c[0] = 0; c[1] = int(uf % ny); c[2] = int(uf / ny);
+    } else if (d == 1) {
+        c[0] = int(uf % nx); c[1] = 0; c[2] = int(uf / nx);
+    } else {
+        c[0] = int(uf % nx); c[1] = int(uf / nx); c[2] = 0;
     }
 }
 

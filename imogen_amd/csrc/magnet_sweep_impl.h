@@ -143,9 +143,11 @@ __device__ inline T transverse_shift(const T* arr, long long cell, int jI, int N
 }
 
 __device__ inline long long coord_of(long long cell, const int n[3], int d) {
-    const long long c0 = cell % n[0];
-    const long long c1 = (cell / n[0]) % n[1];
-    const long long c2 = cell / ((long long)n[0] * n[1]);
+    // fewer than 2^31 cells: 32-bit divisions
+    const unsigned uc = unsigned(cell), n0 = unsigned(n[0]), n1 = unsigned(n[1]);
+    const unsigned c0 = uc % n0;
+    const unsigned c1 = (uc / n0) % n1;
+    const unsigned c2 = uc / (n0 * n1);
     return d == 0 ? c0 : (d == 1 ? c1 : c2);
 }
 
@@ -199,7 +201,11 @@ __device__ __forceinline__ void magnet_advect_body(const MagnetArgs<T>& a, long 
     const int f = a.bc_inf;
 
     long long base = 0, stride = 1;
-    if (pen_ok) pencil_base(nn, dX, pen, base, stride);
+    int pc[3] = {0, 0, 0};  // coordinates of the pencil's first cell
+    if (pen_ok) {
+        pencil_base(nn, dX, pen, base, stride);
+        pencil_coords(nn, dX, pen, pc);
+    }
     const double dt = a.sim->dt;
     const double ff = 0.5 * dt / a.dxX;       // relaxingMagnet.m:47
     const double ff2 = 2.0 * ff;              // relaxingMagnet.m:64
@@ -268,7 +274,7 @@ __device__ __forceinline__ void magnet_advect_body(const MagnetArgs<T>& a, long 
         for (int k = 0; k < KMAX; ++k) {
             if (k < kc) {
                 long long cell = base + (long long)(i0 + k) * stride;
-                int jI = int(coord_of(cell, nn, dI));
+                const int jI = pc[dI];  // I != X: constant along the pencil
                 double* tm = nullptr;
                 double* tr = nullptr;
                 double em = 0, er = 0;
@@ -276,8 +282,8 @@ __device__ __forceinline__ void magnet_advect_body(const MagnetArgs<T>& a, long 
                 bool trkI = (mIlo == IMH_BC_TRANS || mIlo == IMH_BC_WALL) &&
                             jI == 0 && I_enabled;
                 if (trkI) {
-                    fI = face_index(nn, dI, int(coord_of(cell, nn, 0)), int(coord_of(cell, nn, 1)),
-                                    int(coord_of(cell, nn, 2)));
+                    fI = face_index(nn, dI, dX == 0 ? i0 + k : pc[0], dX == 1 ? i0 + k : pc[1],
+                                    dX == 2 ? i0 + k : pc[2]);
                     em = a.edges[a.eoff[((OBJ_PRIM + Xc) * 2 + 0) * 3 + dI] + fI];
                     er = a.edges[a.eoff[((OBJ_PRIM + 0) * 2 + 0) * 3 + dI] + fI];
                     tm = &em;
@@ -473,7 +479,11 @@ __device__ __forceinline__ void magnet_ct_body(const MagnetArgs<T>& a, long long
     const bool ghost_thread = pen_ok && ty == 0;
     const int f = a.bc_inf;
     long long base = 0, stride = 1;
-    if (pen_ok) pencil_base(nn, dX, pen, base, stride);
+    int pc[3] = {0, 0, 0};  // coordinates of the pencil's first cell
+    if (pen_ok) {
+        pencil_base(nn, dX, pen, base, stride);
+        pencil_coords(nn, dX, pen, pc);
+    }
     const double ffI = a.sim->dt / a.dxI;
     const int oFL = OBJ_MFLUX + 3 * dI + dX;
     const int mlo = mode_at(a.modes, 0, dX);
@@ -492,14 +502,14 @@ __device__ __forceinline__ void magnet_ct_body(const MagnetArgs<T>& a, long long
     for (int k = 0; k < KMAX; ++k) {
         if (k < kc) {
             long long cell = base + (long long)(i0 + k) * stride;
-            int jI = int(coord_of(cell, nn, dI));
+            const int jI = pc[dI];  // I != X: constant along the pencil
             double et = 0;
             double* tp = nullptr;
             long long fI = 0;
             bool onface = trkI && jI == NI - 1;
             if (onface) {
-                fI = face_index(nn, dI, int(coord_of(cell, nn, 0)), int(coord_of(cell, nn, 1)),
-                                int(coord_of(cell, nn, 2)));
+                fI = face_index(nn, dI, dX == 0 ? i0 + k : pc[0], dX == 1 ? i0 + k : pc[1],
+                                dX == 2 ? i0 + k : pc[2]);
                 et = a.edges[a.eoff[(oFL * 2 + 1) * 3 + dI] + fI];
                 tp = &et;
             }
