@@ -389,6 +389,28 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
         }
     };
 
+    // Ghost j (0 FR lower, 1 FL lower, 2 FR upper, 3 FL upper) of field gg of this
+    // thread's pencil after ghost_phase.  Circular and constant boundaries read it straight
+    // from the chunk-boundary exchange (ghost_phase and its barrier are skipped for them):
+    // circ lower a[N-1], upper a[0]; const lower a[0], upper a[N-1].
+    auto ghost_of = [&](int gg, int j) __attribute__((always_inline)) -> T {
+        if (simple_bc) {
+            const int which = j & 1, upper = j >> 1;
+            const int mode = upper ? mhi : mlo;
+            const bool first = (mode == IMH_BC_CIRC) == (upper == 1);
+            const int t = first ? (a.xmajor ? px * TL : px)
+                                : (a.xmajor ? px * TL + ty_last : ty_last * PX + px);
+            return s_x[gg * 4 + which * 2 + (first ? 0 : 1)][t];
+        }
+        return T(s_gh[px][gg][j]);
+    };
+    // the ghost phase of one exchange and its barrier (none for circular / constant)
+    auto ghosts = [&](int kind, int v0, int ng) __attribute__((always_inline)) {
+        if (simple_bc) return;
+        ghost_phase(kind, v0, ng);
+        __syncthreads();
+    };
+
     // fade_shift.m blend of a shifted value at cell i (lower: shift(X,-1))
     auto fade_fix = [&](bool lower, int which, int gv, int i, T r) __attribute__((always_inline)) -> T {
         if constexpr (!GEN) return r;
@@ -500,14 +522,13 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
             put_ends(1, v - V0, FL[v - V0]);
         });
         __syncthreads();
-        ghost_phase(0, V0, NG);
-        __syncthreads();
+        ghosts(0, V0, NG);
         for_group(V0C, [&](auto VC, int gv) __attribute__((always_inline)) {
             constexpr int v = decltype(VC)::value;
             constexpr int gg = v - V0;
             const T nbR = s_x[gg * 4 + 1][tl_];
             const T nbL = s_x[gg * 4 + 2][tr_];
-            const T gR = T(s_gh[px][gg][0]), gL = T(s_gh[px][gg][3]);
+            const T gR = ghost_of(gg, 0), gL = ghost_of(gg, 3);
 #pragma unroll
             for (int k = 0; k < K; ++k) {
                 T sR = lower_of(FR[gg], k, nbR, gR);
@@ -550,16 +571,15 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
             put_ends(1, gg, FL[gg]);
         });
         __syncthreads();
-        ghost_phase(1, V0, NG);
-        __syncthreads();
+        ghosts(1, V0, NG);
         // limited fluxes, in place (each cell's update uses the values before it)
         for_group(V0C, [&](auto VC, int gv) __attribute__((always_inline)) {
             constexpr int v = decltype(VC)::value;
             constexpr int gg = v - V0;
             const T nbRl = s_x[gg * 4 + 1][tl_], nbRr = s_x[gg * 4 + 0][tr_];
             const T nbLl = s_x[gg * 4 + 3][tl_], nbLr = s_x[gg * 4 + 2][tr_];
-            const T gRlo = T(s_gh[px][gg][0]), gLlo = T(s_gh[px][gg][1]);
-            const T gRhi = T(s_gh[px][gg][2]), gLhi = T(s_gh[px][gg][3]);
+            const T gRlo = ghost_of(gg, 0), gLlo = ghost_of(gg, 1);
+            const T gRhi = ghost_of(gg, 2), gLhi = ghost_of(gg, 3);
             T R2[K], L2[K];
 #pragma unroll
             for (int k = 0; k < K; ++k) {
@@ -594,14 +614,13 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
             put_ends(1, gg, FL[gg]);
         });
         __syncthreads();
-        ghost_phase(2, V0, NG);
-        __syncthreads();
+        ghosts(2, V0, NG);
         for_group(V0C, [&](auto VC, int gv) __attribute__((always_inline)) {
             constexpr int v = decltype(VC)::value;
             constexpr int gg = v - V0;
             const T nb2R = s_x[gg * 4 + 1][tl_];
             const T nb2L = s_x[gg * 4 + 2][tr_];
-            const T g2R = T(s_gh[px][gg][0]), g2L = T(s_gh[px][gg][3]);
+            const T g2R = ghost_of(gg, 0), g2L = ghost_of(gg, 3);
             T dq[K];
 #pragma unroll
             for (int k = 0; k < K; ++k) {
