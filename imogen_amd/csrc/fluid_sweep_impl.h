@@ -613,6 +613,19 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
             put_ends(0, gg, FR[gg]);
             put_ends(1, gg, FL[gg]);
         });
+        // U^n of the group's fields, re-read (cache-resident) rather than kept live, all
+        // loads issued before the exchange: the addresses are clamped into the pencil and
+        // the pencil range, so the loads need no guard (only the stores do)
+        T un_g[G][K];
+        for_group(V0C, [&](auto VC, int gv) __attribute__((always_inline)) {
+            constexpr int v = decltype(VC)::value;
+            if (vec) {
+                chunk_load<T, K>(a.q[v] + vbase, un_g[v - V0]);
+            } else {
+#pragma unroll
+                for (int k = 0; k < K; ++k) un_g[v - V0][k] = a.q[v][off(k)];
+            }
+        });
         __syncthreads();
         ghosts(2, V0, NG);
         for_group(V0C, [&](auto VC, int gv) __attribute__((always_inline)) {
@@ -632,28 +645,18 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
                 }
                 dq[k] = ffC * (FR[gg][k] - sR + FL[gg][k] - sL);
             }
-            // U^n is re-read here (it is cache-resident) rather than kept live
-            if (vec) {
-                if (pen_ok && i0 < N) {
-                    T un[K];
-                    chunk_load<T, K>(a.q[v] + vbase, un);
+            T un[K];
 #pragma unroll
-                    for (int k = 0; k < K; ++k) {
-                        un[k] = un[k] - dq[k];
-                        if (v == 0) un[k] = fmax(un[k], T(a.minmass));  // mass = max(mass, MINMASS)
-                    }
-                    chunk_store<T, K>(a.q[v] + vbase, un);
-                }
+            for (int k = 0; k < K; ++k) {
+                un[k] = un_g[gg][k] - dq[k];
+                if (v == 0) un[k] = fmax(un[k], T(a.minmass));  // mass.array = max(mass, MINMASS)
+            }
+            if (vec) {
+                if (pen_ok && i0 < N) chunk_store<T, K>(a.q[v] + vbase, un);
             } else {
 #pragma unroll
-                for (int k = 0; k < K; ++k) {
-                    if (valid(k)) {
-                        const unsigned o = off(k);
-                        T qn = a.q[v][o] - dq[k];
-                        if (v == 0) qn = fmax(qn, T(a.minmass));  // mass.array = max(mass, MINMASS)
-                        a.q[v][o] = qn;
-                    }
-                }
+                for (int k = 0; k < K; ++k)
+                    if (valid(k)) a.q[v][off(k)] = un[k];
             }
         });
     });
