@@ -66,9 +66,11 @@ __global__ void __launch_bounds__(256) cfl_batch_kernel(const CflArgs<T>* args,
     for (int r = 0;; ++r) {
         const int t = tiles ? xcd_round_tile(r, ntiles) : int(blockIdx.x + r * gridDim.x);
         if (t < 0 || t >= ntiles) break;
-        const BatchTile bt = tiles ? tiles[t] : BatchTile{0, 0, (long long)t};
-        const CflArgs<T>& a = args[bt.arg];
-        long long c0 = bt.tile * a.chunk;
+        int arg = 0;
+        long long tile = t;
+        if (tiles) uniform_tile(tiles, t, arg, tile);
+        const CflArgs<T> a = args[arg];  // private copy, read once per tile
+        long long c0 = tile * a.chunk;
         long long c1 = c0 + a.chunk < a.ncell ? c0 + a.chunk : a.ncell;
         cfl_body(a, c0, c1, 256);
     }

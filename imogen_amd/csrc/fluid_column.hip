@@ -300,9 +300,7 @@ __global__ void __launch_bounds__(COL_BS) fluid_column_kernel(const FluidArgs<T>
         if (tiles) {
             const int t = xcd_round_tile(r, ntiles);
             if (t < 0) break;
-            const BatchTile bt = tiles[t];
-            arg = bt.arg;
-            tile = bt.tile;
+            uniform_tile(tiles, t, arg, tile);
         } else {
             if (r > 0 || (int)blockIdx.x >= ntiles) break;
             tile = blockIdx.x;

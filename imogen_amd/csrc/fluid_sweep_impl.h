@@ -691,9 +691,7 @@ __global__ void __launch_bounds__(BS) IMH_FLUID_OCCUPANCY fluid_sweep_batch_kern
         if (tiles) {
             const int t = xcd_round_tile(r, ntiles);
             if (t < 0) break;
-            const BatchTile bt = tiles[t];
-            arg = bt.arg;
-            tile = bt.tile;
+            uniform_tile(tiles, t, arg, tile);
         } else {
             if (r > 0 || (int)blockIdx.x >= ntiles) break;
             tile = args[0].xcd_remap ? xcd_tile(blockIdx.x, gridDim.x) : (long long)blockIdx.x;

@@ -62,6 +62,19 @@ struct BatchTile {
 // in memory (adjacent pencils of one simulation) are in flight on the same L2.  Returns
 // the tile of round r of this block, or -1 when the block is done.  A grid that is not a
 // multiple of 8 falls back to the grid-stride order.
+// A tile-list entry read so that the compiler knows it is the same in every lane (the
+// block works on one tile): the argument-block index and tile go to scalar registers,
+// and the argument block's fields are then read as uniform values rather than as
+// per-lane loads that each cost a memory round trip.
+__device__ __forceinline__ void uniform_tile(const BatchTile* tiles, int t, int& arg,
+                                             long long& tile) {
+    const BatchTile bt = tiles[t];
+    arg = __builtin_amdgcn_readfirstlane(bt.arg);
+    const unsigned lo = __builtin_amdgcn_readfirstlane(unsigned(bt.tile & 0xffffffffll));
+    const unsigned hi = __builtin_amdgcn_readfirstlane(unsigned(bt.tile >> 32));
+    tile = (long long)(((unsigned long long)hi << 32) | lo);
+}
+
 __device__ __forceinline__ int xcd_round_tile(int r, int ntiles) {
     const int G = gridDim.x, b = blockIdx.x;
     if (G % 8 != 0) {

@@ -142,6 +142,27 @@ __device__ inline T transverse_shift(const T* arr, long long cell, int jI, int N
     return r;
 }
 
+// transverse_shift with arr[cell] (self) and the interior neighbour arr[cell -+ sI] (nb,
+// any value where the shift needs a ghost) already loaded: the callers issue those loads
+// for all their cells at once and use them unconditionally, so the compiler cannot sink
+// them into per-cell branches (where they would run one memory round trip at a time).
+template <typename T>
+__device__ inline T transverse_shift_pre(const T* arr, long long cell, int jI, int NI,
+                                         long long sI, bool enabled, const GhostSpec& s,
+                                         double* tracker, const double* fade, int f, T self,
+                                         T nb) {
+    if (!enabled) return self;
+    T r;
+    bool ghost = s.upper ? (jI >= NI - 1) : (jI <= 0);
+    if (!ghost) r = nb;
+    else r = T(transverse_ghost(arr, cell - (long long)jI * sI, NI, sI, s.mode, s.upper,
+                                s.flip_sign, s.zero_ghost, s.active_linear, s.bc_infinity,
+                                tracker));
+    if (s.mode == IMH_BC_FADE)
+        r = transverse_fade(r, arr, cell - (long long)jI * sI, jI, NI, sI, s.upper, fade, f);
+    return r;
+}
+
 __device__ inline long long coord_of(long long cell, const int n[3], int d) {
     // fewer than 2^31 cells: 32-bit divisions
     const unsigned uc = unsigned(cell), n0 = unsigned(n[0]), n1 = unsigned(n[1]);
@@ -177,7 +198,10 @@ __device__ __noinline__ void mag_init_face(double* edges, const long long* eoff,
 // Advection sub-step for component I along X (one pencil per PX group of threads).
 // ------------------------------------------------------------------------------------
 template <typename T, int KMAX>
-__device__ __forceinline__ void magnet_advect_body(const MagnetArgs<T>& a, long long tile) {
+__device__ __forceinline__ void magnet_advect_body(const MagnetArgs<T>& ap, long long tile) {
+    // private copy: one read of the argument block per tile; stores to the fields
+    // cannot alias it, so its fields are not re-read after every store
+    const MagnetArgs<T> a = ap;
     if (!a.force && !a.sim->active) return;
     extern __shared__ unsigned char smem_raw[];
     const int dX = a.dX, dI = a.dI;
@@ -270,11 +294,30 @@ __device__ __forceinline__ void magnet_advect_body(const MagnetArgs<T>& a, long 
         GhostSpec sm = {mIlo, 0, 0, 0, a.active_linI, f};  // FluidArray: no flips
         GhostSpec sr = sm;
         sm.flip_sign = (Xc == dI + 1);  // mom(X).component == I ?  (X != I, so never)
+        // every load first, at addresses clamped into the pencil (padding cells read the
+        // pencil's last cell; threads without a pencil read from cell 0)
+        const int jI = pc[dI];  // I != X: constant along the pencil
+        const long long nbo = (I_enabled && jI > 0) ? -sI : 0;  // interior lower neighbour
+        T m0[KMAX], r0[KMAX], mn[KMAX], rn[KMAX];
 #pragma unroll
         for (int k = 0; k < KMAX; ++k) {
-            if (k < kc) {
+            const int i = i0 + k < N - 1 ? i0 + k : N - 1;
+            const long long cell = base + (long long)i * stride;
+            m0[k] = a.mom[cell];
+            r0[k] = a.rho[cell];
+            mn[k] = a.mom[cell + nbo];
+            rn[k] = a.rho[cell + nbo];
+            bI[k] = a.bI[cell];
+        }
+        // the shifted values: interior neighbours unless this pencil's row needs the BC
+        // rule (its I-coordinate on the lower face, or a fade window)
+        const bool slow = I_enabled && (jI <= 0 || mIlo == IMH_BC_FADE);
+#pragma unroll
+        for (int k = 0; k < KMAX; ++k) {
+            T sm_v = I_enabled ? mn[k] : m0[k];
+            T sr_v = I_enabled ? rn[k] : r0[k];
+            if (slow && k < kc) {
                 long long cell = base + (long long)(i0 + k) * stride;
-                const int jI = pc[dI];  // I != X: constant along the pencil
                 double* tm = nullptr;
                 double* tr = nullptr;
                 double em = 0, er = 0;
@@ -289,18 +332,19 @@ __device__ __forceinline__ void magnet_advect_body(const MagnetArgs<T>& a, long 
                     tm = &em;
                     tr = &er;
                 }
-                T sm_v = transverse_shift(a.mom, cell, jI, NI, sI, I_enabled, sm, tm, a.fade, f);
-                T sr_v = transverse_shift(a.rho, cell, jI, NI, sI, I_enabled, sr, tr, a.fade, f);
+                sm_v = transverse_shift_pre(a.mom, cell, jI, NI, sI, I_enabled, sm, tm, a.fade, f,
+                                            m0[k], mn[k]);
+                sr_v = transverse_shift_pre(a.rho, cell, jI, NI, sI, I_enabled, sr, tr, a.fade, f,
+                                            r0[k], rn[k]);
                 if (trkI) {
                     a.edges[a.eoff[((OBJ_PRIM + Xc) * 2 + 0) * 3 + dI] + fI] = em;
                     a.edges[a.eoff[((OBJ_PRIM + 0) * 2 + 0) * 3 + dI] + fI] = er;
                 }
-                vg[k] = (sm_v + a.mom[cell]) / (sr_v + a.rho[cell]);
-                bI[k] = a.bI[cell];
-            } else {
-                vg[k] = T(0);
-                bI[k] = T(0);
             }
+            // evaluated for every k (padding cells hold clamped copies), masked after
+            const T v = (sm_v + m0[k]) / (sr_v + r0[k]);
+            vg[k] = k < kc ? v : T(0);
+            bI[k] = k < kc ? bI[k] : T(0);
         }
     }
     if (a.init_vg) init_faces(oVG, vg);
@@ -456,7 +500,8 @@ __device__ __forceinline__ void magnet_advect_body(const MagnetArgs<T>& a, long 
 // mag(X) = mag(X) - dt/DGRID{I} .* flux   (relaxingMagnet.m:88-92).
 // ------------------------------------------------------------------------------------
 template <typename T, int KMAX>
-__device__ __forceinline__ void magnet_ct_body(const MagnetArgs<T>& a, long long tile) {
+__device__ __forceinline__ void magnet_ct_body(const MagnetArgs<T>& ap, long long tile) {
+    const MagnetArgs<T> a = ap;  // private copy (see magnet_advect_body)
     if (!a.force && !a.sim->active) return;
     extern __shared__ unsigned char smem_raw[];
     const int dX = a.dX, dI = a.dI;
@@ -498,11 +543,24 @@ __device__ __forceinline__ void magnet_ct_body(const MagnetArgs<T>& a, long long
     GhostSpec sI_spec = {mIhi, 1, 1, 1, a.active_linI, f};
     const bool trkI = (mIhi == IMH_BC_TRANS || mIhi == IMH_BC_WALL) && I_enabled;
     T G[KMAX];
+    // loads first, clamped into the pencil and used unconditionally (see the advection)
+    const int jI = pc[dI];  // I != X: constant along the pencil
+    const long long nbo = (I_enabled && jI < NI - 1) ? sI : 0;  // interior upper neighbour
+    T f0[KMAX], fn[KMAX], bx0[KMAX];
 #pragma unroll
     for (int k = 0; k < KMAX; ++k) {
-        if (k < kc) {
+        const int i = i0 + k < N - 1 ? i0 + k : N - 1;
+        const long long cell = base + (long long)i * stride;
+        f0[k] = a.flux[cell];
+        fn[k] = a.flux[cell + nbo];
+        bx0[k] = a.bX[cell];
+    }
+    const bool slow = I_enabled && (jI >= NI - 1 || mIhi == IMH_BC_FADE);
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+        T s = I_enabled ? fn[k] : f0[k];
+        if (slow && k < kc) {
             long long cell = base + (long long)(i0 + k) * stride;
-            const int jI = pc[dI];  // I != X: constant along the pencil
             double et = 0;
             double* tp = nullptr;
             long long fI = 0;
@@ -513,13 +571,11 @@ __device__ __forceinline__ void magnet_ct_body(const MagnetArgs<T>& a, long long
                 et = a.edges[a.eoff[(oFL * 2 + 1) * 3 + dI] + fI];
                 tp = &et;
             }
-            T s = transverse_shift((const T*)a.flux, cell, jI, NI, sI, I_enabled, sI_spec, tp,
-                                   a.fade, f);
+            s = transverse_shift_pre((const T*)a.flux, cell, jI, NI, sI, I_enabled, sI_spec, tp,
+                                     a.fade, f, f0[k], fn[k]);
             if (onface) a.edges[a.eoff[(oFL * 2 + 1) * 3 + dI] + fI] = et;
-            G[k] = a.flux[cell] - s;
-        } else {
-            G[k] = T(0);
         }
+        G[k] = k < kc ? f0[k] - s : T(0);
     }
 #pragma unroll
     for (int k = 0; k < KMAX; ++k)
@@ -539,7 +595,7 @@ __device__ __forceinline__ void magnet_ct_body(const MagnetArgs<T>& a, long long
         if (k < kc) {
             long long cell = base + (long long)(i0 + k) * stride;
             T h = lds_shift(row, N, i0 + k, -1, &s_g[px][0], mlo, a.fade, f);
-            a.bX[cell] = a.bX[cell] - T(ffI) * h;
+            a.bX[cell] = bx0[k] - T(ffI) * h;
         }
     }
 }
@@ -549,7 +605,8 @@ __device__ __forceinline__ void magnet_ct_body(const MagnetArgs<T>& a, long long
 // mag(k) edge tracker on the upper k face.
 // ------------------------------------------------------------------------------------
 template <typename T>
-__device__ __forceinline__ void cell_centered_body(const CellMagArgs<T>& a, long long chunk) {
+__device__ __forceinline__ void cell_centered_body(const CellMagArgs<T>& ap, long long chunk) {
+    const CellMagArgs<T> a = ap;  // private copy (see magnet_advect_body)
     if (!a.force && !a.sim->active) return;
     const long long ncell = (long long)a.n0 * a.n1 * a.n2;
     long long cell = chunk * 256 + threadIdx.x;
@@ -589,9 +646,7 @@ __device__ inline bool next_tile(const BatchTile* tiles, int ntiles, int r, int&
     }
     const int t = xcd_round_tile(r, ntiles);
     if (t < 0) return false;
-    const BatchTile bt = tiles[t];
-    arg = bt.arg;
-    tile = bt.tile;
+    uniform_tile(tiles, t, arg, tile);
     return true;
 }
 
