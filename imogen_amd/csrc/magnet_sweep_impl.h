@@ -268,8 +268,18 @@ __device__ __forceinline__ void magnet_advect_body(const MagnetArgs<T>& ap, long
     // ghosts of one shift call along X (component I != X: no sign flips) into out[]
     auto gh = [&](int upper, int zero_ghost, const T* row, int m, double* trk, double* out)
         __attribute__((always_inline)) {
-        mag_ghost_lds<T>(upper ? mhi : mlo, upper, 0, zero_ghost, a.active_linX, f, row, N, m,
-                         trk, out);
+        const int mode = upper ? mhi : mlo;
+        if (mode == IMH_BC_CONST || mode == IMH_BC_CIRC) {
+            // ghost_values() of these modes (N >= 3), without the out-of-line call, whose
+            // register saves around it cost the waiting block a memory round trip:
+            // const repeats the end cell, circ wraps (lower N-m+t, upper t)
+            for (int t = 0; t < m; ++t) {
+                const int src = mode == IMH_BC_CONST ? (upper ? N - 1 : 0) : (upper ? t : N - m + t);
+                out[t] = double(row[2 + src]);
+            }
+            return;
+        }
+        mag_ghost_lds<T>(mode, upper, 0, zero_ghost, a.active_linX, f, row, N, m, trk, out);
     };
     // record the edges of a freshly assigned object on all faces (InitializedArray.m:43-47)
     auto init_faces = [&](int obj, const T (&vals)[KMAX]) __attribute__((always_inline)) {
@@ -586,7 +596,10 @@ __device__ __forceinline__ void magnet_ct_body(const MagnetArgs<T>& ap, long lon
         bool trk = (mlo == IMH_BC_TRANS || mlo == IMH_BC_WALL);
         long long ei = a.eoff[(oFL * 2 + 0) * 3 + dX] + pen;
         if (trk) tr = a.edges[ei];
-        mag_ghost_lds<T>(mlo, 0, 0, 1, a.active_linX, f, row, N, 1, &tr, &s_g[px][0]);
+        if (mlo == IMH_BC_CONST || mlo == IMH_BC_CIRC)  // ghost_values() of these modes, inline
+            s_g[px][0] = double(row[2 + (mlo == IMH_BC_CONST ? 0 : N - 1)]);
+        else
+            mag_ghost_lds<T>(mlo, 0, 0, 1, a.active_linX, f, row, N, 1, &tr, &s_g[px][0]);
         if (trk) a.edges[ei] = tr;
     }
     __syncthreads();
