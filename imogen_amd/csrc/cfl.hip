@@ -34,12 +34,12 @@ __device__ __forceinline__ void cfl_body(const CflArgs<T>& a, long long c0, long
     if (!a.sim->active) return;
     double m[3] = {NAN, NAN, NAN};
     for (long long c = c0 + threadIdx.x; c < c1; c += stride) {
-        double rho = double(a.q[0][c]);
-        double px = double(a.q[1][c]), py = double(a.q[2][c]), pz = double(a.q[3][c]);
-        double bx = a.bc[0] ? double(a.bc[0][c]) : 0.0;
-        double by = a.bc[1] ? double(a.bc[1][c]) : 0.0;
-        double bz = a.bc[2] ? double(a.bc[2][c]) : 0.0;
-        double cs = pressure_sound(rho, px, py, pz, double(a.q[4][c]), bx, by, bz, a.use, a.gamma);
+        double rho = double(gmem(a.q[0])[c]);
+        double px = double(gmem(a.q[1])[c]), py = double(gmem(a.q[2])[c]), pz = double(gmem(a.q[3])[c]);
+        double bx = a.bc[0] ? double(gmem(a.bc[0])[c]) : 0.0;
+        double by = a.bc[1] ? double(gmem(a.bc[1])[c]) : 0.0;
+        double bz = a.bc[2] ? double(gmem(a.bc[2])[c]) : 0.0;
+        double cs = pressure_sound(rho, px, py, pz, double(gmem(a.q[4])[c]), bx, by, bz, a.use, a.gamma);
         m[0] = fmax(m[0], fabs(px / rho) + cs);
         m[1] = fmax(m[1], fabs(py / rho) + cs);
         m[2] = fmax(m[2], fabs(pz / rho) + cs);

@@ -38,9 +38,9 @@ struct Column {
         const int ic = i < 0 ? 0 : (i > N - 1 ? N - 1 : i);
         const unsigned o = base + unsigned(ic) * stride;
 #pragma unroll
-        for (int v = 0; v < 5; ++v) Q[v] = q[v][o];
+        for (int v = 0; v < 5; ++v) Q[v] = gmem(q[v])[o];
 #pragma unroll
-        for (int v = 0; v < 3; ++v) B[v] = MHD ? b[v][o] : T(0);
+        for (int v = 0; v < 3; ++v) B[v] = MHD ? gmem(b[v])[o] : T(0);
     }
     // pressure.m 'totsnd', velocity along the sweep and the freezing-speed candidate
     __device__ __forceinline__ T speeds(const T (&Q)[5], const T (&B)[3], T& P, T& U,
@@ -268,9 +268,9 @@ __device__ __forceinline__ void fluid_column_body(const FluidArgs<T>& ap, long l
                 const unsigned o = c.base + unsigned(i) * c.stride;
 #pragma unroll
                 for (int v = 0; v < 5; ++v) {
-                    T qn = c.q[v][o] - ffC * (F2c[v] - F2m[v] + G2c[v] - G2n[v]);
+                    T qn = gmem(c.q[v])[o] - ffC * (F2c[v] - F2m[v] + G2c[v] - G2n[v]);
                     if (v == 0) qn = fmax(qn, minmass);  // mass.array = max(mass, MINMASS)
-                    c.q[v][o] = qn;
+                    gmem(c.q[v])[o] = qn;
                 }
             }
             copy5(F2m, F2c);

@@ -130,18 +130,18 @@ __device__ __forceinline__ void chunk_load(const T* p, T (&out)[K]) {
         typedef T V __attribute__((ext_vector_type(L)));
 #pragma unroll
         for (int j = 0; j < K / L; ++j) {
-            const V x = *reinterpret_cast<const V*>(p + j * L);
+            const V x = *gmem(reinterpret_cast<const V*>(p + j * L));
 #pragma unroll
             for (int l = 0; l < L; ++l) out[j * L + l] = x[l];
         }
     } else if constexpr (B == 8 && K == 2) {
         typedef T V __attribute__((ext_vector_type(2)));
-        const V x = *reinterpret_cast<const V*>(p);
+        const V x = *gmem(reinterpret_cast<const V*>(p));
         out[0] = x[0];
         out[1] = x[1];
     } else {
 #pragma unroll
-        for (int k = 0; k < K; ++k) out[k] = p[k];
+        for (int k = 0; k < K; ++k) out[k] = gmem(p)[k];
     }
 }
 
@@ -156,17 +156,17 @@ __device__ __forceinline__ void chunk_store(T* p, const T (&in)[K]) {
             V x;
 #pragma unroll
             for (int l = 0; l < L; ++l) x[l] = in[j * L + l];
-            *reinterpret_cast<V*>(p + j * L) = x;
+            *gmem(reinterpret_cast<V*>(p + j * L)) = x;
         }
     } else if constexpr (B == 8 && K == 2) {
         typedef T V __attribute__((ext_vector_type(2)));
         V x;
         x[0] = in[0];
         x[1] = in[1];
-        *reinterpret_cast<V*>(p) = x;
+        *gmem(reinterpret_cast<V*>(p)) = x;
     } else {
 #pragma unroll
-        for (int k = 0; k < K; ++k) p[k] = in[k];
+        for (int k = 0; k < K; ++k) gmem(p)[k] = in[k];
     }
 }
 
@@ -307,9 +307,9 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
         for (int k = 0; k < K; ++k) {
             const unsigned o = off(k);
 #pragma unroll
-            for (int v = 0; v < 5; ++v) q[v][k] = a.q[v][o];
+            for (int v = 0; v < 5; ++v) q[v][k] = gmem(a.q[v])[o];
 #pragma unroll
-            for (int v = 0; v < 3; ++v) bm[v][k] = MHD ? a.bc[v][o] : T(0);
+            for (int v = 0; v < 3; ++v) bm[v][k] = MHD ? gmem(a.bc[v])[o] : T(0);
         }
     }
 
@@ -623,7 +623,7 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
                 chunk_load<T, K>(a.q[v] + vbase, un_g[v - V0]);
             } else {
 #pragma unroll
-                for (int k = 0; k < K; ++k) un_g[v - V0][k] = a.q[v][off(k)];
+                for (int k = 0; k < K; ++k) un_g[v - V0][k] = gmem(a.q[v])[off(k)];
             }
         });
         __syncthreads();
@@ -656,7 +656,7 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
             } else {
 #pragma unroll
                 for (int k = 0; k < K; ++k)
-                    if (valid(k)) a.q[v][off(k)] = un[k];
+                    if (valid(k)) gmem(a.q[v])[off(k)] = un[k];
             }
         });
     });

@@ -359,6 +359,24 @@ __host__ __device__ inline void pencil_coords(const int n[3], int d, long long f
     }
 }
 
+// Pointers into device memory taken from an argument block are generic (flat) to the
+// compiler; flat loads and stores count against the LDS wait counter too, so every LDS
+// wait would also wait for them.  gmem() makes them global-memory accesses (the host
+// pass of the compiler sees plain pointers).
+#if defined(__HIP_DEVICE_COMPILE__)
+#define IMH_GLOBAL_AS __attribute__((address_space(1)))
+#else
+#define IMH_GLOBAL_AS
+#endif
+template <typename T>
+__device__ __forceinline__ const IMH_GLOBAL_AS T* gmem(const T* p) {
+    return (const IMH_GLOBAL_AS T*)p;
+}
+template <typename T>
+__device__ __forceinline__ IMH_GLOBAL_AS T* gmem(T* p) {
+    return (IMH_GLOBAL_AS T*)p;
+}
+
 // Face index of cell (i,j,k) for dim d (same conventions).
 __host__ __device__ inline long long face_index(const int n[3], int d, int i, int j, int k) {
     if (d == 0) return j + (long long)n[1] * k;

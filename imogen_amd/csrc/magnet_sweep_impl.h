@@ -313,11 +313,11 @@ __device__ __forceinline__ void magnet_advect_body(const MagnetArgs<T>& ap, long
         for (int k = 0; k < KMAX; ++k) {
             const int i = i0 + k < N - 1 ? i0 + k : N - 1;
             const long long cell = base + (long long)i * stride;
-            m0[k] = a.mom[cell];
-            r0[k] = a.rho[cell];
-            mn[k] = a.mom[cell + nbo];
-            rn[k] = a.rho[cell + nbo];
-            bI[k] = a.bI[cell];
+            m0[k] = gmem(a.mom)[cell];
+            r0[k] = gmem(a.rho)[cell];
+            mn[k] = gmem(a.mom)[cell + nbo];
+            rn[k] = gmem(a.rho)[cell + nbo];
+            bI[k] = gmem(a.bI)[cell];
         }
         // the shifted values: interior neighbours unless this pencil's row needs the BC
         // rule (its I-coordinate on the lower face, or a fade window)
@@ -485,8 +485,8 @@ __device__ __forceinline__ void magnet_advect_body(const MagnetArgs<T>& ap, long
         if (k < kc) {
             long long cell = base + (long long)(i0 + k) * stride;
             T s = lds_shift(row1, N, i0 + k, -1, &s_g[px][6], mlo, a.fade, f);
-            a.bI[cell] = bI[k] - T(ff2) * (F[k] - s);
-            a.flux[cell] = F[k];
+            gmem(a.bI)[cell] = bI[k] - T(ff2) * (F[k] - s);
+            gmem(a.flux)[cell] = F[k];
         }
     }
 
@@ -561,9 +561,9 @@ __device__ __forceinline__ void magnet_ct_body(const MagnetArgs<T>& ap, long lon
     for (int k = 0; k < KMAX; ++k) {
         const int i = i0 + k < N - 1 ? i0 + k : N - 1;
         const long long cell = base + (long long)i * stride;
-        f0[k] = a.flux[cell];
-        fn[k] = a.flux[cell + nbo];
-        bx0[k] = a.bX[cell];
+        f0[k] = gmem(a.flux)[cell];
+        fn[k] = gmem(a.flux)[cell + nbo];
+        bx0[k] = gmem(a.bX)[cell];
     }
     const bool slow = I_enabled && (jI >= NI - 1 || mIhi == IMH_BC_FADE);
 #pragma unroll
@@ -608,7 +608,7 @@ __device__ __forceinline__ void magnet_ct_body(const MagnetArgs<T>& ap, long lon
         if (k < kc) {
             long long cell = base + (long long)(i0 + k) * stride;
             T h = lds_shift(row, N, i0 + k, -1, &s_g[px][0], mlo, a.fade, f);
-            a.bX[cell] = bx0[k] - T(ffI) * h;
+            gmem(a.bX)[cell] = bx0[k] - T(ffI) * h;
         }
     }
 }
@@ -642,7 +642,7 @@ __device__ __forceinline__ void cell_centered_body(const CellMagArgs<T>& ap, lon
     T sh = transverse_shift(a.b, cell, jd, Nd, sd, enabled, s, trk ? &et : nullptr, a.fade,
                             a.bc_inf);
     if (trk) a.edges[a.eoff[((OBJ_MAG + d) * 2 + 1) * 3 + d] + fi] = et;
-    a.cb[cell] = T(0.5) * (a.b[cell] + sh);
+    gmem(a.cb)[cell] = T(0.5) * (gmem(a.b)[cell] + sh);
 }
 
 // Persistent batched forms: blocks walk a tile list spanning many simulations.
