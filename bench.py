@@ -21,6 +21,10 @@ Printed as one JSON line:
   plain_roofline   the same for the plain FP64 path on a 256^3 hydro grid
   cpu_baseline     the CPU oracle (oracle/, C FP64 restatement), single thread, on a
                    bounded sample of the ensemble
+  configs          Sod 1024, Orszag-Tang 512^2 and Sedov 256^3 one at a time on the plain
+                   path: FP64 and FP32-bulk Mcell-updates/s, every field's FP32 error
+                   against FP64 after the same iterations, the FP64 conservation sums
+                   before / after, and the CPU oracle on one core and on all cores
 
 roofline.traffic is the HBM traffic per launch of the dominant kernel measured by two
 rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) over this same workload, read from
@@ -61,6 +65,8 @@ def parse():
                          "for scaling HBM counters)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving stream")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the Sod / Orszag-Tang / Sedov single-simulation leg")
     ap.add_argument("--no-autotune", action="store_true",
                     help="use the default tiles instead of the tuned table")
     ap.add_argument("--serve-requests", type=int, default=48)
@@ -396,6 +402,68 @@ def serving_leg(args):
     return out
 
 
+# (name, generator, GPU iterations, CPU iterations)
+CONFIGS = [("sod 1024 (vanLeer, trans)", lambda p: p.sod(1024), 200, 200),
+           ("orszag-tang 512^2 MHD (superbee, periodic)", lambda p: p.orszag_tang(512), 100, 2),
+           ("sedov 256^3 hydro (minmod, trans)", lambda p: p.sedov(256), 10, 1)]
+
+
+def configs_leg(args):
+    """Each config on the plain path: FP64 and FP32-bulk rates over the same iterations
+    (after one warm iteration), every field's FP32 error against FP64, conservation sums,
+    and the oracle on 1 core and on all cores (at most 16; a bounded number of
+    iterations from the initial state)."""
+    import imogen_amd
+    from imogen_amd import problems
+    sys.path.insert(0, os.path.join(HERE, "tests"))
+    from helpers import make_oracle
+    threads = max(1, min(16, os.cpu_count() or 1))
+    out = []
+    for name, gen, n_gpu, n_cpu in CONFIGS:
+        ini, st = gen(problems)
+        ini = dict(ini, iterMax=10 ** 8, timeMax=1e9)
+        cells = ncells(ini)
+        e = {"config": name, "cells": cells, "gpu_iterations": n_gpu}
+        res = {}
+        for prec in ("fp64", "fp32"):
+            run = imogen_amd.initialize(ini, prec)
+            imogen_amd.create_arrays(run, st)
+            s0 = run.conservation_sums()
+            run.run_iterations(1)
+            run.synchronize()
+            t0 = time.perf_counter()
+            run.run_iterations(n_gpu)
+            run.synchronize()
+            dt = time.perf_counter() - t0
+            e["gpu_%s_mcell_updates_per_s" % prec] = round(2.0 * cells * n_gpu / dt / 1e6, 1)
+            res[prec] = {f: run.download(f) for f in FIELDS}
+            if prec == "fp64":
+                s1 = run.conservation_sums()
+                e["fp64_sums"] = {"before": [float("%.12e" % v) for v in s0[:5]],
+                                  "after": [float("%.12e" % v) for v in s1[:5]],
+                                  "order": "mass, mom x/y/z, energy"}
+            run.close()
+        e["fp32_error_vs_fp64"] = field_errors([(res["fp32"], res["fp64"])])
+        del res
+        log("config %s: gpu fp64 %.1f, fp32 %.1f Mcell-updates/s" % (
+            name, e["gpu_fp64_mcell_updates_per_s"], e["gpu_fp32_mcell_updates_per_s"]))
+        if not args.no_cpu:
+            for label, th in (("cpu_1core", 1), ("cpu_all_cores", threads)):
+                orc = make_oracle(ini, st, threads=th)  # no warm-up: plain C, nothing to JIT
+                t0 = time.perf_counter()
+                for _ in range(n_cpu):
+                    orc.iteration()
+                dt = time.perf_counter() - t0
+                orc.close()
+                e[label] = {"mcell_updates_per_s": round(2.0 * cells * n_cpu / dt / 1e6, 3),
+                            "threads": th, "iterations": n_cpu}
+            log("config %s: cpu %.2f (1 core), %.2f (%d threads) Mcell-updates/s" % (
+                name, e["cpu_1core"]["mcell_updates_per_s"],
+                e["cpu_all_cores"]["mcell_updates_per_s"], threads))
+        out.append(e)
+    return out
+
+
 def large_plain_roofline(args):
     import imogen_amd
     from imogen_amd import problems
@@ -485,6 +553,10 @@ def main():
                         "HBM-budget admission, continuous batching",
             "latency": "completion - arrival, host wall clock",
             "runs": serving_leg(args)}
+
+    if not args.no_configs:
+        log("configs: sod / orszag-tang / sedov")
+        out["configs"] = configs_leg(args)
 
     if args.copy_probe:
         import ctypes
