@@ -17,6 +17,12 @@
 // (TimeManager.running false) turn into no-ops inside the kernels.  The first iteration of
 // a simulation (which creates the edge stores of its flux objects, InitializedArray.m:43-47)
 // runs on its own before it joins the replayed plans.
+//
+// Lanes: simulations with a magnetic field and pure-hydro ones share no data, so their
+// sweeps run on two streams, forked from the batch stream after the time step is known
+// and joined before the next batch-wide kernel (captured into the same graph as parallel
+// branches).  The 1-D MHD sweeps of an ensemble are latency-bound and leave most of the
+// device idle; the hydro sweeps fill it meanwhile.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -31,7 +37,10 @@ using namespace imh;
 
 namespace {
 
-enum LaunchKind { L_BEGIN, L_CFL_RESET, L_CFL, L_FINALIZE, L_FLUID, L_MAG_ADV, L_MAG_CT, L_CELLMAG, L_END };
+enum LaunchKind { L_BEGIN, L_CFL_RESET, L_CFL, L_FINALIZE, L_FLUID, L_MAG_ADV, L_MAG_CT, L_CELLMAG, L_END,
+                  L_FORK, L_JOIN };
+
+constexpr int MAX_LANES = 2;
 
 struct Launch {
     LaunchKind kind;
@@ -43,6 +52,7 @@ struct Launch {
     long long off;  // into the tile array
     int ntiles;
     double bytes;   // algorithmic HBM bytes of the launch
+    int lane;       // -1: the batch stream; else the lane's stream
 };
 
 int launch_kind_class(LaunchKind k) {
@@ -80,6 +90,10 @@ struct imh_batch {
     long long step = 0;
     bool any_step_mode = false;
     Profiler* prof = nullptr;
+    int nlanes = 1;                    // lanes used by the plans (1: no fork)
+    hipStream_t lane_st[MAX_LANES] = {};
+    hipEvent_t fork_ev = nullptr;
+    hipEvent_t join_ev[MAX_LANES] = {};
 };
 
 static void free_plans(imh_batch* b) {
@@ -106,6 +120,14 @@ static void batch_free(imh_batch* b) {
         for (imh_ctx* c : b->ctx) imh_ctx_free(c);
     else
         for (imh_ctx* c : b->ctx) (void)c;
+    for (int l = 0; l < MAX_LANES; ++l) {
+        if (b->lane_st[l]) {
+            hipStreamSynchronize(b->lane_st[l]);
+            hipStreamDestroy(b->lane_st[l]);
+        }
+        if (b->join_ev[l]) hipEventDestroy(b->join_ev[l]);
+    }
+    if (b->fork_ev) hipEventDestroy(b->fork_ev);
     if (b->st) hipStreamDestroy(b->st);
     delete b;
 }
@@ -225,8 +247,20 @@ static int build_plans(imh_batch* b) {
         b->plan[p].push_back(L);
     };
     const size_t es = b->prec == IMH_FP64 ? 8 : 4;
+    // lane of each simulation: those with a magnetic field, then the pure-hydro ones
+    std::vector<int> lane(n);
+    bool has[2] = {false, false};
+    for (int i = 0; i < n; ++i) {
+        const imh_ctx* c = b->ctx[i];
+        lane[i] = (c->cfg.magnet_active || c->mhd_kernel) ? 0 : 1;
+        has[lane[i]] = true;
+    }
+    b->nlanes = (has[0] && has[1]) ? 2 : 1;
+    if (b->nlanes == 1)
+        for (int i = 0; i < n; ++i) lane[i] = 0;
     for (int p = 0; p < 6; ++p) {
         Launch L{};
+        L.lane = -1;
         std::vector<BatchTile> none;
         L.kind = L_BEGIN;
         add_launch(p, L, none);
@@ -243,24 +277,40 @@ static int build_plans(imh_batch* b) {
             }
             Launch r{};
             r.kind = L_CFL_RESET;
+            r.lane = -1;
             add_launch(p, r, none);
             Launch c{};
             c.kind = L_CFL;
+            c.lane = -1;
             c.bytes = by;
             add_launch(p, c, tl);
             Launch f{};
             f.kind = L_FINALIZE;
+            f.lane = -1;
             f.step = step;
             add_launch(p, f, none);
         };
+        auto fork_join = [&](LaunchKind k) {
+            if (b->nlanes < 2) return;
+            Launch x{};
+            x.kind = k;
+            x.lane = -1;
+            add_launch(p, x, none);
+        };
         cfl_block(1);
+        fork_join(L_FORK);
         for (int oi = 0; oi < 2; ++oi) {
             const int order = oi == 0 ? 1 : -1;
-            if (oi == 1 && b->any_step_mode) cfl_block(2);
+            if (oi == 1 && b->any_step_mode) {
+                fork_join(L_JOIN);
+                cfl_block(2);
+                fork_join(L_FORK);
+            }
             for (int t = 0; t < 3; ++t)
                 for (int o = 0; o < 2; ++o) {
                     // fluid groups keyed by (bs, kmax, mhd, general)
                     struct G {
+                        int lane;
                         int bs, kmax;
                         bool mhd, general;
                         double bytes;
@@ -290,11 +340,12 @@ static int build_plans(imh_batch* b) {
                             const int gbs = col ? 0 : ft.bs, gkm = col ? 0 : ft.kmax;
                             G* g = nullptr;
                             for (G& gg : groups)
-                                if (gg.bs == gbs && gg.kmax == gkm && gg.mhd == c->mhd_kernel &&
+                                if (gg.lane == lane[i] && gg.bs == gbs && gg.kmax == gkm &&
+                                    gg.mhd == c->mhd_kernel &&
                                     gg.general == gen)
                                     g = &gg;
                             if (!g) {
-                                groups.push_back({gbs, gkm, c->mhd_kernel, gen, 0.0, {}});
+                                groups.push_back({lane[i], gbs, gkm, c->mhd_kernel, gen, 0.0, {}});
                                 g = &groups.back();
                             }
                             g->bytes += double(10 + (c->mhd_kernel ? 3 : 0)) * c->ncell * es;
@@ -339,6 +390,7 @@ static int build_plans(imh_batch* b) {
                     for (G& g : groups) {
                         Launch f{};
                         f.kind = L_FLUID;
+                        f.lane = g.lane;
                         f.bs = g.bs;
                         f.kmax = g.kmax;
                         f.mhd = g.mhd;
@@ -351,6 +403,7 @@ static int build_plans(imh_batch* b) {
                         for (int s2 = 0; s2 < 2; ++s2) {
                             Launch a1{};
                             a1.kind = L_MAG_ADV;
+                            a1.lane = 0;  // simulations with a magnetic field
                             a1.kmax = magkmax[s2];
                             a1.smem = magsmem[s2];
                             a1.bytes = 5.0 * mag_cells * es;
@@ -358,6 +411,7 @@ static int build_plans(imh_batch* b) {
                             add_launch(p, a1, tl);
                             Launch c1{};
                             c1.kind = L_MAG_CT;
+                            c1.lane = 0;
                             c1.kmax = magkmax[s2];
                             c1.smem = magsmem[s2] / 2;
                             c1.bytes = 3.0 * mag_cells * es;
@@ -366,12 +420,15 @@ static int build_plans(imh_batch* b) {
                         }
                         Launch c2{};
                         c2.kind = L_CELLMAG;
+                        c2.lane = 0;
                         c2.bytes = 4.0 * mag_cells * es;
                         add_launch(p, c2, cm);
                     }
                 }
         }
+        fork_join(L_JOIN);
         Launch E{};
+        E.lane = -1;
         E.kind = L_END;
         add_launch(p, E, none);
     }
@@ -392,39 +449,72 @@ static int build_plans(imh_batch* b) {
 }
 
 template <typename T>
-static hipError_t run_launch(imh_batch* b, const Launch& L) {
+static hipError_t run_launch(imh_batch* b, const Launch& L, hipStream_t st) {
     const int n = b->n;
     const BatchTile* tl = b->tiles + L.off;
     // a multiple of 8 blocks: the XCD-range schedule (xcd_round_tile)
     const int nb = ((std::max(1, std::min(L.ntiles, b->ncu * 4)) + 7) / 8) * 8;
     switch (L.kind) {
-    case L_BEGIN: return launch_begin_iteration(b->sims_dev, b->pars_dev, n, 0, b->st);
-    case L_CFL_RESET: return launch_cfl_reset(b->sims_dev, n, b->st);
-    case L_CFL: return launch_cfl_batch<T>((const CflArgs<T>*)b->cflargs, tl, L.ntiles, nb, b->st);
-    case L_FINALIZE: return launch_time_finalize(b->sims_dev, b->pars_dev, n, L.step, b->st);
+    case L_BEGIN: return launch_begin_iteration(b->sims_dev, b->pars_dev, n, 0, st);
+    case L_CFL_RESET: return launch_cfl_reset(b->sims_dev, n, st);
+    case L_CFL: return launch_cfl_batch<T>((const CflArgs<T>*)b->cflargs, tl, L.ntiles, nb, st);
+    case L_FINALIZE: return launch_time_finalize(b->sims_dev, b->pars_dev, n, L.step, st);
     case L_FLUID:
         if (L.bs == 0)
             return launch_fluid_column<T>((const FluidArgs<T>*)b->fargs, tl, L.ntiles, nb, L.mhd,
-                                          b->st);
+                                          st);
         return launch_fluid_batch<T>((const FluidArgs<T>*)b->fargs, tl, L.ntiles, nb, L.bs, L.kmax,
-                                     L.mhd, L.general, b->st);
+                                     L.mhd, L.general, st);
     case L_MAG_ADV:
         return launch_magnet_batch<T>(0, (const MagnetArgs<T>*)b->margs, tl, L.ntiles, nb, L.kmax,
-                                      L.smem, b->st);
+                                      L.smem, st);
     case L_MAG_CT:
         return launch_magnet_batch<T>(1, (const MagnetArgs<T>*)b->margs, tl, L.ntiles, nb, L.kmax,
-                                      L.smem, b->st);
+                                      L.smem, st);
     case L_CELLMAG:
-        return launch_cell_centered_batch<T>((const CellMagArgs<T>*)b->cargs, tl, L.ntiles, nb, b->st);
-    case L_END: return launch_end_iteration(b->sims_dev, n, b->st);
+        return launch_cell_centered_batch<T>((const CellMagArgs<T>*)b->cargs, tl, L.ntiles, nb, st);
+    case L_END: return launch_end_iteration(b->sims_dev, n, st);
+    case L_FORK:
+    case L_JOIN: return hipSuccess;  // handled by run_plan
     }
     return hipErrorInvalidValue;
 }
 
+// Lane streams and events (default priority), made once the plans need them.
+static int ensure_lanes(imh_batch* b) {
+    if (b->nlanes < 2 || b->fork_ev) return IMH_OK;
+    hipError_t e = hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming);
+    for (int l = 0; l < MAX_LANES && e == hipSuccess; ++l) {
+        e = hipStreamCreateWithFlags(&b->lane_st[l], hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&b->join_ev[l], hipEventDisableTiming);
+    }
+    return e == hipSuccess ? IMH_OK : hip_fail(e, "batch lanes");
+}
+
+// The launches of plan p.  With a profiler attached every launch goes to the batch
+// stream (one at a time, so each launch's events bracket it alone).
 static int run_plan(imh_batch* b, int p) {
+    const bool lanes = b->nlanes > 1 && !b->prof && b->fork_ev;
     for (const Launch& L : b->plan[p]) {
+        hipError_t e = hipSuccess;
+        if (L.kind == L_FORK || L.kind == L_JOIN) {
+            if (!lanes) continue;
+            if (L.kind == L_FORK) {
+                e = hipEventRecord(b->fork_ev, b->st);
+                for (int l = 0; l < b->nlanes && e == hipSuccess; ++l)
+                    e = hipStreamWaitEvent(b->lane_st[l], b->fork_ev, 0);
+            } else {
+                for (int l = 0; l < b->nlanes && e == hipSuccess; ++l) {
+                    e = hipEventRecord(b->join_ev[l], b->lane_st[l]);
+                    if (e == hipSuccess) e = hipStreamWaitEvent(b->st, b->join_ev[l], 0);
+                }
+            }
+            if (e != hipSuccess) return hip_fail(e, "batch lane fork/join");
+            continue;
+        }
+        hipStream_t st = (lanes && L.lane >= 0) ? b->lane_st[L.lane] : b->st;
         if (b->prof) prof_begin(b->prof, b->st, launch_kind_class(L.kind), L.bytes);
-        hipError_t e = b->prec == IMH_FP64 ? run_launch<double>(b, L) : run_launch<float>(b, L);
+        e = b->prec == IMH_FP64 ? run_launch<double>(b, L, st) : run_launch<float>(b, L, st);
         if (b->prof) prof_end(b->prof, b->st);
         if (e != hipSuccess) return hip_fail(e, "batch launch");
     }
@@ -544,6 +634,8 @@ static int batch_prepare(imh_batch* b) {
         if (e != hipSuccess) return hip_fail(e, "batch sync");
         if ((rc = upload_args(b))) return rc;
         if ((rc = build_plans(b))) return rc;
+        // lane streams and events exist before any capture (no creation while capturing)
+        if ((rc = ensure_lanes(b))) return rc;
         b->step = 0;
     }
     return IMH_OK;
