@@ -67,6 +67,8 @@ def parse():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving stream")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the Sod / Orszag-Tang / Sedov single-simulation leg")
+    ap.add_argument("--save-tiles", metavar="PATH",
+                    help="write the tile table (with the classes tuned in this run) here")
     ap.add_argument("--no-autotune", action="store_true",
                     help="use the default tiles instead of the tuned table")
     ap.add_argument("--serve-requests", type=int, default=48)
@@ -283,10 +285,11 @@ def batched_leg(sims, args, cells):
     if not args.no_autotune:
         table = autotune.load_table()
         table = autotune.tune(sims, "fp32", table=table, log=log)
-        gout = os.path.join(HERE, "gpurun_out")
-        autotune.save_table(table, os.path.join(gout, "tuned_tiles.json")
-                            if os.path.isdir(gout) and not os.environ.get("IMH_TILE_TABLE")
-                            else autotune.TABLE)
+        # tiles tuned in this run are kept where asked (an IMH_TILE_TABLE file is updated)
+        if os.environ.get("IMH_TILE_TABLE"):
+            autotune.save_table(table, autotune.TABLE)
+        elif args.save_tiles:
+            autotune.save_table(table, args.save_tiles)
     ens = Ensemble(sims, "fp32")
     if not args.no_autotune:
         tuned = autotune.apply(ens, sims, table, "fp32")
