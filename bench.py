@@ -98,7 +98,8 @@ def trace_summary(trace_csv, bench_log):
     between the iteration start before the first of them and the next double-precision
     launch) and the plain 256^3 grid (the last `plain_roofline.launches` double fluid
     launches and what runs between them), with the average fluid launch duration beside
-    the one bench.py measured with HIP events."""
+    the one bench.py measured with HIP events.  The traced run must have no FP32 work after
+    the profiled pass: run it with --no-serve --no-configs.
     import collections
     import csv
     out = _bench_json(bench_log)
@@ -165,7 +166,8 @@ def pmc_traffic(fetch_csv, write_csv, bench_log):
     FETCH_SIZE / WRITE_SIZE (KiB per dispatch) are scaled by what they read for the copy
     probe, which reads and writes a known number of bytes.  The profiled pass of the
     batched step is the last `roofline.launches` float fluid launches; that of the plain
-    256^3 grid the last `plain_roofline.launches` double ones."""
+    256^3 grid the last `plain_roofline.launches` double ones (so the counted run has no
+    FP32 work after the profiled pass: run it with --no-serve --no-configs).
     import csv
     out = _bench_json(bench_log)
     probe = out["copy_probe"]["bytes_read"]
