@@ -99,7 +99,7 @@ def trace_summary(trace_csv, bench_log):
     launch) and the plain 256^3 grid (the last `plain_roofline.launches` double fluid
     launches and what runs between them), with the average fluid launch duration beside
     the one bench.py measured with HIP events.  The traced run must have no FP32 work after
-    the profiled pass: run it with --no-serve --no-configs.
+    the profiled pass: run it with --no-serve --no-configs."""
     import collections
     import csv
     out = _bench_json(bench_log)
@@ -167,7 +167,7 @@ def pmc_traffic(fetch_csv, write_csv, bench_log):
     probe, which reads and writes a known number of bytes.  The profiled pass of the
     batched step is the last `roofline.launches` float fluid launches; that of the plain
     256^3 grid the last `plain_roofline.launches` double ones (so the counted run has no
-    FP32 work after the profiled pass: run it with --no-serve --no-configs).
+    FP32 work after the profiled pass: run it with --no-serve --no-configs)."""
     import csv
     out = _bench_json(bench_log)
     probe = out["copy_probe"]["bytes_read"]
