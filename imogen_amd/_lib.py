@@ -81,6 +81,9 @@ SIGNATURES = {
     "imh_conservation_sums": (_I, [_P, _D]),
     "imh_memory_bytes": (_I, [_P, ctypes.POINTER(ctypes.c_size_t)]),
     "imh_memory_estimate": (_I, [ctypes.POINTER(ImhConfig), ctypes.POINTER(ctypes.c_size_t)]),
+    "imh_checkpoint_bytes": (_I, [_P, ctypes.POINTER(ctypes.c_size_t)]),
+    "imh_checkpoint_save": (_I, [_P, ctypes.c_void_p, ctypes.c_size_t]),
+    "imh_checkpoint_load": (_I, [_P, ctypes.c_void_p, ctypes.c_size_t]),
     "imh_batch_create": (_I, [ctypes.POINTER(ImhConfig), _I, ctypes.POINTER(_P)]),
     "imh_batch_destroy": (_I, [_P]),
     "imh_batch_upload_state": (_I, [_P, _I, _D, _D * 3, _D, _D * 3, ctypes.c_size_t]),

@@ -1004,6 +1004,112 @@ int imh_synchronize(imh_ctx* c) {
     return e == hipSuccess ? IMH_OK : hip_fail(e, "synchronize");
 }
 
+// ---- checkpoint / resume --------------------------------------------------------------
+
+namespace {
+struct CkptHeader {
+    char magic[8];            // "IMHCKPT\0"
+    int version;              // 1
+    int precision;
+    int grid[3];
+    int nfields;
+    int nobj;
+    long long ncell;
+    long long edges_len;
+    int host_iteration;
+    int fluid_init;
+    int have_state;
+    int mhd_kernel;
+    int mag_is_zero[3];
+    int pad;
+};
+const char kCkptMagic[8] = {'I', 'M', 'H', 'C', 'K', 'P', 'T', 0};
+
+size_t ckpt_bytes(const imh_ctx* c) {
+    return sizeof(CkptHeader) + sizeof(SimDev) + NOBJ + (size_t)IMH_NFIELDS * c->ncell * esize(c) +
+           c->edges_len * sizeof(double);
+}
+}  // namespace
+
+extern "C" int imh_checkpoint_bytes(imh_ctx* c, size_t* bytes) {
+    if (!c || !bytes) return fail(IMH_EINVAL, "null argument");
+    *bytes = ckpt_bytes(c);
+    return IMH_OK;
+}
+
+extern "C" int imh_checkpoint_save(imh_ctx* c, void* buf, size_t bytes) {
+    if (!c || !buf) return fail(IMH_EINVAL, "null argument");
+    if (!c->have_state) return fail(IMH_ESTATE, "state not uploaded");
+    if (bytes < ckpt_bytes(c)) return fail(IMH_EINVAL, "checkpoint buffer too small");
+    hipError_t e = hipStreamSynchronize(c->st);
+    if (e != hipSuccess) return hip_fail(e, "stream sync");
+    char* p = (char*)buf;
+    CkptHeader h;
+    memset(&h, 0, sizeof(h));
+    memcpy(h.magic, kCkptMagic, 8);
+    h.version = 1;
+    h.precision = c->cfg.precision;
+    for (int d = 0; d < 3; ++d) h.grid[d] = c->n[d];
+    h.nfields = IMH_NFIELDS;
+    h.nobj = NOBJ;
+    h.ncell = c->ncell;
+    h.edges_len = (long long)c->edges_len;
+    h.host_iteration = c->host_iteration;
+    h.fluid_init = c->fluid_init;
+    h.have_state = c->have_state;
+    h.mhd_kernel = c->mhd_kernel;
+    for (int k = 0; k < 3; ++k) h.mag_is_zero[k] = c->mag_is_zero[k];
+    memcpy(p, &h, sizeof(h));
+    p += sizeof(h);
+    e = hipMemcpy(p, c->sim, sizeof(SimDev), hipMemcpyDeviceToHost);
+    p += sizeof(SimDev);
+    for (int o = 0; o < NOBJ; ++o) *p++ = c->obj_init[o] ? 1 : 0;
+    const size_t fb = (size_t)c->ncell * esize(c);
+    for (int f = 0; f < IMH_NFIELDS && e == hipSuccess; ++f, p += fb)
+        e = hipMemcpy(p, c->f[f], fb, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && c->edges_len)
+        e = hipMemcpy(p, c->edges, c->edges_len * sizeof(double), hipMemcpyDeviceToHost);
+    return e == hipSuccess ? IMH_OK : hip_fail(e, "checkpoint save");
+}
+
+extern "C" int imh_checkpoint_load(imh_ctx* c, const void* buf, size_t bytes) {
+    if (!c || !buf) return fail(IMH_EINVAL, "null argument");
+    if (bytes < sizeof(CkptHeader)) return fail(IMH_EINVAL, "checkpoint too short");
+    const char* p = (const char*)buf;
+    CkptHeader h;
+    memcpy(&h, p, sizeof(h));
+    if (memcmp(h.magic, kCkptMagic, 8) != 0 || h.version != 1)
+        return fail(IMH_EINVAL, "not a checkpoint of this library");
+    if (h.precision != c->cfg.precision || h.grid[0] != c->n[0] || h.grid[1] != c->n[1] ||
+        h.grid[2] != c->n[2] || h.nfields != IMH_NFIELDS || h.nobj != NOBJ ||
+        h.ncell != c->ncell || h.edges_len != (long long)c->edges_len)
+        return fail(IMH_EINVAL, "checkpoint of another configuration (precision, grid or "
+                    "boundary layout differ)");
+    if (bytes < ckpt_bytes(c)) return fail(IMH_EINVAL, "checkpoint truncated");
+    hipError_t e = hipStreamSynchronize(c->st);
+    if (e != hipSuccess) return hip_fail(e, "stream sync");
+    p += sizeof(h);
+    e = hipMemcpy(c->sim, p, sizeof(SimDev), hipMemcpyHostToDevice);
+    p += sizeof(SimDev);
+    for (int o = 0; o < NOBJ; ++o) c->obj_init[o] = *p++ != 0;
+    const size_t fb = (size_t)c->ncell * esize(c);
+    for (int f = 0; f < IMH_NFIELDS && e == hipSuccess; ++f, p += fb)
+        e = hipMemcpy(c->f[f], p, fb, hipMemcpyHostToDevice);
+    if (e == hipSuccess && c->edges_len)
+        e = hipMemcpy(c->edges, p, c->edges_len * sizeof(double), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, "checkpoint load");
+    c->host_iteration = h.host_iteration;
+    c->fluid_init = h.fluid_init != 0;
+    c->have_state = h.have_state != 0;
+    c->mhd_kernel = h.mhd_kernel != 0;
+    for (int k = 0; k < 3; ++k) c->mag_is_zero[k] = h.mag_is_zero[k];
+    for (int i = 0; i < 6; ++i) {  // captured iterations hold argument blocks of the old state
+        if (c->graph[i]) hipGraphExecDestroy(c->graph[i]);
+        c->graph[i] = nullptr;
+    }
+    return IMH_OK;
+}
+
 int imh_conservation_sums(imh_ctx* c, double out[8]) {
     if (!c || !out) return fail(IMH_EINVAL, "null argument");
     hipError_t e;

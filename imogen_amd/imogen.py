@@ -194,6 +194,19 @@ class ImogenManager:
         check(lib().imh_memory_bytes(self._ctx, ctypes.byref(b)))
         return b.value
 
+    def checkpoint(self):
+        """The whole resumable state (fields, edge trackers, flux-object flags, time) as
+        bytes; `restore` on a run of the same configuration continues bit for bit."""
+        n = ctypes.c_size_t()
+        check(lib().imh_checkpoint_bytes(self._ctx, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value)
+        check(lib().imh_checkpoint_save(self._ctx, buf, n.value))
+        return buf.raw
+
+    def restore(self, data):
+        buf = ctypes.create_string_buffer(bytes(data), len(data))
+        check(lib().imh_checkpoint_load(self._ctx, buf, len(data)))
+
     def set_fluid_tile(self, X, bs, kmax, px):
         check(lib().imh_set_fluid_tile(self._ctx, int(X), int(bs), int(kmax), int(px)))
 

@@ -153,6 +153,19 @@ int imh_memory_bytes(imh_ctx *ctx, size_t *bytes);
 /* Device bytes a context with this configuration would hold (validates cfg first). */
 int imh_memory_estimate(const imh_config *cfg, size_t *bytes);
 
+/* ---- checkpoint / resume --------------------------------------------------------------
+ * No counterpart in the reference: its saves (save/resultsHandler.m:26-113) hold the
+ * fields and the time only, so a restarted run loses the per-object edge trackers of
+ * the transparent and wall boundaries (Edges.m:121-136) and the first-assignment state
+ * of the flux objects (InitializedArray.m:43-47).  A checkpoint holds those as well, so a
+ * run resumed from it continues bit for bit.  Layout: a fixed header (magic, version,
+ * precision, grid, sizes), the device time state, the flux-object flags, the 11 field
+ * arrays in the context's precision, the edge stores (FP64).  Load into a context created
+ * with the same configuration; its captured iterations are dropped. */
+int imh_checkpoint_bytes(imh_ctx *ctx, size_t *bytes);
+int imh_checkpoint_save(imh_ctx *ctx, void *buf, size_t bytes);
+int imh_checkpoint_load(imh_ctx *ctx, const void *buf, size_t bytes);
+
 /* ---- batched ensemble (many independent simulations, one launch per sweep phase) -- */
 typedef struct imh_batch imh_batch;
 

@@ -204,3 +204,44 @@ def test_conservation_sums_periodic():
     # periodic boundaries: totals conserved to rounding
     assert abs(s1[0] - s0[0]) <= 1e-12 * abs(s0[0])
     assert abs(s1[4] - s0[4]) <= 1e-12 * abs(s0[4])
+
+
+@pytest.mark.parametrize("case", ["sod_trans", "mhd2d_trans"])
+def test_checkpoint_resume_is_bit_exact(case):
+    """A run resumed from a checkpoint (fields, edge trackers, flux-object flags, time)
+    continues exactly as the uninterrupted run."""
+    from imogen_amd import iteration
+    if case == "sod_trans":
+        ini, st = problems.sod(512)
+        fields = FLUID_FIELDS
+    else:
+        grid = (24, 22, 1)
+        st = problems.random_state(grid, seed=13, magnet=True)
+        ini = {"grid": grid, "dGrid": (1.0 / 24, 1.0 / 22, 1.0 / 24), "gamma": 5.0 / 3.0,
+               "cfl": 0.35, "bcMode": "trans", "bcInfinity": 20, "fluxLimiter": "vanleer",
+               "minMass": 1e-5, "mode": {"fluid": True, "magnet": True}, "iterMax": 100,
+               "timeMax": 1e9}
+        fields = FLUID_FIELDS + MAG_FIELDS
+    a = make_gpu(ini, st)
+    for _ in range(5):
+        iteration(a)
+    ck = a.checkpoint()
+    for _ in range(6):
+        iteration(a)
+    b = make_gpu(ini, st)   # a fresh run of the same configuration
+    b.restore(ck)
+    assert b.time.iteration == 6
+    for _ in range(6):
+        iteration(b)
+    for f in fields:
+        assert rel_linf(b.download(f), a.download(f)) == 0.0, f
+    assert b.time.time == a.time.time and b.time.iteration == a.time.iteration
+    if case == "sod_trans":  # a checkpoint of another grid is refused
+        from imogen_amd._lib import ImogenError
+        ini2, st2 = problems.sod(256)
+        c = make_gpu(ini2, st2)
+        with pytest.raises(ImogenError):
+            c.restore(ck)
+        c.close()
+    a.close()
+    b.close()
