@@ -33,16 +33,31 @@ __device__ __forceinline__ void cfl_body(const CflArgs<T>& a, long long c0, long
     __shared__ double s_red[3][4];
     if (!a.sim->active) return;
     double m[3] = {NAN, NAN, NAN};
-    for (long long c = c0 + threadIdx.x; c < c1; c += stride) {
-        double rho = double(gmem(a.q[0])[c]);
-        double px = double(gmem(a.q[1])[c]), py = double(gmem(a.q[2])[c]), pz = double(gmem(a.q[3])[c]);
-        double bx = a.bc[0] ? double(gmem(a.bc[0])[c]) : 0.0;
-        double by = a.bc[1] ? double(gmem(a.bc[1])[c]) : 0.0;
-        double bz = a.bc[2] ? double(gmem(a.bc[2])[c]) : 0.0;
-        double cs = pressure_sound(rho, px, py, pz, double(gmem(a.q[4])[c]), bx, by, bz, a.use, a.gamma);
-        m[0] = fmax(m[0], fabs(px / rho) + cs);
-        m[1] = fmax(m[1], fabs(py / rho) + cs);
-        m[2] = fmax(m[2], fabs(pz / rho) + cs);
+    // U cells per thread and round, every load of the round issued before any use (a
+    // loop of one cell at a time waits a memory round trip per cell); cells past c1 read
+    // cell c0 and are left out of the maxima
+    constexpr int U = 4;
+    for (long long cb = c0 + threadIdx.x; cb < c1; cb += stride * U) {
+        T q[5][U], b[3][U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const long long c = cb + u * stride < c1 ? cb + u * stride : c0;
+#pragma unroll
+            for (int v = 0; v < 5; ++v) q[v][u] = gmem(a.q[v])[c];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) b[k][u] = a.bc[k] ? gmem(a.bc[k])[c] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (cb + u * stride >= c1) continue;
+            const double rho = double(q[0][u]);
+            const double px = double(q[1][u]), py = double(q[2][u]), pz = double(q[3][u]);
+            const double cs = pressure_sound(rho, px, py, pz, double(q[4][u]), double(b[0][u]),
+                                             double(b[1][u]), double(b[2][u]), a.use, a.gamma);
+            m[0] = fmax(m[0], fabs(px / rho) + cs);
+            m[1] = fmax(m[1], fabs(py / rho) + cs);
+            m[2] = fmax(m[2], fabs(pz / rho) + cs);
+        }
     }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int k = 0; k < 3; ++k) {
