@@ -81,6 +81,7 @@ SIGNATURES = {
     "imh_conservation_sums": (_I, [_P, _D]),
     "imh_memory_bytes": (_I, [_P, ctypes.POINTER(ctypes.c_size_t)]),
     "imh_memory_estimate": (_I, [ctypes.POINTER(ImhConfig), ctypes.POINTER(ctypes.c_size_t)]),
+    "imh_init_problem": (_I, [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_int]),
     "imh_checkpoint_bytes": (_I, [_P, ctypes.POINTER(ctypes.c_size_t)]),
     "imh_checkpoint_save": (_I, [_P, ctypes.c_void_p, ctypes.c_size_t]),
     "imh_checkpoint_load": (_I, [_P, ctypes.c_void_p, ctypes.c_size_t]),

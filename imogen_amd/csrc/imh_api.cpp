@@ -1004,6 +1004,119 @@ int imh_synchronize(imh_ctx* c) {
     return e == hipSuccess ? IMH_OK : hip_fail(e, "synchronize");
 }
 
+// ---- initial conditions on the device ---------------------------------------------------
+
+extern "C" int imh_init_problem(imh_ctx* c, int kind, const double* params, int nparams) {
+    if (!c) return fail(IMH_EINVAL, "null context");
+    static const int need[6] = {0, 0, 0, 3, 15, 4};
+    if (kind < IMH_IC_SOD || kind > IMH_IC_KELVIN_HELMHOLTZ)
+        return fail(IMH_EINVAL, "unknown initial-condition kind %d", kind);
+    if (nparams < need[kind] || (need[kind] > 0 && !params))
+        return fail(IMH_EINVAL, "initial condition %d takes %d parameters", kind, need[kind]);
+    IcParams P;
+    memset(&P, 0, sizeof(P));
+    P.kind = kind;
+    for (int d = 0; d < 3; ++d) {
+        P.n[d] = c->n[d];
+        P.dx[d] = c->cfg.dgrid[d];
+    }
+    P.gamma = c->cfg.gamma;
+    for (int i = 0; i < need[kind] && i < 24; ++i) P.p[i] = params[i];
+    bool anyB = false;
+    const double g = c->cfg.gamma;
+    switch (kind) {
+    case IMH_IC_SOD:
+        P.ip[0] = int(std::floor(c->n[0] / 2.0 + 0.5));  // MATLAB round(N/2)
+        break;
+    case IMH_IC_ORSZAG_TANG:
+        anyB = true;
+        break;
+    case IMH_IC_SEDOV: {
+        // cells inside the blast radius, with the kernel's expression (a small box around
+        // the centre holds them all)
+        const int n = c->n[0];
+        const double dx = c->cfg.dgrid[0], rr = params[0] * dx;
+        if (c->n[1] != n || c->n[2] != n) return fail(IMH_EINVAL, "Sedov needs a cubic grid");
+        long long inside = 0;
+        const int h = int(std::ceil(params[0])) + 2;
+        for (int k = n / 2 - h; k <= n / 2 + h; ++k)
+            for (int j = n / 2 - h; j <= n / 2 + h; ++j)
+                for (int i = n / 2 - h; i <= n / 2 + h; ++i) {
+                    if (i < 0 || j < 0 || k < 0 || i >= n || j >= n || k >= n) continue;
+                    const double cx = (double(i) + 0.5) * dx - 0.5, cy = (double(j) + 0.5) * dx - 0.5,
+                                 cz = (double(k) + 0.5) * dx - 0.5;
+                    if (cz * cz + cy * cy + cx * cx <= rr * rr) ++inside;
+                }
+        if (inside == 0) return fail(IMH_EINVAL, "Sedov blast radius holds no cell");
+        P.ip[0] = int(inside);
+        break;
+    }
+    case IMH_IC_RIEMANN:
+        // the kernel reads the left state at p[0..6], the right at p[8..14], Bx at p[16]
+        for (int i = 0; i < 7; ++i) {
+            P.p[i] = params[i];
+            P.p[8 + i] = params[7 + i];
+        }
+        P.p[16] = params[14];
+        P.ip[0] = (nparams > 15 && params[15] >= 0) ? int(params[15]) : c->n[0] / 2;
+        for (int i : {5, 6, 13, 14, 16}) anyB = anyB || P.p[i] != 0.0;
+        break;
+    case IMH_IC_KELVIN_HELMHOLTZ: {
+        // external/speedFromMach.m with rho = 1, E = 1 / (gamma - 1), B = 0
+        const double mach = params[0];
+        const double ener = 1.0 / (g - 1.0);
+        P.p[0] = mach * std::sqrt((g * ener) / (1.0 / (g - 1.0) + g * mach * mach / 2.0));
+        P.p[1] = params[1];
+        P.p[2] = params[2];
+        P.seed = (unsigned long long)(long long)params[3];
+        P.ip[0] = int(std::ceil(c->n[1] / 2.0));
+        break;
+    }
+    }
+    hipError_t e = hipStreamSynchronize(c->st);
+    void* const f[8] = {c->f[0], c->f[1], c->f[2], c->f[3], c->f[4], c->f[5], c->f[6], c->f[7]};
+    if (e == hipSuccess) e = launch_ic_fill(P, f, c->cfg.precision == IMH_FP64, c->st);
+    // edge stores of the FluidArray and MagnetArray objects come from the initial state
+    if (e == hipSuccess && c->edges_len)
+        e = hipMemsetAsync(c->edges, 0, sizeof(double) * c->edges_len, c->st);
+    for (int fld = 0; fld < 8 && e == hipSuccess; ++fld) {
+        const int obj = fld < 5 ? OBJ_PRIM + fld : OBJ_MAG + (fld - 5);
+        for (int s = 0; s < 2 && e == hipSuccess; ++s)
+            for (int d = 0; d < c->ndims && e == hipSuccess; ++d) {
+                const int m = c->cfg.bc[s][d];
+                if (m != IMH_BC_TRANS && m != IMH_BC_WALL) continue;
+                e = launch_ic_edges(P, fld, s, d, c->edges, c->eoff[obj][s][d], c->nface[d], c->st);
+            }
+    }
+    if (e != hipSuccess) return hip_fail(e, "initial condition");
+    // MagnetArray.isZero = (sumND(array) == 0) && ~run.magnet.ACTIVE: the B of these
+    // problems is zero exactly when none of its parameters is nonzero
+    for (int k = 0; k < 3; ++k) c->mag_is_zero[k] = !anyB && !c->cfg.magnet_active;
+    c->mhd_kernel = c->cfg.magnet_active || anyB;
+    SimDev sd;  // time state of a new run (imogen.m:50)
+    memset(&sd, 0, sizeof(sd));
+    sd.iteration = 1;
+    sd.active = 1;
+    e = hipMemcpyAsync(c->sim, &sd, sizeof(sd), hipMemcpyHostToDevice, c->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    if (e != hipSuccess) return hip_fail(e, "sim reset");
+    c->host_iteration = 1;
+    c->fluid_init = false;
+    for (int o = 0; o < NOBJ; ++o) c->obj_init[o] = false;
+    for (int i = 0; i < 6; ++i) {
+        if (c->graph[i]) hipGraphExecDestroy(c->graph[i]);
+        c->graph[i] = nullptr;
+    }
+    for (int k = 0; k < 3; ++k) {  // MagnetArray.m:197
+        int rc = cellmag(c, k, 1);
+        if (rc) return rc;
+    }
+    e = hipStreamSynchronize(c->st);
+    if (e != hipSuccess) return hip_fail(e, "cellMag");
+    c->have_state = true;
+    return IMH_OK;
+}
+
 // ---- checkpoint / resume --------------------------------------------------------------
 
 namespace {

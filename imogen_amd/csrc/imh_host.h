@@ -111,6 +111,22 @@ constexpr int CAP_SLOTS = 96;          // launches in one captured iteration (44
 // with the error recorded).
 const void* imh_stage_args(imh_ctx* c, const void* a, size_t n);
 
+// Parameters of a device-generated initial state (initial.hip).
+struct IcParams {
+    int kind;
+    int n[3];
+    int ip[4];            // integer parameters (split cells, counts)
+    double dx[3];
+    double gamma;
+    double p[24];         // real parameters
+    unsigned long long seed;
+};
+namespace imh {
+hipError_t launch_ic_fill(const IcParams& P, void* const f[8], bool fp64, hipStream_t st);
+hipError_t launch_ic_edges(const IcParams& P, int field, int side, int d, double* edges,
+                           long long off, long long nface, hipStream_t st);
+}  // namespace imh
+
 // internal entry points (force = run regardless of the loop's running flag)
 int imh_fluid_sweep_internal(imh_ctx* c, int X, int force);
 int imh_magnet_sweep_internal(imh_ctx* c, int X, const int I[2], int force);

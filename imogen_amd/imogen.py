@@ -194,6 +194,15 @@ class ImogenManager:
         check(lib().imh_memory_bytes(self._ctx, ctypes.byref(b)))
         return b.value
 
+    IC_KINDS = {"sod": 1, "orszag_tang": 2, "sedov": 3, "riemann": 4, "kelvin_helmholtz": 5}
+
+    def init_problem(self, kind, params=()):
+        """Generate the initial state of a benchmark problem in device memory (with the
+        constructor side effects of create_arrays); params as imh_init_problem takes them
+        (include/imogen_hip.h)."""
+        arr = (ctypes.c_double * max(1, len(params)))(*[float(p) for p in params])
+        check(lib().imh_init_problem(self._ctx, self.IC_KINDS[kind], arr, len(params)))
+
     def checkpoint(self):
         """The whole resumable state (fields, edge trackers, flux-object flags, time) as
         bytes; `restore` on a run of the same configuration continues bit for bit."""

@@ -153,6 +153,27 @@ int imh_memory_bytes(imh_ctx *ctx, size_t *bytes);
 /* Device bytes a context with this configuration would hold (validates cfg first). */
 int imh_memory_estimate(const imh_config *cfg, size_t *bytes);
 
+/* ---- initial conditions on the device ---------------------------------------------------
+ * No counterpart in the reference (its experiment/*Initializer.m classes build the state
+ * on the host): the state of one of these problems is generated in device memory, with
+ * the constructor side effects of imh_upload_state (edge stores from the initial arrays,
+ * isZero, cell-centred B, a new run's time state), so a simulation starts without a host
+ * state or a host-to-device copy.  params (nparams values):
+ *   SOD              none (Sod shock tube along x, split at MATLAB round(nx/2))
+ *   ORSZAG_TANG      none (periodic [0, 2pi)^2 with the x spacing of the grid)
+ *   SEDOV            radius in cells, ambient pressure, blast energy
+ *   RIEMANN          left rho, P, vx, vy, vz, By, Bz; right the same; Bx; split cell
+ *                    (< 0: nx / 2)
+ *   KELVIN_HELMHOLTZ Mach number, density ratio, perturbation amplitude, seed
+ * The formulas are those of the benchmark generators (imogen_amd/problems.py); the KH
+ * perturbation comes from a counter-based generator of the seed. */
+#define IMH_IC_SOD 1
+#define IMH_IC_ORSZAG_TANG 2
+#define IMH_IC_SEDOV 3
+#define IMH_IC_RIEMANN 4
+#define IMH_IC_KELVIN_HELMHOLTZ 5
+int imh_init_problem(imh_ctx *ctx, int kind, const double *params, int nparams);
+
 /* ---- checkpoint / resume --------------------------------------------------------------
  * No counterpart in the reference: its saves (save/resultsHandler.m:26-113) hold the
  * fields and the time only, so a restarted run loses the per-object edge trackers of
