@@ -98,6 +98,9 @@ SIGNATURES = {
     "imh_serve_destroy": (_I, [_P]),
     "imh_serve_submit": (_I, [_P, ctypes.POINTER(ImhConfig), _D, _D * 3, _D, _D * 3,
                               ctypes.c_size_t, ctypes.POINTER(ctypes.c_longlong)]),
+    "imh_serve_submit_problem": (_I, [_P, ctypes.POINTER(ImhConfig), ctypes.c_int,
+                                      ctypes.POINTER(ctypes.c_double), ctypes.c_int,
+                                      ctypes.POINTER(ctypes.c_longlong)]),
     "imh_serve_step": (_I, [_P, _I, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     "imh_serve_status": (_I, [_P, ctypes.c_longlong, ctypes.POINTER(_I), _D,
                               ctypes.POINTER(_I), _D]),

@@ -30,6 +30,8 @@ struct Request {
     imh_config cfg;
     std::vector<double> state;  // mass, mom x3, ener, mag x3 (8 x n), until admission
     size_t n = 0;
+    int ic_kind = 0;                  // > 0: the state is generated on the device
+    std::vector<double> ic_params;
     int status = IMH_REQ_QUEUED;
     imh_ctx* ctx = nullptr;
     size_t bytes = 0;
@@ -88,7 +90,9 @@ static int admit(imh_server* s) {
         const long long id = s->queue.front();
         s->queue.pop_front();
         int rc = imh_create(&r.cfg, &r.ctx);
-        if (rc == IMH_OK) {
+        if (rc == IMH_OK && r.ic_kind > 0) {
+            rc = imh_init_problem(r.ctx, r.ic_kind, r.ic_params.data(), (int)r.ic_params.size());
+        } else if (rc == IMH_OK) {
             const double* d = r.state.data();
             const size_t n = r.n;
             const double* mom[3] = {d + n, d + 2 * n, d + 3 * n};
@@ -219,6 +223,31 @@ int imh_serve_submit(imh_server* s, const imh_config* cfg, const double* mass,
         if (!src[f]) return fail(IMH_EINVAL, "null field %d", f);
         std::memcpy(r.state.data() + f * n, src[f], n * sizeof(double));
     }
+    r.t_submit = now_s(s);
+    *id = s->next_id++;
+    s->reqs[*id] = std::move(r);
+    s->queue.push_back(*id);
+    return IMH_OK;
+}
+
+int imh_serve_submit_problem(imh_server* s, const imh_config* cfg, int kind,
+                             const double* params, int nparams, long long* id) {
+    if (!s || !cfg || !id || nparams < 0 || (nparams > 0 && !params))
+        return fail(IMH_EINVAL, "bad argument");
+    if (kind < IMH_IC_SOD || kind > IMH_IC_KELVIN_HELMHOLTZ)
+        return fail(IMH_EINVAL, "unknown initial-condition kind %d", kind);
+    imh_config c = *cfg;
+    c.precision = s->precision;
+    c.device = s->device;
+    size_t bytes = 0;
+    int rc = imh_memory_estimate(&c, &bytes);
+    if (rc) return rc;
+    Request r;
+    r.cfg = c;
+    r.n = (size_t)c.grid[0] * c.grid[1] * c.grid[2];
+    r.bytes = bytes;
+    r.ic_kind = kind;
+    r.ic_params.assign(params, params + nparams);
     r.t_submit = now_s(s);
     *id = s->next_id++;
     s->reqs[*id] = std::move(r);

@@ -57,6 +57,19 @@ class Server:
         self._cfg[rid.value] = rc
         return rid.value
 
+    def submit_problem(self, ini, kind, params=()):
+        """A request whose initial state is generated on the device at admission (kind:
+        sod, orszag_tang, sedov, riemann, kelvin_helmholtz; params as imh_init_problem)."""
+        from .imogen import ImogenManager
+        rc = from_ini(ini)
+        c = make_imh_config(rc, self.precision)
+        arr = (ctypes.c_double * max(1, len(params)))(*[float(p) for p in params])
+        rid = ctypes.c_longlong()
+        check(lib().imh_serve_submit_problem(self._s, ctypes.byref(c), ImogenManager.IC_KINDS[kind], arr,
+                                             len(params), ctypes.byref(rid)))
+        self._cfg[rid.value] = rc
+        return rid.value
+
     def step(self, iterations):
         """One scheduling round; returns (running, queued)."""
         a, q = ctypes.c_int(), ctypes.c_int()

@@ -232,6 +232,10 @@ int imh_serve_destroy(imh_server *s);
 int imh_serve_submit(imh_server *s, const imh_config *cfg, const double *mass,
                      const double *const mom[3], const double *ener,
                      const double *const mag[3], size_t n, long long *id);
+/* A request whose initial state is generated on the device at admission (kind and params
+ * as imh_init_problem takes them): no host state, no host-to-device copy. */
+int imh_serve_submit_problem(imh_server *s, const imh_config *cfg, int kind,
+                             const double *params, int nparams, long long *id);
 /* One scheduling round: admit, advance the running batch by `iterations` iterations,
  * retire the finished requests and admit into the freed slots. */
 int imh_serve_step(imh_server *s, int iterations, int *active, int *queued);

@@ -88,3 +88,37 @@ def test_admission_by_budget_and_slots():
     for k in range(2):
         assert ev[k + 1][1] >= ev[k][2]  # admitted after the previous one finished
     srv.close()
+
+
+def test_device_generated_requests():
+    """Requests whose initial state is generated on the device at admission give the
+    result of a run initialised with init_problem alone (exactly), mixed with host-state
+    requests."""
+    import imogen_amd
+    sod_ini, _ = problems.sod(300, time_max=0.05)
+    ot_ini, _ = problems.orszag_tang(40)
+    sed_ini, _ = problems.sedov(16, time_max=0.02)
+    dev = [(sod_ini, "sod", ()), (ot_ini, "orszag_tang", ()), (sed_ini, "sedov", (3.0, 1e-5, 1.0))]
+    dev = [(dict(ini, iterMax=30), k, p) for ini, k, p in dev]
+    host = limited([problems.mhd_riemann(400, seed=1)], 30)
+    srv = Server("fp32", max_active=3)
+    ids = [srv.submit_problem(ini, k, p) for ini, k, p in dev]
+    hid = srv.submit(*host[0])
+    for _ in range(300):
+        running, queued = srv.step(4)
+        if running == 0 and queued == 0:
+            break
+    for rid, (ini, k, p) in zip(ids, dev):
+        status, t, it, ev = srv.status(rid)
+        assert status == DONE, (rid, status)
+        run = imogen_amd.initialize(ini, "fp32")
+        run.init_problem(k, p)
+        run.run_iterations(10 ** 6)
+        assert it == run.time.iteration and t == run.time.time
+        for f in ("mass", "momx", "ener", "magy"):
+            assert rel_linf(srv.download(rid, f), run.download(f)) == 0.0, (k, f)
+        run.close()
+        srv.release(rid)
+    assert srv.status(hid)[0] == DONE
+    srv.release(hid)
+    srv.close()
