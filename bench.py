@@ -361,7 +361,9 @@ SERVE_3D = [64, 96, 128, 192]
 
 def serve_requests(n, seed):
     """Solve requests of the serving stream: Sod, Orszag-Tang and Kelvin-Helmholtz on
-    64^2..1024^2 and Sedov on 64^3..192^3, each run to a fixed end time, seeded ICs."""
+    64^2..1024^2 and Sedov on 64^3..192^3, each run to a fixed end time, seeded; the
+    initial states are generated on the device at admission (imh_serve_submit_problem),
+    so a request carries only its configuration."""
     from imogen_amd import problems
     rng = np.random.default_rng(seed)
     reqs, kinds = [], []
@@ -370,19 +372,27 @@ def serve_requests(n, seed):
         big = 10 ** 8
         if kind == "sedov":
             m = int(rng.choice(SERVE_3D, p=[0.4, 0.3, 0.2, 0.1]))
-            ini, st = problems.sedov(m, time_max=5e-3, iter_max=big)
+            ini, _ = problems.sedov(1, time_max=5e-3, iter_max=big)
+            r = {"kind": "sedov", "params": (3.0, 1e-5, 1.0)}
+            dims = (m, m, m)
         else:
             m = int(rng.choice(SERVE_2D, p=[0.3, 0.3, 0.2, 0.15, 0.05]))
+            dims = (m, m, 1)
             if kind == "sod":
-                ini, st = problems.sod(m, ny=m, bc="const", iter_max=big, time_max=0.2)
+                ini, _ = problems.sod(1, ny=1, bc="const", iter_max=big, time_max=0.2)
+                r = {"kind": "sod"}
             elif kind == "ot":
-                ini, st = problems.orszag_tang(m, iter_max=big)
+                ini, _ = problems.orszag_tang(1, iter_max=big)
                 ini["timeMax"] = 0.5
+                r = {"kind": "orszag_tang"}
             else:
-                ini, st = problems.kelvin_helmholtz(m, seed=int(rng.integers(1 << 30)),
-                                                    iter_max=big)
+                ini, _ = problems.kelvin_helmholtz(1, iter_max=big)
                 ini["timeMax"] = 0.2
-        reqs.append((ini, st))
+                r = {"kind": "kelvin_helmholtz",
+                     "params": (0.5, 8.0, 0.01, int(rng.integers(1 << 30)))}
+        ini = problems.resize(ini, dims)
+        r["ini"] = ini
+        reqs.append(r)
         kinds.append("%s%d" % (kind, m))
     return reqs, kinds
 
@@ -555,6 +565,7 @@ def main():
         out["serving"] = {
             "workload": "Poisson arrivals of Sod / Orszag-Tang / KH on 64^2..1024^2 and "
                         "Sedov on 64^3..192^3, each to a fixed end time; FP32 bulk, "
+                        "initial states generated on the device at admission, "
                         "HBM-budget admission, continuous batching",
             "latency": "completion - arrival, host wall clock",
             "runs": serving_leg(args)}

@@ -49,6 +49,16 @@ def _base_ini(grid, dgrid, gamma, cfl, bc, limiter, magnet, iter_max=1000, time_
     }
 
 
+def resize(ini, grid):
+    """The configuration of the same problem (same domain) on another grid: the spacing
+    scales with the cell count of each dimension."""
+    out = dict(ini)
+    old = ini["grid"]
+    out["grid"] = tuple(int(g) for g in grid)
+    out["dGrid"] = tuple(float(ini["dGrid"][d]) * old[d] / grid[d] for d in range(3))
+    return out
+
+
 def _zeros(shape):
     return np.zeros(shape, dtype=np.float64)
 

@@ -104,7 +104,9 @@ class Server:
 
 def run_stream(server, requests, arrivals, iterations_per_round=4, keep=None, timeout=600.0):
     """Submit requests[i] at wall-clock offset arrivals[i] (seconds from the start), run
-    scheduling rounds until every request has finished, and return a report.
+    scheduling rounds until every request has finished, and return a report.  A request
+    is (ini, state) for a host initial state, or {"ini", "kind", "params"} for one
+    generated on the device at admission.
 
     ``keep``: optional set of request indices whose final density is downloaded (into
     report["results"]) before release; all other results are released on completion."""
@@ -121,7 +123,9 @@ def run_stream(server, requests, arrivals, iterations_per_round=4, keep=None, ti
             raise TimeoutError("serving stream did not finish in %.0f s" % timeout)
         while nxt < len(order) and arrivals[order[nxt]] <= now:
             i = int(order[nxt])
-            ids[i] = server.submit(*requests[i])
+            r = requests[i]
+            ids[i] = (server.submit_problem(r["ini"], r["kind"], r.get("params", ()))
+                      if isinstance(r, dict) else server.submit(*r))
             submitted_at[i] = time.perf_counter() - t0
             pending.add(i)
             nxt += 1
