@@ -8,6 +8,6 @@ of the reference's functions.
 from .config import ENUM, FluxLimiterEnum, RunConfig, from_ini  # noqa: F401
 from ._lib import CFLError, ImogenError  # noqa: F401
 from .imogen import (FluidArray, ImogenManager, MagnetArray, create_arrays, flux,  # noqa: F401
-                     imogen, initialize, iteration, magnetFlux, relaxingFluid)
+                     imogen, initialize, iteration, magnetFlux, relaxingFluid, source)
 
 __version__ = "0.1.0"

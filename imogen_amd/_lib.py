@@ -82,6 +82,10 @@ SIGNATURES = {
     "imh_memory_bytes": (_I, [_P, ctypes.POINTER(ctypes.c_size_t)]),
     "imh_memory_estimate": (_I, [ctypes.POINTER(ImhConfig), ctypes.POINTER(ctypes.c_size_t)]),
     "imh_init_problem": (_I, [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_int]),
+    "imh_source": (_I, [_P]),
+    "imh_set_gravity": (_I, [_P, _D, ctypes.c_size_t, ctypes.c_double]),
+    "imh_set_radiation": (_I, [_P, _I, ctypes.c_double, ctypes.c_double]),
+    "imh_get_radiation_strength": (_I, [_P, _D]),
     "imh_checkpoint_bytes": (_I, [_P, ctypes.POINTER(ctypes.c_size_t)]),
     "imh_checkpoint_save": (_I, [_P, ctypes.c_void_p, ctypes.c_size_t]),
     "imh_checkpoint_load": (_I, [_P, ctypes.c_void_p, ctypes.c_size_t]),

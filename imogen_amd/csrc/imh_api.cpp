@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -210,6 +211,8 @@ static void ctx_free(imh_ctx* c) {
     if (c->fade_dev) hipFree(c->fade_dev);
     if (c->csum_part) hipFree(c->csum_part);
     if (c->csum_out) hipFree(c->csum_out);
+    if (c->grav) hipFree(c->grav);
+    if (c->grav_ghost) hipFree(c->grav_ghost);
     if (c->st && !c->external_stream) hipStreamDestroy(c->st);
     delete c;
 }
@@ -312,13 +315,13 @@ static long long edges_layout(const imh_config& cfg, long long eoff[NOBJ][2][3])
         for (int s = 0; s < 2; ++s)
             for (int d = 0; d < 3; ++d) eoff[o][s][d] = 0;
     auto exists = [&](int o) {
-        if (o < OBJ_MAG + 3) return true;
+        if (o < OBJ_MAG + 3 || o == OBJ_GRAV) return true;
         if (!cfg.magnet_active) return false;
         int k = (o - OBJ_VG) % 9;
         return (k / 3) != (k % 3);  // (I, X) pairs with I != X
     };
     for (int o = 0; o < NOBJ; ++o) {
-        if (o >= OBJ_MFLUX + 9) break;
+        if (o >= OBJ_MFLUX + 9 && o != OBJ_GRAV) continue;
         if (!exists(o)) continue;
         for (int s = 0; s < 2; ++s)
             for (int d = 0; d < ndims; ++d) {
@@ -845,6 +848,51 @@ int imh_flux_internal(imh_ctx* c, int order, int force) {
     return IMH_OK;
 }
 
+// ---- source terms (source.m:16-31) ----------------------------------------------------
+
+static void drop_graphs(imh_ctx* c) {
+    for (int i = 0; i < 6; ++i) {
+        if (c->graph[i]) hipGraphExecDestroy(c->graph[i]);
+        c->graph[i] = nullptr;
+    }
+}
+
+int imh_source_internal(imh_ctx* c) {
+    if (!c->grav && c->rad_type == 0) return IMH_OK;
+    SourceArgs a;
+    memset(&a, 0, sizeof(a));
+    a.phi = c->grav;
+    a.ghosts = c->grav_ghost;
+    a.edges = c->edges;
+    long long g = 0;
+    for (int d = 0; d < 3; ++d) {
+        a.n[d] = c->n[d];
+        a.nface[d] = c->nface[d];
+        a.dgrid[d] = c->cfg.dgrid[d];
+        a.active_lin[d] = c->active_lin[d];
+        a.goff[d] = g;
+        if (d < c->ndims) g += c->nface[d];
+        for (int s = 0; s < 2; ++s) {
+            const int m = c->cfg.bc[s][d];
+            a.modes[s][d] = m;
+            a.eoff[s][d] = (d < c->ndims && (m == IMH_BC_TRANS || m == IMH_BC_WALL))
+                               ? c->eoff[OBJ_GRAV][s][d] : -1;
+        }
+    }
+    a.ndims = c->ndims;
+    a.bc_inf = c->cfg.bc_infinity;
+    a.rad_type = c->rad_type;
+    a.sim = c->sim;
+    a.mass_threshold = c->mass_threshold;
+    a.gamma = c->cfg.gamma;
+    a.rad_strength = c->rad_strength;
+    a.rad_exponent = c->rad_exponent;
+    a.use = {!c->mag_is_zero[0], !c->mag_is_zero[1], !c->mag_is_zero[2]};
+    ProfScope ps(c, IMH_K_BOOK, 0.0);
+    hipError_t e = launch_source(a, c->f, c->cfg.precision == IMH_FP64, c->st);
+    return e == hipSuccess ? IMH_OK : hip_fail(e, "source launch");
+}
+
 // One imogen.m:61-71 loop body; `conditional` = 0 runs it unconditionally (imh_iteration).
 int imh_iteration_body(imh_ctx* c, int conditional) {
     int rc;
@@ -857,7 +905,9 @@ int imh_iteration_body(imh_ctx* c, int conditional) {
     for (int i = 1; i <= 2; ++i) {
         if ((rc = imh_time_update_internal(c, i))) return rc;
         if ((rc = imh_flux_internal(c, i == 1 ? 1 : -1, 0))) return rc;
-        // treadmill, gravity: inactive; source(): ener - dt*0 leaves every value unchanged
+        // treadmill: inactive; gravity.solvePotential: the fixed potential is set once;
+        // source(): without gravity and radiation, ener - dt*0 changes no value
+        if ((rc = imh_source_internal(c))) return rc;
     }
     {
         ProfScope ps(c, IMH_K_BOOK, 0.0);
@@ -892,6 +942,109 @@ int imh_flux(imh_ctx* c, int order) {
     if (!c) return fail(IMH_EINVAL, "null context");
     if (!c->have_state) return fail(IMH_ESTATE, "state not uploaded");
     return imh_flux_internal(c, order, 1);
+}
+
+int imh_source(imh_ctx* c) {
+    if (!c) return fail(IMH_EINVAL, "null context");
+    if (!c->have_state) return fail(IMH_ESTATE, "state not uploaded");
+    // a direct call behaves as inside an unconditional loop body
+    hipError_t e = launch_begin_iteration(c->sim_ptr, c->par_ptr, 1, 1, c->st);
+    if (e != hipSuccess) return hip_fail(e, "begin");
+    return imh_source_internal(c);
+}
+
+int imh_set_gravity(imh_ctx* c, const double* potential, size_t n, double mass_threshold) {
+    if (!c) return fail(IMH_EINVAL, "null context");
+    if (potential && n != (size_t)c->ncell)
+        return fail(IMH_EINVAL, "potential holds %zu values, the grid %lld", n, c->ncell);
+    if (potential) {
+        for (int d = 0; d < c->ndims; ++d) {
+            if (c->n[d] < 3)
+                return fail(IMH_ENOTSUP, "gravity needs >= 3 cells along dim %d", d + 1);
+            for (int s = 0; s < 2; ++s)
+                if (c->cfg.bc[s][d] == IMH_BC_FADE)
+                    return fail(IMH_ENOTSUP, "gravity with a fade boundary is not supported");
+        }
+    }
+    hipError_t e = hipStreamSynchronize(c->st);
+    if (e != hipSuccess) return hip_fail(e, "sync");
+    drop_graphs(c);
+    c->mass_threshold = mass_threshold;
+    if (!potential) {
+        if (c->grav) hipFree(c->grav);
+        if (c->grav_ghost) hipFree(c->grav_ghost);
+        c->grav = c->grav_ghost = nullptr;
+        return IMH_OK;
+    }
+    long long nghost = 0;
+    for (int d = 0; d < c->ndims; ++d) nghost += c->nface[d];
+    if (!c->grav && (e = hipMalloc(&c->grav, sizeof(double) * c->ncell)) != hipSuccess)
+        return hip_fail(e, "hipMalloc(potential)");
+    if (!c->grav_ghost &&
+        (e = hipMalloc(&c->grav_ghost, sizeof(double) * 6 * (nghost > 0 ? nghost : 1))) != hipSuccess)
+        return hip_fail(e, "hipMalloc(potential ghosts)");
+    e = hipMemcpy(c->grav, potential, sizeof(double) * c->ncell, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, "potential upload");
+    // a new GravityArray: zeros are its first array, so its edge stores hold zeros
+    for (int s = 0; s < 2; ++s)
+        for (int d = 0; d < c->ndims; ++d) {
+            const int m = c->cfg.bc[s][d];
+            if (m != IMH_BC_TRANS && m != IMH_BC_WALL) continue;
+            e = hipMemset(c->edges + c->eoff[OBJ_GRAV][s][d], 0, sizeof(double) * c->nface[d]);
+            if (e != hipSuccess) return hip_fail(e, "potential edges");
+        }
+    return IMH_OK;
+}
+
+int imh_set_radiation(imh_ctx* c, int type, double exponent, double initial_maximum) {
+    if (!c) return fail(IMH_EINVAL, "null context");
+    if (type != 0 && type != 1) return fail(IMH_EINVAL, "radiation type %d", type);
+    hipError_t e = hipStreamSynchronize(c->st);
+    if (e != hipSuccess) return hip_fail(e, "sync");
+    drop_graphs(c);
+    c->rad_type = type;
+    c->rad_exponent = exponent;
+    if (type == 0) {
+        c->rad_strength = 0.0;
+        return IMH_OK;
+    }
+    if (!c->have_state) return fail(IMH_ESTATE, "radiation needs the state first");
+    // Radiation.initialize (Radiation.m:61-77), from the current state with strength 1
+    const size_t n = (size_t)c->ncell;
+    std::vector<double> q[9];
+    const int fid[9] = {IMH_F_MASS, IMH_F_MOMX, IMH_F_MOMY, IMH_F_MOMZ, IMH_F_ENER,
+                        IMH_F_MAGX, IMH_F_MAGY, IMH_F_MAGZ, IMH_F_CMAGX};
+    for (int v = 0; v < 9; ++v) {
+        q[v].resize(n);
+        int rc = imh_download_field(c, fid[v], q[v].data(), n);
+        if (rc) return rc;
+    }
+    std::vector<double> cm[2];
+    for (int k = 0; k < 2; ++k) {
+        cm[k].resize(n);
+        int rc = imh_download_field(c, IMH_F_CMAGY + k, cm[k].data(), n);
+        if (rc) return rc;
+    }
+    const MagUse u = {!c->mag_is_zero[0], !c->mag_is_zero[1], !c->mag_is_zero[2]};
+    double mn = NAN;
+    for (size_t i = 0; i < n; ++i) {
+        const double rho = q[0][i];
+        const double pg = pressure_gas(rho, q[1][i], q[2][i], q[3][i], q[4][i], q[8][i], cm[0][i],
+                                       cm[1][i], u, c->cfg.gamma);
+        const double unscaled = 1.0 * pow(rho, 2.0 - exponent) * pow(pg, exponent);
+        const double ke = 0.5 * (q[1][i] * q[1][i] + q[2][i] * q[2][i] + q[3][i] * q[3][i]) / rho;
+        const double me = 0.5 * (q[5][i] * q[5][i] + q[6][i] * q[6][i] + q[7][i] * q[7][i]);
+        const double v = (q[4][i] - ke - me) / unscaled;
+        if (!std::isnan(v) && (std::isnan(mn) || v < mn)) mn = v;  // MATLAB min skips NaN
+    }
+    c->rad_strength = (initial_maximum / 100.0) * mn;
+    return IMH_OK;
+}
+
+int imh_get_radiation_strength(imh_ctx* c, double* strength) {
+    if (!c || !strength) return fail(IMH_EINVAL, "bad argument");
+    *strength = c->rad_strength;
+    return IMH_OK;
 }
 
 int imh_iteration(imh_ctx* c) {

@@ -30,7 +30,8 @@ enum ObjId : int {
     OBJ_MSFR = 37,   // mag(I).store(X).fluxR
     OBJ_WMAG = 46,   // mag(I).wMag(X)
     OBJ_MFLUX = 55,  // mag(I).flux(X)
-    NOBJ = 64
+    OBJ_GRAV = 64,   // grav (GravityArray)
+    NOBJ = 65
 };
 
 // FluxArray.fluxType for zero_shift.m.
@@ -299,6 +300,22 @@ __host__ __device__ inline double pressure_sound(double rho, double px, double p
     double r = (g - 1.0) * (E - 0.5 * rho * v2);
     r = r - (g - 1.0) * 0.5 * b2;
     r = sqrt(fabs((g * r + 2.0 * b2) / rho));
+    if (r < 0.0) r = 0.0;
+    return r;
+}
+
+// pressure.m 'gas': thermal pressure, floored at 0.
+__host__ __device__ inline double pressure_gas(double rho, double px, double py, double pz,
+                                               double E, double bx, double by, double bz,
+                                               MagUse u, double g) {
+    double v2 = (px * px + py * py + pz * pz) / (rho * rho);
+    double b2 = 0.0;
+    bool any = false;
+    if (u.x) { b2 = 0.0 + bx * bx; any = true; }
+    if (u.y) { b2 = any ? b2 + by * by : 0.0 + by * by; any = true; }
+    if (u.z) { b2 = any ? b2 + bz * bz : 0.0 + bz * bz; any = true; }
+    double r = (g - 1.0) * (E - 0.5 * rho * v2);
+    r = r - (g - 1.0) * 0.5 * b2;
     if (r < 0.0) r = 0.0;
     return r;
 }

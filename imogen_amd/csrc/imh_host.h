@@ -99,6 +99,13 @@ struct imh_ctx {
     char* cap_host = nullptr;           // host image of the captured row's slots
     char* cap_dev[6] = {};              // device, CAP_SLOTS slots per row
     imh::Profiler* prof = nullptr;      // set while imh_profile_iterations runs
+    // source terms (source.m): fixed gravitational potential and radiation
+    double* grav = nullptr;             // grav.array (FP64), nullptr: gravity off
+    double* grav_ghost = nullptr;       // 6 ghost values per pencil end (source.hip)
+    double mass_threshold = 0.0;        // run.fluid.MASS_THRESHOLD
+    int rad_type = 0;                   // 0 'empty', 1 'optically_thin'
+    double rad_exponent = 0.5;
+    double rad_strength = 0.0;
 };
 
 namespace imh {
@@ -127,6 +134,33 @@ hipError_t launch_ic_edges(const IcParams& P, int field, int side, int d, double
                            long long off, long long nface, hipStream_t st);
 }  // namespace imh
 
+// Arguments of the source step (source.hip); plain values, passed by value.
+struct SourceArgs {
+    const double* phi;        // potential, nullptr: no gravity
+    double* ghosts;           // [sum of nface over d < ndims][6]
+    double* edges;            // the context's edge stores
+    long long eoff[2][3];     // offsets of the potential's trackers, -1 where none
+    long long goff[3];        // first ghost face of each dim
+    long long nface[3];
+    int n[3];
+    int ndims;
+    int modes[2][3];
+    int active_lin[3];
+    int bc_inf;
+    int rad_type;
+    const imh::SimDev* sim;
+    double mass_threshold;
+    double dgrid[3];
+    double gamma;
+    double rad_strength;
+    double rad_exponent;
+    imh::MagUse use;
+};
+namespace imh {
+hipError_t launch_source(const SourceArgs& a, void* const f[IMH_NFIELDS], bool fp64,
+                         hipStream_t st);
+}  // namespace imh
+
 // internal entry points (force = run regardless of the loop's running flag)
 int imh_fluid_sweep_internal(imh_ctx* c, int X, int force);
 int imh_magnet_sweep_internal(imh_ctx* c, int X, const int I[2], int force);
@@ -143,4 +177,5 @@ imh::CellMagArgs<T> make_cellmag_args(const imh_ctx* c, int k, int force);
 template <typename T>
 imh::CflArgs<T> make_cfl_args(const imh_ctx* c);
 int imh_iteration_body(imh_ctx* c, int conditional);
+int imh_source_internal(imh_ctx* c);
 void imh_ctx_free(imh_ctx* c);

@@ -203,6 +203,30 @@ class ImogenManager:
         arr = (ctypes.c_double * max(1, len(params)))(*[float(p) for p in params])
         check(lib().imh_init_problem(self._ctx, self.IC_KINDS[kind], arr, len(params)))
 
+    # -- source terms (source.m) ------------------------------------------------------
+    def set_gravity(self, potential, mass_threshold=0.0):
+        """run.gravity active with the 'empty' solver: grav.array = fixedPotential (a host
+        array of the grid's shape; None turns gravity off) and run.fluid.MASS_THRESHOLD."""
+        if potential is None:
+            check(lib().imh_set_gravity(self._ctx, None, 0, float(mass_threshold)))
+            return
+        arr = as_field(potential, self.config.ncell)
+        check(lib().imh_set_gravity(self._ctx, dptr(arr), arr.size, float(mass_threshold)))
+
+    RADIATION = {"empty": 0, "optically_thin": 1}
+
+    def set_radiation(self, kind, exponent=0.5, initial_maximum=1.0):
+        """run.fluid.radiation type, exponent and initialMaximum; the strength follows from
+        the current state (Radiation.initialize), so call it after the state is set."""
+        check(lib().imh_set_radiation(self._ctx, self.RADIATION[kind], float(exponent),
+                                      float(initial_maximum)))
+
+    @property
+    def radiation_strength(self):
+        v = ctypes.c_double()
+        check(lib().imh_get_radiation_strength(self._ctx, ctypes.byref(v)))
+        return v.value
+
     def checkpoint(self):
         """The whole resumable state (fields, edge trackers, flux-object flags, time) as
         bytes; `restore` on a run of the same configuration continues bit for bit."""
@@ -310,6 +334,12 @@ def flux(run, mass, mom, ener, mag, grav, order):
     if order not in (1, -1):
         raise ValueError("%g is not a recognized direction. Fluxing aborted." % order)
     check(lib().imh_flux(run._ctx, int(order)))
+
+
+def source(run, mass, mom, ener, mag, grav):
+    """source.m: gravity from the fixed potential and the radiation sink."""
+    _check_run(run, mass, ener)
+    check(lib().imh_source(run._ctx))
 
 
 def iteration(run):

@@ -137,6 +137,28 @@ int imh_iteration(imh_ctx *ctx);
  * launch sequence is captured once into a hipGraph and replayed. */
 int imh_run(imh_ctx *ctx, int max_iterations, int *done);
 
+/* ---- source terms ------------------------------------------------------------------
+ * source(run,mass,mom,ener,mag,grav) -- source.m:1-31, run after every flux call of
+ * imh_iteration / imh_run (imogen.m:63-66) and callable on its own as imh_source.
+ * Gravity: GravityManager active with the 'empty' solver, whose potential is the fixed
+ * one (GravityManager.m:104-116; initialize.m:343-354 ini.gravity.fixedPotential):
+ * mom(i) -= dt * mass.thresholdArray .* grav.calculate5PtDerivative(i, DGRID{i}) and the
+ * matching energy change (source.m:16-25).  imh_set_gravity copies the potential
+ * (column-major, n cells, FP64; NULL turns gravity off) and sets run.fluid.MASS_THRESHOLD
+ * (initialize.m:72-77); the potential's edge stores start at zero, as a new GravityArray's
+ * do.  Grids need >= 3 cells along every active dimension; fade boundaries are not
+ * supported with gravity.  The Poisson solvers stay out (a solved potential is uploaded
+ * with imh_set_gravity).
+ * Radiation: run.fluid.radiation (Radiation.m): type 0 'empty', 1 'optically_thin'
+ * with exponent and initialMaximum (initialize.m:403-409); imh_set_radiation computes
+ * the strength from the current state as Radiation.initialize does (Radiation.m:61-77),
+ * so it is called after the state is in place.  Sources run on single contexts; a batch
+ * holding a context with sources returns IMH_ENOTSUP. */
+int imh_source(imh_ctx *ctx);
+int imh_set_gravity(imh_ctx *ctx, const double *potential, size_t n, double mass_threshold);
+int imh_set_radiation(imh_ctx *ctx, int type, double exponent, double initial_maximum);
+int imh_get_radiation_strength(imh_ctx *ctx, double *strength);
+
 /* ---- time state -------------------------------------------------------------------- */
 int imh_set_dtime(imh_ctx *ctx, double dt);
 /* Blocks until the stream is idle, then reads the device time state. */

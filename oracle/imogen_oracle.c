@@ -61,6 +61,11 @@ struct orc_sim {
     int iteration;
     int grid_index;
     double *fade;                  /* fade_shift.m:27 coefficients, bc_infinity long */
+    obj_t grav;                    /* GravityArray (InitializedArray) */
+    double *grav_potential;        /* run.gravity.fixedPotential, NULL: gravity inactive */
+    double mass_threshold;         /* run.fluid.MASS_THRESHOLD */
+    int rad_type;                  /* 0 'empty', 1 'optically_thin' */
+    double rad_exponent, rad_strength;
     int have_state;
     char err[256];
 };
@@ -406,7 +411,7 @@ static void do_shift(orc_sim *s, obj_t *o, int D, int n, double *out) {
 /* pressure.m                                                                            */
 /* ------------------------------------------------------------------------------------ */
 
-/* mode 0 = 'totsnd' (result = total pressure, aux = fast speed), 1 = 'sound'. */
+/* mode 0 = 'totsnd' (result = total pressure, aux = fast speed), 1 = 'sound', 2 = 'gas'. */
 static void pressure_arr(int mode, long n, double g, const double *rho, const double *px,
                          const double *py, const double *pz, const double *ener,
                          const double *bx, const double *by, const double *bz,
@@ -422,6 +427,8 @@ static void pressure_arr(int mode, long n, double g, const double *rho, const do
         if (mode == 0) {
             aux[c] = sqrt(fabs((g * (r - (g - 1.0) * 0.5 * b2) + 2.0 * b2) / rho[c]));
             r = r + 0.5 * (2.0 - g) * b2;
+        } else if (mode == 2) { /* 'gas' */
+            r = r - (g - 1.0) * 0.5 * b2;
         } else {
             r = r - (g - 1.0) * 0.5 * b2;
             r = sqrt(fabs((g * r + 2.0 * b2) / rho[c]));
@@ -882,17 +889,152 @@ int orc_time_update(orc_sim *s, int i) {
     return 0;
 }
 
+/* ------------------------------------------------------------------------------------ */
+/* source.m: gravity (fixed potential) and radiation                                     */
+/* ------------------------------------------------------------------------------------ */
+
+/* GravityArray(ENUM.GRAV, run) assigns zeros first, so its edge stores are zero
+ * (GravityArray.m:31-35, InitializedArray.m); GravityManager.solvePotential then sets
+ * grav.array = fixedPotential at iteration 0 ('empty' solver, GravityManager.m:104-116). */
+static void grav_create(orc_sim *s) {
+    obj_free(&s->grav);
+    obj_init(&s->grav, K_INIT, 0, FT_OTHER);
+    double *zeros = alloc_arr(s->ncell);
+    memset(zeros, 0, sizeof(double) * (size_t)s->ncell);
+    obj_assign(s, &s->grav, zeros);
+    obj_assign(s, &s->grav, dup_arr(s->grav_potential, s->ncell));
+}
+
+int orc_set_gravity(orc_sim *s, const double *potential, double mass_threshold) {
+    free(s->grav_potential);
+    s->grav_potential = NULL;
+    obj_free(&s->grav);
+    s->mass_threshold = mass_threshold;
+    if (!potential) return 0;
+    for (int d = 0; d < s->ndims; ++d)
+        for (int side = 0; side < 2; ++side) {
+            if (s->n[d] < 3) {
+                snprintf(s->err, sizeof(s->err), "gravity needs >= 3 cells along dim %d", d + 1);
+                return -1;
+            }
+            if (s->cfg.bc[side][d] == ORC_BC_FADE) {
+                snprintf(s->err, sizeof(s->err), "gravity with a fade BC is not supported");
+                return -1;
+            }
+        }
+    s->grav_potential = dup_arr(potential, s->ncell);
+    grav_create(s);
+    return 0;
+}
+
+/* Radiation.initialize (Radiation.m:61-77): strength from the current state, the
+ * unscaled radiation evaluated with the constructor's strength of 1. */
+int orc_set_radiation(orc_sim *s, int type, double exponent, double initial_maximum) {
+    s->rad_type = type;
+    s->rad_exponent = exponent;
+    if (type == 0) {
+        s->rad_strength = 0.0;
+        return 0;
+    }
+    if (!s->have_state) {
+        snprintf(s->err, sizeof(s->err), "radiation needs the state first");
+        return -1;
+    }
+    long n = s->ncell;
+    double g = s->cfg.gamma;
+    int use[3] = {!s->mag_is_zero[0], !s->mag_is_zero[1], !s->mag_is_zero[2]};
+    double *pg = alloc_arr(n);
+    pressure_arr(2, n, g, s->prim[0].a, s->prim[1].a, s->prim[2].a, s->prim[3].a, s->prim[4].a,
+                 s->cmag[0].a, s->cmag[1].a, s->cmag[2].a, use, pg, NULL);
+    double mn = NAN;
+    for (long c = 0; c < n; ++c) {
+        const double rho = s->prim[0].a[c];
+        const double *m1 = s->prim[1].a, *m2 = s->prim[2].a, *m3 = s->prim[3].a;
+        const double *b1 = s->mag[0].a, *b2 = s->mag[1].a, *b3 = s->mag[2].a;
+        double unscaled = 1.0 * pow(rho, 2.0 - exponent) * pow(pg[c], exponent);
+        double ke = 0.5 * (m1[c] * m1[c] + m2[c] * m2[c] + m3[c] * m3[c]) / rho;
+        double me = 0.5 * (b1[c] * b1[c] + b2[c] * b2[c] + b3[c] * b3[c]);
+        double v = (s->prim[4].a[c] - ke - me) / unscaled;
+        if (!isnan(v) && (isnan(mn) || v < mn)) mn = v; /* MATLAB min ignores NaN */
+    }
+    free(pg);
+    s->rad_strength = (initial_maximum / 100.0) * mn;
+    return 0;
+}
+
+double orc_radiation_strength(const orc_sim *s) { return s->rad_strength; }
+
+/* source(run, mass, mom, ener, mag, grav) (source.m:16-31). */
+int orc_source(orc_sim *s) {
+    long n = s->ncell;
+    double dt = s->dtime;
+    if (s->grav_potential) {
+        double *a2 = alloc_arr(n), *a1 = alloc_arr(n), *b1 = alloc_arr(n), *b2 = alloc_arr(n);
+        double *es = alloc_arr(n), *thr = alloc_arr(n);
+        const double *rho = s->prim[0].a;
+        double th = s->mass_threshold;
+        /* mass.thresholdArray (FluidArray.m:29-34) */
+        EW_LOOP(n) {
+            es[c] = 0.0;
+            thr[c] = th > 0 ? rho[c] * ((rho[c] > th) ? 1.0 : 0.0) : rho[c];
+        }
+        for (int i = 1; i <= 3; ++i) {
+            /* grav.calculate5PtDerivative(i, DGRID{i}) (ImogenArray.m:189-192): the
+             * shifts in the order the expression evaluates them */
+            do_shift(s, &s->grav, i, 2, a2);
+            do_shift(s, &s->grav, i, 1, a1);
+            do_shift(s, &s->grav, i, -1, b1);
+            do_shift(s, &s->grav, i, -2, b2);
+            double den = 12.0 * s->cfg.dgrid[i - 1];
+            const double *mom = s->prim[i].a;
+            double *nm = alloc_arr(n);
+            EW_LOOP(n) {
+                double d5 = (((-a2[c] + 8.0 * a1[c]) - 8.0 * b1[c]) + b2[c]) / den;
+                double ms = (dt * thr[c]) * d5;
+                es[c] = es[c] + ms * mom[c] / rho[c];
+                nm[c] = mom[c] - ms;
+            }
+            obj_assign(s, &s->prim[i], nm);
+        }
+        const double *e = s->prim[4].a;
+        double *ne = alloc_arr(n);
+        EW_LOOP(n) ne[c] = e[c] - es[c];
+        obj_assign(s, &s->prim[4], ne);
+        free(a2);
+        free(a1);
+        free(b1);
+        free(b2);
+        free(es);
+        free(thr);
+    }
+    double *e = s->prim[4].a;
+    if (s->rad_type == 1) {
+        /* Radiation.opticallyThinSolver (Radiation.m:86-89) */
+        int use[3] = {!s->mag_is_zero[0], !s->mag_is_zero[1], !s->mag_is_zero[2]};
+        double *pg = alloc_arr(n);
+        pressure_arr(2, n, s->cfg.gamma, s->prim[0].a, s->prim[1].a, s->prim[2].a,
+                     s->prim[3].a, e, s->cmag[0].a, s->cmag[1].a, s->cmag[2].a, use, pg, NULL);
+        const double *rho = s->prim[0].a;
+        double st = s->rad_strength, ex = s->rad_exponent;
+        EW_LOOP(n) {
+            double rad = st * pow(rho[c], 2.0 - ex) * pow(pg[c], ex);
+            e[c] = e[c] - dt * rad;
+        }
+        free(pg);
+    } else {
+        /* 'empty' radiation subtracts dt*0 */
+        EW_LOOP(n) e[c] = e[c] - dt * 0.0;
+    }
+    return 0;
+}
+
 int orc_iteration(orc_sim *s) {
     static const int direction[2] = {1, -1};
     for (int i = 1; i <= 2; ++i) {
         if (orc_time_update(s, i)) return -1;
         if (orc_flux(s, direction[i - 1])) return -1;
-        /* treadmillGrid, gravity.solvePotential: inactive; source(): radiation 'empty'
-         * subtracts dt*0 from the energy. */
-        double *e = s->prim[4].a;
-        double dt = s->dtime;
-        long n = s->ncell;
-        EW_LOOP(n) e[c] = e[c] - dt * 0.0;
+        /* treadmillGrid: inactive; gravity.solvePotential: the fixed potential is set once */
+        if (orc_source(s)) return -1;
     }
     s->iteration += 1;
     return 0;
@@ -967,6 +1109,8 @@ static void all_objs(orc_sim *s, void (*fn)(obj_t *)) {
 void orc_destroy(orc_sim *s) {
     if (!s) return;
     all_objs(s, obj_free);
+    obj_free(&s->grav);
+    free(s->grav_potential);
     free(s->fade);
     free(s);
 }
@@ -1027,6 +1171,7 @@ int orc_set_state(orc_sim *s, const double *mass, const double *const mom[3],
         obj_init(&s->cmag[k], K_INIT, k + 1, FT_OTHER);
         update_cell_centered(s, k);
     }
+    if (s->grav_potential) grav_create(s); /* a new run builds a new GravityArray */
     s->have_state = 1;
     return 0;
 }

@@ -17,6 +17,8 @@
  *   classes/InitializedArray.m:43-47        edge store created at first assignment
  *   shifting/{circ,constant,transparent,fade,mirror,flip,wall,zero}_shift.m, Edges.m:37-142
  *   time/TimeManager.m:55-99, utils/maxFinderND.m:15-19   CFL reduction
+ *   source.m:16-31, classes/ImogenArray.m:189-192, fluid/FluidArray.m:29-34,
+ *   gravity/GravityManager.m:104-116, fluid/Radiation.m:61-89   source step
  *
  * Every array operation keeps MATLAB's left-to-right evaluation order so that the
  * rounding is the one the .m text prescribes.  Build without FP contraction
@@ -34,6 +36,10 @@
  *   sweeps         uniform state preserved exactly; periodic BCs conserve sum(q) to rounding;
  *                  mirror symmetry; Sod exact solution L1 convergence; golden vectors from
  *                  the independent NumPy restatement in tests/golden/make_golden.py.
+ *   source         5-point derivative exact on cubic potentials (interior cells);
+ *                  momentum change -dt*rho*dphi/dx and energy change from the old momentum;
+ *                  mass threshold; dimensions beyond ndims; radiation strength and sink
+ *                  closed forms on uniform states.
  *   CFL            closed form on a uniform state.
  * Parity unpinned (no MATLAB anywhere): MATLAB's own perms row order (two orders are
  * offered, see orc_config.perms_order) and MATLAB pchip's last-bit rounding.
@@ -101,6 +107,17 @@ int orc_relaxing_fluid(orc_sim *s, int X);                    /* relaxingFluid(.
 int orc_magnet_flux(orc_sim *s, int X, const int I[2]);       /* magnetFlux(...,X,I2) */
 int orc_iteration(orc_sim *s);                                /* imogen.m:61-71 body */
 int orc_running(const orc_sim *s);                            /* TimeManager.running */
+
+/* Source terms (source.m:16-31).  orc_set_gravity: GravityManager active with the
+ * 'empty' solver, grav.array = fixedPotential (column-major [nx ny nz]; NULL turns gravity
+ * off) and run.fluid.MASS_THRESHOLD.  orc_set_radiation: run.fluid.radiation type (0
+ * 'empty', 1 'optically_thin'), exponent, initialMaximum; the strength is computed from
+ * the current state as Radiation.initialize does.  orc_source runs after every flux call
+ * inside orc_iteration. */
+int orc_set_gravity(orc_sim *s, const double *potential, double mass_threshold);
+int orc_set_radiation(orc_sim *s, int type, double exponent, double initial_maximum);
+double orc_radiation_strength(const orc_sim *s);
+int orc_source(orc_sim *s);
 
 /* Time state. */
 double orc_time(const orc_sim *s);

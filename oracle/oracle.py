@@ -82,6 +82,12 @@ def lib():
                                      ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                      ctypes.c_int, P, P, P, P, ctypes.c_int]
         L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_set_gravity.argtypes = [ctypes.c_void_p, P, ctypes.c_double]
+        L.orc_set_radiation.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
+                                        ctypes.c_double]
+        L.orc_radiation_strength.argtypes = [ctypes.c_void_p]
+        L.orc_radiation_strength.restype = ctypes.c_double
+        L.orc_source.argtypes = [ctypes.c_void_p]
         _lib = L
     return _lib
 
@@ -195,6 +201,26 @@ class Oracle:
 
     def iteration(self):
         self._check(self.L.orc_iteration(self.h))
+
+    def set_gravity(self, potential, mass_threshold=0.0):
+        """GravityManager with the 'empty' solver: grav = fixedPotential (None: off)."""
+        if potential is None:
+            self._check(self.L.orc_set_gravity(self.h, None, float(mass_threshold)))
+            return
+        v = np.ascontiguousarray(potential, dtype=np.float64).reshape(self.shape)
+        self._check(self.L.orc_set_gravity(self.h, _ptr(v), float(mass_threshold)))
+
+    def set_radiation(self, kind, exponent=0.5, initial_maximum=1.0):
+        """Radiation type ('empty' / 'optically_thin'), strength from the current state."""
+        t = {"empty": 0, "optically_thin": 1}[kind] if isinstance(kind, str) else int(kind)
+        self._check(self.L.orc_set_radiation(self.h, t, float(exponent), float(initial_maximum)))
+
+    @property
+    def radiation_strength(self):
+        return self.L.orc_radiation_strength(self.h)
+
+    def source(self):
+        self._check(self.L.orc_source(self.h))
 
     @property
     def running(self):

@@ -1,0 +1,160 @@
+"""The oracle's source step (source.m:16-31) against what the mathematics fixes.
+
+Gravity: the 5-point centred derivative is exact for polynomials up to degree 4, so on a
+cubic potential the momentum change of interior cells is -dt*rho*phi'(x) to rounding, and
+the energy change is -dt*rho*phi'*v with the momentum before the update.  Cells at or
+below MASS_THRESHOLD feel no gravity.  Radiation: on a uniform state at rest the
+strength is initialMaximum/100 * e/(rho^(2-a) P^a) and one step removes
+dt*initialMaximum/100*e of the energy density.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+GAMMA = 5.0 / 3.0
+
+
+def uniform(grid, rho=1.0, P=1.0, v=(0.0, 0.0, 0.0)):
+    shape = (grid[2], grid[1], grid[0])
+    mass = np.full(shape, rho)
+    mom = [mass * vi for vi in v]
+    ener = P / (GAMMA - 1.0) + 0.5 * rho * sum(vi * vi for vi in v) + 0 * mass
+    mag = [np.zeros(shape) for _ in range(3)]
+    return mass, mom, ener, mag
+
+
+def sim(grid, dgrid, bc="circ", threshold=0.0, potential=None, state=None, dt=1e-3):
+    o = O.Oracle(O.make_config(grid, dgrid, gamma=GAMMA, bc=bc, bc_infinity=20))
+    o.set_state(*(state or uniform(grid)))
+    if potential is not None:
+        o.set_gravity(potential, threshold)
+    o.L.orc_set_dtime(o.h, dt)
+    return o
+
+
+def coords(grid, dgrid):
+    x = (np.arange(grid[0]) + 0.5) * dgrid[0]
+    y = (np.arange(grid[1]) + 0.5) * dgrid[1]
+    return np.meshgrid(x, y, indexing="xy")  # (ny, nx)
+
+
+@pytest.mark.parametrize("bc", ["circ", "const", "trans"])
+def test_cubic_potential_gives_exact_interior_force(bc):
+    grid, dgrid, dt = (32, 24, 1), (0.1, 0.2, 0.1), 1e-3
+    X, Y = coords(grid, dgrid)
+    phi = (X ** 3 - 2.0 * X + 0.5 * Y ** 2)[None]
+    v = (0.3, -0.2, 0.1)
+    st = uniform(grid, rho=1.3, P=1.0, v=v)
+    o = sim(grid, dgrid, bc, potential=phi, state=st, dt=dt)
+    before = o.state()
+    o.source()
+    after = o.state()
+    gx = (3.0 * X ** 2 - 2.0)[None]
+    gy = Y[None]
+    inner = (slice(None), slice(2, -2), slice(2, -2))
+    dmx = (after["momx"] - before["momx"])[inner]
+    dmy = (after["momy"] - before["momy"])[inner]
+    np.testing.assert_allclose(dmx, (-dt * 1.3 * gx)[inner], rtol=1e-9, atol=1e-14)
+    np.testing.assert_allclose(dmy, (-dt * 1.3 * gy)[inner], rtol=1e-9, atol=1e-14)
+    # no z extent: the z shifts return the array, the derivative is rounding only
+    assert np.max(np.abs(after["momz"] - before["momz"])) < 1e-15
+    # energy: -sum_i momSource_i * v_i with the old momentum
+    de = (after["ener"] - before["ener"])[inner]
+    expect = -dt * 1.3 * (gx * v[0] + gy * v[1])[inner]
+    np.testing.assert_allclose(de, expect, rtol=1e-8, atol=1e-14)
+    np.testing.assert_array_equal(after["mass"], before["mass"])
+
+
+def test_mass_threshold_switches_gravity_off():
+    grid, dgrid = (20, 16, 1), (0.1, 0.1, 0.1)
+    X, Y = coords(grid, dgrid)
+    phi = (X * X)[None]
+    mass, mom, ener, mag = uniform(grid)
+    mass = mass.copy()
+    mass[0, :, :10] = 0.01
+    ener = ener + 0 * mass
+    o = sim(grid, dgrid, potential=phi, threshold=0.05, state=(mass, mom, ener, mag))
+    o.source()
+    mx = o.get("momx")
+    assert np.all(mx[0, :, :10] == 0.0)
+    assert np.all(mx[0, 2:-2, 12:-2] < 0.0)
+
+
+def test_zero_momentum_leaves_energy_unchanged():
+    grid, dgrid = (24, 20, 1), (0.1, 0.1, 0.1)
+    X, Y = coords(grid, dgrid)
+    phi = (np.sin(X) * np.cos(Y))[None]
+    o = sim(grid, dgrid, "trans", potential=phi)
+    e0 = o.get("ener")
+    o.source()
+    np.testing.assert_array_equal(o.get("ener"), e0)
+
+
+def test_transparent_potential_ghosts_follow_the_tracker():
+    """The potential's stored edges start at zero (a new GravityArray) and are nudged by
+    at most 0.005 per shift call, so the boundary force changes from call to call while
+    the interior force does not."""
+    grid, dgrid = (24, 20, 1), (0.1, 0.1, 0.1)
+    X, Y = coords(grid, dgrid)
+    phi = (1.0 + X)[None] + 0 * Y[None]
+    o = sim(grid, dgrid, "trans", potential=phi)
+    m0 = o.get("momx").copy()
+    o.source()
+    d1 = o.get("momx") - m0
+    m1 = o.get("momx").copy()
+    o.source()
+    d2 = o.get("momx") - m1
+    np.testing.assert_allclose(d1[:, :, 2:-2], d2[:, :, 2:-2], rtol=1e-12)
+    assert np.max(np.abs(d1[:, :, -1] - d2[:, :, -1])) > 0
+
+
+def test_wall_potential_ghosts_are_the_zero_first_array():
+    """A new GravityArray's first array is zeros, so its wall ghosts are 0: the upper
+    x-face cell sees d5 = (-8 phi[N-2] + phi[N-3]) / (12 dx), the lower one
+    (-phi[2] + 8 phi[1]) / (12 dx)."""
+    grid, dgrid, dt = (20, 16, 1), (0.1, 0.1, 0.1), 1e-3
+    X, Y = coords(grid, dgrid)
+    phi = (2.0 + X + 0.0 * Y)[None]
+    o = sim(grid, dgrid, {"x": "wall", "y": "circ"}, potential=phi, dt=dt)
+    m0 = o.get("momx").copy()
+    o.source()
+    d = o.get("momx") - m0
+    p = phi[0, 0]
+    hi = (((-0.0 + 8.0 * 0.0) - 8.0 * p[-2]) + p[-3]) / (12.0 * dgrid[0])
+    lo = (((-p[2] + 8.0 * p[1]) - 8.0 * 0.0) + 0.0) / (12.0 * dgrid[0])
+    np.testing.assert_allclose(d[0, :, -1], -dt * hi, rtol=1e-14)
+    np.testing.assert_allclose(d[0, :, 0], -dt * lo, rtol=1e-14)
+
+
+def test_radiation_strength_and_sink_closed_form():
+    grid, dgrid, dt = (16, 12, 1), (0.1, 0.1, 0.1), 2e-3
+    rho, P, a, imax = 1.7, 0.8, 0.5, 3.0
+    o = sim(grid, dgrid, state=uniform(grid, rho=rho, P=P), dt=dt)
+    o.set_radiation("optically_thin", a, imax)
+    e = P / (GAMMA - 1.0)
+    assert o.radiation_strength == pytest.approx(imax / 100.0 * e / (rho ** (2 - a) * P ** a),
+                                                 rel=1e-14)
+    o.source()
+    np.testing.assert_allclose(o.get("ener"), e - dt * imax / 100.0 * e, rtol=1e-14)
+
+
+def test_empty_sources_change_nothing():
+    grid, dgrid = (16, 12, 1), (0.1, 0.1, 0.1)
+    o = sim(grid, dgrid, state=uniform(grid, v=(0.2, 0.1, 0.0)))
+    before = o.state()
+    o.source()
+    after = o.state()
+    for f in before:
+        np.testing.assert_array_equal(after[f], before[f])
+
+
+def test_gravity_rejects_short_dimension_and_fade():
+    o = O.Oracle(O.make_config((16, 2, 1), (0.1, 0.1, 0.1), bc="circ"))
+    o.set_state(*uniform((16, 2, 1)))
+    with pytest.raises(RuntimeError):
+        o.set_gravity(np.zeros((1, 2, 16)))
+    o = O.Oracle(O.make_config((32, 24, 1), (0.1, 0.1, 0.1), bc="fade", bc_infinity=8))
+    o.set_state(*uniform((32, 24, 1)))
+    with pytest.raises(RuntimeError):
+        o.set_gravity(np.zeros((1, 24, 32)))
