@@ -150,6 +150,18 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
         int done = 0;
         check(imh_run(h, (int)mxGetScalar(prhs[2]), &done));
         plhs[0] = mxCreateDoubleScalar(done);
+    } else if (cmd == "source") {
+        check(imh_source(h));
+    } else if (cmd == "setGravity") {
+        // setGravity(h, fixedPotential, MASS_THRESHOLD); an empty potential turns it off
+        if (nrhs < 4) mexErrMsgIdAndTxt("Imogen:hip", "setGravity(h, potential, threshold)");
+        size_t n = mxGetNumberOfElements(prhs[2]);
+        check(imh_set_gravity(h, n ? mxGetPr(prhs[2]) : nullptr, n, mxGetScalar(prhs[3])));
+    } else if (cmd == "setRadiation") {
+        // setRadiation(h, type, exponent, initialMaximum); type 0 'empty', 1 'optically_thin'
+        if (nrhs < 5) mexErrMsgIdAndTxt("Imogen:hip", "setRadiation(h, type, exponent, max)");
+        check(imh_set_radiation(h, (int)mxGetScalar(prhs[2]), mxGetScalar(prhs[3]),
+                                mxGetScalar(prhs[4])));
     } else if (cmd == "download") {
         if (nrhs < 4) mexErrMsgIdAndTxt("Imogen:hip", "download(h, fieldId, numel)");
         size_t n = (size_t)mxGetScalar(prhs[3]);
