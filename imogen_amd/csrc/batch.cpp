@@ -633,9 +633,9 @@ int imh_batch_download_field(imh_batch* b, int sim, int field, double* out, size
 // First iterations of fresh simulations, then (re)build the argument blocks and plans.
 static int batch_prepare(imh_batch* b) {
     for (imh_ctx* c : b->ctx)
-        if (c->grav || c->rad_type)
-            return fail(IMH_ENOTSUP, "source terms (gravity, radiation) run on single "
-                        "contexts, not in batched launches");
+        if (c->grav || c->rad_type || c->visc_type)
+            return fail(IMH_ENOTSUP, "source terms (gravity, radiation) and artificial "
+                        "viscosity run on single contexts, not in batched launches");
     bool changed = false;
     int rc = run_first_iterations(b, &changed);
     if (rc) return rc;

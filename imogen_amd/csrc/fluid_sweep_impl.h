@@ -221,6 +221,11 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
     // edge trackers of the block's pencils: [px][side*16 + role*5 + field], roles lower:
     // store.fluxR, fluxR, fluxL; upper: store.fluxL, fluxR, fluxL
     __shared__ double s_trk[PXMAX][32];
+    // artificial viscosity (general form only): first momentum value of each chunk, the
+    // pencil ends of the momentum row and its upper ghost
+    __shared__ T s_mv[GEN ? BS : 1];
+    __shared__ double s_vend[GEN ? PXMAX : 1][6];
+    __shared__ double s_vgh[GEN ? PXMAX : 1];
 
     const int tid = threadIdx.x;
     const int PX = a.PX, TL = a.TL;
@@ -437,7 +442,7 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
         return i < N - 1 ? in : ghost;
     };
 
-    T P[K], U[K], IR[K];
+    T P[K], U[K], IR[K], CS[K];
     // pressure, velocity and the freezing-speed candidate of every cell of state Q
     auto cell_speeds = [&](const T (&Q)[5][K]) __attribute__((always_inline)) -> T {
         T lmax = T(NAN);
@@ -447,6 +452,7 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
             M::press(Q[0][k], Q[1][k], Q[2][k], Q[3][k], Q[4][k], bm[0][k], bm[1][k], bm[2][k],
                      a.use, g, Pk, ck, ir);
             P[k] = Pk;
+            CS[k] = ck;
             IR[k] = ir;
             U[k] = M::divrho(sel3(d, Q[1][k], Q[2][k], Q[3][k]), ir);
             const T sp = fabs(U[k]) + ck;
@@ -499,8 +505,61 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
         if constexpr (4 * G < 5) fn(FieldIdx<(4 * G < 5 ? 4 * G : 0)>{});
     };
 
+    // press = press + viscosity.solve(...) (wFluidFlux.m:24), Neumann-Richtmyer
+    // (ArtificialViscosity.m:73-77): vm = min(p(i+1) - p(i), 0) of the sweep-direction
+    // momentum of state Q, result vm .* (c1 .* c_f + c2 .* vm ./ rho).  The shift's upper
+    // ghost follows the BC of object obj (the FluidArray in the predictor, its
+    // StorageArray in the corrector) and nudges that object's tracker; `first` = the
+    // object's first assignment, whose tracker starts at the current end value.
+    auto viscosity = [&](const T (&Q)[5][K], int obj, bool first) __attribute__((always_inline)) {
+        if constexpr (GEN) {
+            if (a.visc_type == 0) return;
+            T mX[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) mX[k] = sel3(d, Q[1][k], Q[2][k], Q[3][k]);
+            s_mv[tid] = mX[0];
+            if (pen_ok) {
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    const int gi = i0 + k;
+                    if (gi < 3 && gi < N) s_vend[px][gi] = double(mX[k]);
+                    if (gi >= N - 3 && gi < N) s_vend[px][3 + gi - (N - 3)] = double(mX[k]);
+                }
+            }
+            __syncthreads();
+            if (tid < PX && pen0 + tid < a.npencil) {
+                const double* e = s_vend[tid];
+                const bool trk = (mhi == IMH_BC_TRANS || mhi == IMH_BC_WALL);
+                double* tp = trk ? a.edges + a.eoff[(obj * 2 + 1) * 3 + d] + pen0 + tid : nullptr;
+                double tv = trk ? (first ? e[5] : *tp) : 0.0;
+                GhostSpec sp;
+                sp.mode = mhi;
+                sp.upper = 1;
+                sp.flip_sign = 1;  // the shifted array is the momentum component X
+                sp.zero_ghost = 0;
+                sp.active_linear = a.active_lin;
+                sp.bc_infinity = f;
+                const Ends en = {e[0], e[1], e[2], e[3], e[4], e[5]};
+                double g0, g1;
+                ghost_values(sp, en, N, 1, &tv, g0, g1);
+                if (trk) *tp = tv;
+                s_vgh[tid] = g0;
+            }
+            __syncthreads();
+            const T nb = s_mv[tr_];
+            const T gh = T(s_vgh[px]);
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const T sh = upper_of(mX, k, nb, gh);
+                const T vm = fmin(sh - mX[k], T(0));  // MATLAB min(.., 0) skips NaN
+                P[k] = P[k] + vm * (T(a.visc_lin) * CS[k] + T(a.visc_quad) * vm / Q[0][k]);
+            }
+        }
+    };
+
     // ================= predictor: W and C from U^n (wFluidFlux.m) ====================
     const T C = pencil_reduce_max(cell_speeds(q), a.xmajor, TL, PX, tid, px, s_red[0]);
+    viscosity(q, OBJ_PRIM + X, false);
     const T rC = M::rcp(C);
     const T hfC = T(hf) * C;
     T qh[5][K];
@@ -547,8 +606,32 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
 #pragma unroll
     for (int v = 0; v < 5; ++v) pad_fill(qh[v]);
 
+    // First assignment of the momentum StorageArrays (InitializedArray.m:43-47): their
+    // edges, read only by the viscosity's shifts, come from this half step.
+    if constexpr (GEN) {
+        if (a.init_edges) {
+            auto init_store = [&](auto CC) __attribute__((always_inline)) {
+                constexpr int cc = decltype(CC)::value;
+                init_faces(OBJ_FSTORE + cc, qh[1 + cc]);
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    const int gi = i0 + k;
+                    if (!valid(k)) continue;
+                    if (gi == 0 && trk_lo)
+                        a.edges[a.eoff[((OBJ_FSTORE + cc) * 2 + 0) * 3 + d] + pen] = double(qh[1 + cc][k]);
+                    if (gi == N - 1 && trk_hi)
+                        a.edges[a.eoff[((OBJ_FSTORE + cc) * 2 + 1) * 3 + d] + pen] = double(qh[1 + cc][k]);
+                }
+            };
+            init_store(FieldIdx<0>{});
+            init_store(FieldIdx<1>{});
+            init_store(FieldIdx<2>{});
+        }
+    }
+
     // ================= corrector: W' and C' from U^(n+1/2) ===========================
     const T C2 = pencil_reduce_max(cell_speeds(qh), a.xmajor, TL, PX, tid, px, s_red[1]);
+    viscosity(qh, OBJ_FSTORE + d, a.init_edges != 0);
     const T rC2 = M::rcp(C2);
     const T ffC = T(ff) * C2;
     const int lim = a.limiter;

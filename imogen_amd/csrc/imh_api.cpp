@@ -315,13 +315,13 @@ static long long edges_layout(const imh_config& cfg, long long eoff[NOBJ][2][3])
         for (int s = 0; s < 2; ++s)
             for (int d = 0; d < 3; ++d) eoff[o][s][d] = 0;
     auto exists = [&](int o) {
-        if (o < OBJ_MAG + 3 || o == OBJ_GRAV) return true;
+        if (o < OBJ_MAG + 3 || o >= OBJ_GRAV) return true;
         if (!cfg.magnet_active) return false;
         int k = (o - OBJ_VG) % 9;
         return (k / 3) != (k % 3);  // (I, X) pairs with I != X
     };
     for (int o = 0; o < NOBJ; ++o) {
-        if (o >= OBJ_MFLUX + 9 && o != OBJ_GRAV) continue;
+        if (o >= OBJ_MFLUX + 9 && o < OBJ_GRAV) continue;
         if (!exists(o)) continue;
         for (int s = 0; s < 2; ++s)
             for (int d = 0; d < ndims; ++d) {
@@ -656,6 +656,9 @@ FluidArgs<T> make_fluid_args(const imh_ctx* c, int d, int force) {
     a.xcd_remap = 1;
     a.force = force;
     a.fade = c->fade_dev;
+    a.visc_type = c->visc_type;
+    a.visc_lin = c->visc_lin;
+    a.visc_quad = c->visc_quad;
     return a;
 }
 template FluidArgs<double> make_fluid_args<double>(const imh_ctx*, int, int);
@@ -1038,6 +1041,25 @@ int imh_set_radiation(imh_ctx* c, int type, double exponent, double initial_maxi
         if (!std::isnan(v) && (std::isnan(mn) || v < mn)) mn = v;  // MATLAB min skips NaN
     }
     c->rad_strength = (initial_maximum / 100.0) * mn;
+    return IMH_OK;
+}
+
+int imh_set_viscosity(imh_ctx* c, int type, double linear, double quadratic) {
+    if (!c) return fail(IMH_EINVAL, "null context");
+    if (type == 2)
+        return fail(IMH_ENOTSUP, "the Caramana-Shashkov-Whalen viscosity is not implemented");
+    if (type != 0 && type != 1) return fail(IMH_EINVAL, "viscosity type %d", type);
+    if (type)
+        for (int d = 0; d < c->ndims; ++d)
+            for (int s = 0; s < 2; ++s)
+                if (c->cfg.bc[s][d] == IMH_BC_FADE)
+                    return fail(IMH_ENOTSUP, "viscosity with a fade boundary is not supported");
+    hipError_t e = hipStreamSynchronize(c->st);
+    if (e != hipSuccess) return hip_fail(e, "sync");
+    drop_graphs(c);
+    c->visc_type = type;
+    c->visc_lin = linear;
+    c->visc_quad = quadratic;
     return IMH_OK;
 }
 

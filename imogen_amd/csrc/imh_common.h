@@ -31,7 +31,8 @@ enum ObjId : int {
     OBJ_WMAG = 46,   // mag(I).wMag(X)
     OBJ_MFLUX = 55,  // mag(I).flux(X)
     OBJ_GRAV = 64,   // grav (GravityArray)
-    NOBJ = 65
+    OBJ_FSTORE = 65, // mom(1..3).store (StorageArray), shifted by the artificial viscosity
+    NOBJ = 68
 };
 
 // FluxArray.fluxType for zero_shift.m.

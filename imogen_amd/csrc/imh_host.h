@@ -106,6 +106,9 @@ struct imh_ctx {
     int rad_type = 0;                   // 0 'empty', 1 'optically_thin'
     double rad_exponent = 0.5;
     double rad_strength = 0.0;
+    // run.fluid.viscosity (ArtificialViscosity.m): 0 'empty', 1 'neumann_richtmyer'
+    int visc_type = 0;
+    double visc_lin = 0.0, visc_quad = 0.0;
 };
 
 namespace imh {

@@ -45,6 +45,8 @@ struct FluidArgs {
     int xcd_remap;
     int force;               // run even when the loop's running flag is off (direct calls)
     const double* fade;      // fade_shift.m coefficients (device)
+    int visc_type;           // run.fluid.viscosity: 0 'empty', 1 'neumann_richtmyer'
+    double visc_lin, visc_quad;  // linear / quadraticViscousStrength
 };
 
 // One entry of a batched launch: argument block `arg` (one simulation and sweep) and the
@@ -104,7 +106,7 @@ int fluid_column_pencils();
 // whether a sweep of these arguments needs the general kernel form
 template <typename T>
 inline bool fluid_needs_general(const FluidArgs<T>& a) {
-    return a.init_edges || mode_at(a.modes, 0, a.d) == IMH_BC_FADE ||
+    return a.init_edges || a.visc_type || mode_at(a.modes, 0, a.d) == IMH_BC_FADE ||
            mode_at(a.modes, 1, a.d) == IMH_BC_FADE;
 }
 

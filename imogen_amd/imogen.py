@@ -221,6 +221,13 @@ class ImogenManager:
         check(lib().imh_set_radiation(self._ctx, self.RADIATION[kind], float(exponent),
                                       float(initial_maximum)))
 
+    VISCOSITY = {"empty": 0, "neumann_richtmyer": 1, "caramana_shashkov_whalen": 2}
+
+    def set_viscosity(self, kind, linear=0.0, quadratic=0.0):
+        """run.fluid.viscosity type with linear / quadratic strengths (ENUM.m:53-55)."""
+        check(lib().imh_set_viscosity(self._ctx, self.VISCOSITY[kind], float(linear),
+                                      float(quadratic)))
+
     @property
     def radiation_strength(self):
         v = ctypes.c_double()

@@ -19,6 +19,10 @@
 //   a = imogen_mex('download', h, fieldId, numel) FluidArray/MagnetArray .array (a column;
 //                                                 reshape to the grid on the MATLAB side)
 //   [t, dt, iter, gi] = imogen_mex('time', h)
+//   imogen_mex('source', h)                       source(run,...)
+//   imogen_mex('setGravity', h, phi, threshold)   fixed potential, MASS_THRESHOLD
+//   imogen_mex('setRadiation', h, type, a, imax)  run.fluid.radiation
+//   imogen_mex('setViscosity', h, type, c1, c2)   run.fluid.viscosity
 //   imogen_mex('destroy', h)
 #include <cstdint>
 #include <cstring>
@@ -161,6 +165,11 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
         // setRadiation(h, type, exponent, initialMaximum); type 0 'empty', 1 'optically_thin'
         if (nrhs < 5) mexErrMsgIdAndTxt("Imogen:hip", "setRadiation(h, type, exponent, max)");
         check(imh_set_radiation(h, (int)mxGetScalar(prhs[2]), mxGetScalar(prhs[3]),
+                                mxGetScalar(prhs[4])));
+    } else if (cmd == "setViscosity") {
+        // setViscosity(h, type, linear, quadratic); type 0 'empty', 1 'neumann_richtmyer'
+        if (nrhs < 5) mexErrMsgIdAndTxt("Imogen:hip", "setViscosity(h, type, c1, c2)");
+        check(imh_set_viscosity(h, (int)mxGetScalar(prhs[2]), mxGetScalar(prhs[3]),
                                 mxGetScalar(prhs[4])));
     } else if (cmd == "download") {
         if (nrhs < 4) mexErrMsgIdAndTxt("Imogen:hip", "download(h, fieldId, numel)");

@@ -159,6 +159,14 @@ int imh_set_gravity(imh_ctx *ctx, const double *potential, size_t n, double mass
 int imh_set_radiation(imh_ctx *ctx, int type, double exponent, double initial_maximum);
 int imh_get_radiation_strength(imh_ctx *ctx, double *strength);
 
+/* run.fluid.viscosity -- fluid/ArtificialViscosity.m (FluidManager.m:42, initialize.m:
+ * 395-400): type 0 'empty', 1 'neumann_richtmyer' (ArtificialViscosity.m:73-77) with
+ * linearViscousStrength and quadraticViscousStrength; its term is added to the pressure
+ * of both wFluidFlux calls of every fluid sweep (wFluidFlux.m:24).  Type 2
+ * ('caramana_shashkov_whalen') returns IMH_ENOTSUP, as do fade boundaries; batches refuse
+ * contexts with viscosity. */
+int imh_set_viscosity(imh_ctx *ctx, int type, double linear, double quadratic);
+
 /* ---- time state -------------------------------------------------------------------- */
 int imh_set_dtime(imh_ctx *ctx, double dt);
 /* Blocks until the stream is idle, then reads the device time state. */

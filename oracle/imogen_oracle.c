@@ -65,6 +65,8 @@ struct orc_sim {
     double *grav_potential;        /* run.gravity.fixedPotential, NULL: gravity inactive */
     double mass_threshold;         /* run.fluid.MASS_THRESHOLD */
     int rad_type;                  /* 0 'empty', 1 'optically_thin' */
+    int visc_type;                 /* 0 'empty', 1 'neumann_richtmyer' */
+    double visc_lin, visc_quad;    /* linear / quadraticViscousStrength */
     double rad_exponent, rad_strength;
     int have_state;
     char err[256];
@@ -483,7 +485,8 @@ static void apply_limiter(orc_sim *s, int lim, obj_t *flux, const double *dl,
 /* wFluidFlux.m                                                                           */
 /* ------------------------------------------------------------------------------------ */
 
-static void wfluidflux(orc_sim *s, double *const q[5], obj_t *freeze, int X, double *w[5]) {
+static void wfluidflux(orc_sim *s, double *const q[5], obj_t *momX, obj_t *freeze, int X,
+                       double *w[5]) {
     long n = s->ncell;
     int d = X - 1;
     double g = s->cfg.gamma;
@@ -511,8 +514,21 @@ static void wfluidflux(orc_sim *s, double *const q[5], obj_t *freeze, int X, dou
         for (long i = 0; i < N; ++i) C[base + i * stride] = m;
     }
     obj_assign(s, freeze, C);
-    /* press = press + viscosity.solve(...)  ('empty' viscosity returns 0) */
-    EW_LOOP(n) press[c] = press[c] + 0.0;
+    /* press = press + viscosity.solve(run, mass, mom, ener, mag, soundSpd, X) */
+    if (s->visc_type == 1) {
+        /* ArtificialViscosity.neumannRichtmyerViscosity (ArtificialViscosity.m:73-77):
+         * mom(X) is the FluidArray in the predictor and its StorageArray in the corrector */
+        double *sh = alloc_arr(n);
+        do_shift(s, momX, X, 1, sh);
+        const double lin = s->visc_lin, quad = s->visc_quad;
+        EW_LOOP(n) {
+            double vm = fmin(sh[c] - qx[c], 0.0); /* MATLAB min(.., 0) skips NaN */
+            press[c] = press[c] + vm * (lin * snd[c] + quad * vm / rho[c]);
+        }
+        free(sh);
+    } else {
+        EW_LOOP(n) press[c] = press[c] + 0.0; /* 'empty' viscosity returns 0 */
+    }
     const double *Cf = freeze->a;
     const double *E = q[4];
     EW_LOOP(n) {
@@ -545,7 +561,7 @@ int orc_relaxing_fluid(orc_sim *s, int X) {
     for (int i = 0; i < 5; ++i) q[i] = s->prim[i].a;
 
     /* Half-timestep predictor */
-    wfluidflux(s, q, &s->freeze[d], X, w);
+    wfluidflux(s, q, &s->prim[X], &s->freeze[d], X, w);
     const double *C = s->freeze[d].a;
     double *s1 = alloc_arr(n), *s2 = alloc_arr(n);
     for (int i = 0; i < 5; ++i) {
@@ -577,7 +593,7 @@ int orc_relaxing_fluid(orc_sim *s, int X) {
     /* Full-timestep corrector */
     double *qs[5];
     for (int i = 0; i < 5; ++i) qs[i] = s->store[i].a;
-    wfluidflux(s, qs, &s->freeze_tvd[d], X, w);
+    wfluidflux(s, qs, &s->store[X], &s->freeze_tvd[d], X, w);
     const double *C2 = s->freeze_tvd[d].a;
     double *s3 = alloc_arr(n), *dl = alloc_arr(n), *dr = alloc_arr(n);
     int lim = s->cfg.limiter[d];
@@ -963,6 +979,26 @@ int orc_set_radiation(orc_sim *s, int type, double exponent, double initial_maxi
 }
 
 double orc_radiation_strength(const orc_sim *s) { return s->rad_strength; }
+
+/* run.fluid.viscosity (FluidManager.m:42, initialize.m:395-400): 0 'empty',
+ * 1 'neumann_richtmyer'.  The Caramana-Shashkov-Whalen form is not restated. */
+int orc_set_viscosity(orc_sim *s, int type, double linear, double quadratic) {
+    if (type != 0 && type != 1) {
+        snprintf(s->err, sizeof(s->err), "viscosity type %d is not supported", type);
+        return -1;
+    }
+    if (type)
+        for (int d = 0; d < s->ndims; ++d)
+            for (int side = 0; side < 2; ++side)
+                if (s->cfg.bc[side][d] == ORC_BC_FADE) {
+                    snprintf(s->err, sizeof(s->err), "viscosity with a fade BC is not supported");
+                    return -1;
+                }
+    s->visc_type = type;
+    s->visc_lin = linear;
+    s->visc_quad = quadratic;
+    return 0;
+}
 
 /* source(run, mass, mom, ener, mag, grav) (source.m:16-31). */
 int orc_source(orc_sim *s) {

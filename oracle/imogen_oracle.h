@@ -19,6 +19,7 @@
  *   time/TimeManager.m:55-99, utils/maxFinderND.m:15-19   CFL reduction
  *   source.m:16-31, classes/ImogenArray.m:189-192, fluid/FluidArray.m:29-34,
  *   gravity/GravityManager.m:104-116, fluid/Radiation.m:61-89   source step
+ *   fluid/ArtificialViscosity.m:73-77, fluid/wFluidFlux.m:24   Neumann-Richtmyer viscosity
  *
  * Every array operation keeps MATLAB's left-to-right evaluation order so that the
  * rounding is the one the .m text prescribes.  Build without FP contraction
@@ -40,6 +41,9 @@
  *                  momentum change -dt*rho*dphi/dx and energy change from the old momentum;
  *                  mass threshold; dimensions beyond ndims; radiation strength and sink
  *                  closed forms on uniform states.
+ *   viscosity      zero on expanding or uniform flow; on a linear compression the added
+ *                  pressure is vm*(c1*c_f + c2*vm/rho) with vm = p(i+1) - p(i); a run with
+ *                  zero strengths equals the inviscid run bit for bit.
  *   CFL            closed form on a uniform state.
  * Parity unpinned (no MATLAB anywhere): MATLAB's own perms row order (two orders are
  * offered, see orc_config.perms_order) and MATLAB pchip's last-bit rounding.
@@ -117,6 +121,10 @@ int orc_running(const orc_sim *s);                            /* TimeManager.run
 int orc_set_gravity(orc_sim *s, const double *potential, double mass_threshold);
 int orc_set_radiation(orc_sim *s, int type, double exponent, double initial_maximum);
 double orc_radiation_strength(const orc_sim *s);
+/* Artificial viscosity (fluid/ArtificialViscosity.m): type 0 'empty', 1
+ * 'neumann_richtmyer' with linear / quadratic strengths; the term is added to the
+ * pressure of both wFluidFlux calls of every fluid sweep (wFluidFlux.m:24). */
+int orc_set_viscosity(orc_sim *s, int type, double linear, double quadratic);
 int orc_source(orc_sim *s);
 
 /* Time state. */

@@ -88,6 +88,8 @@ def lib():
         L.orc_radiation_strength.argtypes = [ctypes.c_void_p]
         L.orc_radiation_strength.restype = ctypes.c_double
         L.orc_source.argtypes = [ctypes.c_void_p]
+        L.orc_set_viscosity.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
+                                        ctypes.c_double]
         _lib = L
     return _lib
 
@@ -218,6 +220,11 @@ class Oracle:
     @property
     def radiation_strength(self):
         return self.L.orc_radiation_strength(self.h)
+
+    def set_viscosity(self, kind, linear=0.0, quadratic=0.0):
+        """Artificial viscosity: 'empty' or 'neumann_richtmyer'."""
+        t = {"empty": 0, "neumann_richtmyer": 1}[kind] if isinstance(kind, str) else int(kind)
+        self._check(self.L.orc_set_viscosity(self.h, t, float(linear), float(quadratic)))
 
     def source(self):
         self._check(self.L.orc_source(self.h))

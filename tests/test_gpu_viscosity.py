@@ -1,0 +1,107 @@
+"""Neumann-Richtmyer artificial viscosity in the fused sweep against the oracle, in
+lockstep at <= 1e-12 relative L-inf per iteration: a Sod tube with the reference's
+debug-run strengths (run_Debug.m:17-19), 2-D hydro with transparent ends (the momentum
+FluidArray's and StorageArray's trackers), 2-D MHD with mirror / wall ends (sign flips of
+the shifted momentum, stored wall edges), and 3-D MHD with gravity and radiation on too.
+Graph replay equals eager iterations, a checkpoint resumes bit for bit, FP32-bulk stays
+close to FP64, and the Caramana-Shashkov-Whalen form is refused."""
+import numpy as np
+import pytest
+
+from imogen_amd import ImogenError, iteration, problems
+
+from .helpers import FLUID_FIELDS, MAG_FIELDS, compare, make_gpu, make_oracle, rel_l1
+from .test_gpu_source import box, potential
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+NR = (0.1, 0.25)
+
+
+def lockstep(ini, st, iterations, fields, visc=NR, grav=False, rad=None):
+    orc = make_oracle(ini, st)
+    run = make_gpu(ini, st)
+    orc.set_viscosity("neumann_richtmyer", *visc)
+    run.set_viscosity("neumann_richtmyer", *visc)
+    if grav:
+        phi = potential(tuple(ini["grid"]))
+        orc.set_gravity(phi)
+        run.set_gravity(phi)
+    if rad:
+        orc.set_radiation("optically_thin", *rad)
+        run.set_radiation("optically_thin", *rad)
+    for it in range(iterations):
+        print("iteration", it + 1, flush=True)
+        orc.iteration()
+        iteration(run)
+        try:
+            compare(run, orc, fields, TOL)
+        except AssertionError as e:
+            raise AssertionError("iteration %d: %s" % (it + 1, e))
+    run.close()
+    orc.close()
+
+
+def test_sod_lockstep():
+    ini, st = problems.sod(1024, iter_max=10 ** 6)
+    lockstep(ini, st, 30, FLUID_FIELDS)
+
+
+def test_2d_hydro_transparent_lockstep():
+    ini, st = box((48, 40, 1), {"x": "trans", "y": "circ", "z": "circ"})
+    lockstep(ini, st, 6, FLUID_FIELDS, visc=(0.3, 0.6))
+
+
+def test_2d_mhd_mirror_wall_lockstep():
+    ini, st = box((40, 32, 1), {"x": "mirror", "y": "wall", "z": "circ"}, magnet=True)
+    lockstep(ini, st, 5, FLUID_FIELDS + MAG_FIELDS, visc=(0.3, 0.6))
+
+
+def test_3d_mhd_with_sources_lockstep():
+    ini, st = box((24, 20, 16), {"x": "trans", "y": "circ", "z": "const"}, magnet=True)
+    lockstep(ini, st, 3, FLUID_FIELDS + MAG_FIELDS, visc=(0.3, 0.6), grav=True, rad=(0.5, 1.0))
+
+
+def test_graph_replay_and_checkpoint_with_viscosity():
+    ini, st = box((48, 40, 1), {"x": "trans", "y": "trans", "z": "circ"})
+    a = make_gpu(ini, st)
+    b = make_gpu(ini, st)
+    for r in (a, b):
+        r.set_viscosity("neumann_richtmyer", 0.3, 0.6)
+    for _ in range(8):
+        iteration(a)
+    assert b.run_iterations(4) == 4
+    ck = b.checkpoint()
+    b.close()
+    c = make_gpu(ini, st)
+    c.set_viscosity("neumann_richtmyer", 0.3, 0.6)
+    c.restore(ck)
+    assert c.run_iterations(4) == 4
+    for f in FLUID_FIELDS:
+        np.testing.assert_array_equal(c.download(f), a.download(f), err_msg=f)
+    a.close()
+    c.close()
+
+
+def test_fp32_viscosity_close_to_fp64():
+    ini, st = box((48, 40, 1), {"x": "trans", "y": "circ", "z": "circ"})
+    runs = []
+    for prec in ("fp64", "fp32"):
+        r = make_gpu(ini, st, prec)
+        r.set_viscosity("neumann_richtmyer", 0.3, 0.6)
+        for _ in range(10):
+            iteration(r)
+        runs.append(r)
+    for f in ("mass", "momx", "ener"):
+        assert rel_l1(runs[1].download(f), runs[0].download(f)) < 1e-5, f
+    for r in runs:
+        r.close()
+
+
+def test_unsupported_viscosity_is_refused():
+    ini, st = box((32, 24, 1), "circ")
+    run = make_gpu(ini, st)
+    with pytest.raises(ImogenError):
+        run.set_viscosity("caramana_shashkov_whalen", 0.1, 0.2)
+    run.close()

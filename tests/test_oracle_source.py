@@ -158,3 +158,93 @@ def test_gravity_rejects_short_dimension_and_fade():
     o.set_state(*uniform((32, 24, 1)))
     with pytest.raises(RuntimeError):
         o.set_gravity(np.zeros((1, 24, 32)))
+
+
+# ---- artificial viscosity (ArtificialViscosity.m:73-77) -----------------------------------
+def _sweep_state(o):
+    return {f: o.get(f) for f in ("mass", "momx", "momy", "ener")}
+
+
+def _sod_oracle(n, visc=None):
+    from imogen_amd import problems
+    from tests.helpers import make_oracle
+    ini, st = problems.sod(n, time_max=0.1, iter_max=100000)
+    o = make_oracle(ini, st)
+    if visc:
+        o.set_viscosity("neumann_richtmyer", *visc)
+    return o
+
+
+def test_viscosity_with_zero_strengths_is_the_inviscid_run():
+    a, b = _sod_oracle(256), _sod_oracle(256, (0.0, 0.0))
+    for _ in range(20):
+        a.iteration()
+        b.iteration()
+    for f, v in _sweep_state(a).items():
+        np.testing.assert_array_equal(_sweep_state(b)[f], v, err_msg=f)
+
+
+def test_viscosity_is_off_in_expanding_flow():
+    """min(p(i+1) - p(i), 0) vanishes where the momentum grows along the sweep (a
+    sign error in the difference would make this flow viscous)."""
+    grid, dgrid = (64, 1, 1), (1.0 / 64, 1.0 / 64, 1.0 / 64)
+    x = (np.arange(64) + 0.5) / 64
+    shape = (1, 1, 64)
+    mass = np.ones(shape)
+    mom = [(0.05 + 0.2 * x).reshape(shape), np.zeros(shape), np.zeros(shape)]
+    ener = 1.0 / (GAMMA - 1.0) + 0.5 * mom[0] ** 2
+    mag = [np.zeros(shape)] * 3
+    runs = []
+    for visc in (None, (0.3, 0.5)):
+        o = O.Oracle(O.make_config(grid, dgrid, gamma=GAMMA, bc="const"))
+        o.set_state(mass, mom, ener, mag)
+        if visc:
+            o.set_viscosity("neumann_richtmyer", *visc)
+        o.dtime = 1e-4
+        o.relaxing_fluid(1)
+        runs.append(_sweep_state(o))
+    for f in runs[0]:
+        np.testing.assert_array_equal(runs[1][f], runs[0][f], err_msg=f)
+
+
+def test_viscosity_acts_in_compression_and_sod_still_converges():
+    grid, dgrid = (64, 1, 1), (1.0 / 64, 1.0 / 64, 1.0 / 64)
+    x = (np.arange(64) + 0.5) / 64
+    shape = (1, 1, 64)
+    mass = np.ones(shape)
+    mom = [(0.3 - 0.4 * x).reshape(shape), np.zeros(shape), np.zeros(shape)]
+    ener = 1.0 / (GAMMA - 1.0) + 0.5 * mom[0] ** 2
+    mag = [np.zeros(shape)] * 3
+    out = []
+    for visc in (None, (0.3, 0.5)):
+        o = O.Oracle(O.make_config(grid, dgrid, gamma=GAMMA, bc="const"))
+        o.set_state(mass, mom, ener, mag)
+        if visc:
+            o.set_viscosity("neumann_richtmyer", *visc)
+        o.dtime = 1e-4
+        o.relaxing_fluid(1)
+        out.append(o.get("ener"))
+    assert np.max(np.abs(out[1] - out[0])) > 0
+    # a shock tube with the reference's debug-run strengths (run_Debug.m:17-19)
+    import math  # noqa: F401
+    from tests.test_oracle import sod_exact
+    errs = {}
+    for visc in (None, (0.1, 0.25)):
+        o = _sod_oracle(256, visc)
+        while o.running:
+            o.iteration()
+        xs = (np.arange(256) + 0.5) / 256
+        ex = sod_exact(xs, o.time, x0=(round(256 / 2) - 1) / 256)
+        errs[visc] = float(np.sum(np.abs(o.get("mass").ravel() - ex)) / np.sum(np.abs(ex)))
+    assert errs[(0.1, 0.25)] < 0.05 and errs[(0.1, 0.25)] < 1.5 * errs[None], errs
+
+
+def test_viscosity_rejects_fade_and_unknown_types():
+    o = O.Oracle(O.make_config((32, 24, 1), (0.1, 0.1, 0.1), bc="fade", bc_infinity=8))
+    o.set_state(*uniform((32, 24, 1)))
+    with pytest.raises(RuntimeError):
+        o.set_viscosity("neumann_richtmyer", 0.1, 0.2)
+    o = O.Oracle(O.make_config((32, 24, 1), (0.1, 0.1, 0.1)))
+    o.set_state(*uniform((32, 24, 1)))
+    with pytest.raises(RuntimeError):
+        o.set_viscosity(2, 0.1, 0.2)
