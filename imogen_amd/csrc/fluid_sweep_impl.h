@@ -226,6 +226,10 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
     __shared__ T s_mv[GEN ? BS : 1];
     __shared__ double s_vend[GEN ? PXMAX : 1][6];
     __shared__ double s_vgh[GEN ? PXMAX : 1];
+    // Caramana-Shashkov-Whalen rows at the chunk edges (S first/last, m first/last, V at
+    // k = 0, 1, K-2, K-1) and at the pencil ends (cells 0..2, N-3..N-1)
+    __shared__ T s_cw[GEN ? 8 : 1][GEN ? BS : 1];
+    __shared__ double s_cend[GEN ? PXMAX : 1][3][6];
 
     const int tid = threadIdx.x;
     const int PX = a.PX, TL = a.TL;
@@ -511,9 +515,104 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
     // ghost follows the BC of object obj (the FluidArray in the predictor, its
     // StorageArray in the corrector) and nudges that object's tracker; `first` = the
     // object's first assignment, whose tracker starts at the current end value.
+    // Caramana-Shashkov-Whalen (ArtificialViscosity.m:80-148 with cswSoundSpeed, :155-190)
+    // for boundaries that are index maps (circ, const, zero, mirror, flip; the others are
+    // refused at set-up): every shifted array is then a cell row at a wrapped, clamped or
+    // reflected index, mom(X) changing sign at mirror and flip ends.  Three per-cell rows
+    // carry it: S (cswSoundSpeed of one cell, whose velSquared divides only the third
+    // momentum by mass^2, as the .m text does), m (mass) and V (mom(X) ./ mass).
+    auto csw = [&](const T (&Q)[5][K]) __attribute__((always_inline)) {
+        const T G = T(g);
+        T Sr[K], Mr[K], Vr[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const T rho = Q[0][k], m1 = Q[1][k], m2 = Q[2][k], m3 = Q[3][k];
+            const T vel2 = m1 * m1 + m2 * m2 + m3 * m3 / (rho * rho);
+            const T mag2 = bm[0][k] * bm[0][k] + bm[1][k] * bm[1][k] + bm[2][k] * bm[2][k];
+            T sp = (G - T(1)) * (Q[4][k] - T(0.5) * rho * vel2);
+            sp = sp - (G - T(1)) * T(0.5) * mag2;
+            Sr[k] = sqrt(fabs((G * sp + T(2) * mag2) / rho));
+            Mr[k] = rho;
+            Vr[k] = sel3(d, m1, m2, m3) / rho;
+        }
+        s_cw[0][tid] = Sr[0];
+        s_cw[1][tid] = Sr[K - 1];
+        s_cw[2][tid] = Mr[0];
+        s_cw[3][tid] = Mr[K - 1];
+        s_cw[4][tid] = Vr[0];
+        s_cw[5][tid] = Vr[K > 1 ? 1 : 0];
+        s_cw[6][tid] = Vr[K > 1 ? K - 2 : 0];
+        s_cw[7][tid] = Vr[K - 1];
+        if (pen_ok) {
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const int gi = i0 + k;
+                int slot = -1;
+                if (gi < 3 && gi < N) slot = gi;
+                else if (gi >= N - 3 && gi < N) slot = 3 + gi - (N - 3);
+                if (slot >= 0) {
+                    s_cend[px][0][slot] = double(Sr[k]);
+                    s_cend[px][1][slot] = double(Mr[k]);
+                    s_cend[px][2][slot] = double(Vr[k]);
+                }
+            }
+        }
+        __syncthreads();
+        // row value (0 S, 1 m, 2 V) at pencil index i0 + k + o, |o| <= 2
+        auto at = [&](int row, const T (&R)[K], int k, int o) __attribute__((always_inline)) -> T {
+            const int j = i0 + k + o;
+            if (j < 0 || j >= N) {
+                const int mode = j < 0 ? mlo : mhi;
+                int jj;
+                bool neg = false;
+                if (mode == IMH_BC_CIRC) {
+                    jj = j < 0 ? j + N : j - N;
+                } else if (mode == IMH_BC_MIRROR) {
+                    jj = j < 0 ? -j : 2 * (N - 1) - j;
+                    neg = true;
+                } else {  // const, zero (not a flux array), flip
+                    jj = j < 0 ? 0 : N - 1;
+                    neg = (mode == IMH_BC_FLIP);
+                }
+                const int slot = jj <= 2 ? jj : 3 + jj - (N - 3);
+                const T v = T(s_cend[px][row][slot]);
+                return (neg && row == 2) ? -v : v;
+            }
+            const int kk = k + o;
+            if (kk >= 0 && kk < K) return R[kk >= 0 && kk < K ? kk : 0];
+            const int q = kk < 0 ? -((K - 1 - kk) / K) : kk / K;  // floor(kk / K)
+            const int r = kk - q * K;
+            const int t = tid + q * dstep;  // the thread holding cell j of this pencil
+            if (row == 0) return s_cw[r == 0 ? 0 : 1][t];
+            if (row == 1) return s_cw[r == 0 ? 2 : 3][t];
+            return s_cw[r == 0 ? 4 : (r == 1 ? 5 : (r == K - 1 ? 7 : 6))][t];
+        };
+        const T c1 = T(a.visc_lin), c2 = T(a.visc_quad);
+        const T q4 = (G + T(1)) / T(4);
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const T mB = at(1, Mr, k, -1), mC = at(1, Mr, k, 1);
+            const T rho = T(2) * mC * mB / (mC + mB);
+            const T snd = fmax(fmin(at(0, Sr, k, -1), at(0, Sr, k, 1)), T(0));
+            const T v0 = at(2, Vr, k, -2), v1 = at(2, Vr, k, -1);
+            const T v2 = at(2, Vr, k, 1), v3 = at(2, Vr, k, 2);
+            const T dv = v2 - v1;
+            const T rL = (v3 - v2) / (T(2) * dv);
+            const T rR = (v1 - v0) / (T(2) * dv);
+            const T lim = fmax(T(0), fmin(fmin(T(0.5) * (rL + rR), fmin(T(2) * rL, T(2) * rR)), T(1)));
+            const T A = c2 * (G + T(1)) * fabs(dv) / T(4);
+            const T B = sqrt(c2 * c2 * (q4 * q4) * (dv * dv) + c1 * c1 * (snd * snd));
+            P[k] = P[k] + (dv < T(0) ? T(1) : T(0)) * rho * (A + B) * (T(1) - lim) * fabs(dv);
+        }
+    };
+
     auto viscosity = [&](const T (&Q)[5][K], int obj, bool first) __attribute__((always_inline)) {
         if constexpr (GEN) {
             if (a.visc_type == 0) return;
+            if (a.visc_type == 2) {
+                csw(Q);
+                return;
+            }
             T mX[K];
 #pragma unroll
             for (int k = 0; k < K; ++k) mX[k] = sel3(d, Q[1][k], Q[2][k], Q[3][k]);

@@ -1046,9 +1046,17 @@ int imh_set_radiation(imh_ctx* c, int type, double exponent, double initial_maxi
 
 int imh_set_viscosity(imh_ctx* c, int type, double linear, double quadratic) {
     if (!c) return fail(IMH_EINVAL, "null context");
+    if (type < 0 || type > 2) return fail(IMH_EINVAL, "viscosity type %d", type);
     if (type == 2)
-        return fail(IMH_ENOTSUP, "the Caramana-Shashkov-Whalen viscosity is not implemented");
-    if (type != 0 && type != 1) return fail(IMH_EINVAL, "viscosity type %d", type);
+        for (int d = 0; d < c->ndims; ++d) {
+            if (c->n[d] < 3) continue;  // never swept (flux.m:39)
+            for (int s = 0; s < 2; ++s) {
+                const int m = c->cfg.bc[s][d];
+                if (m == IMH_BC_TRANS || m == IMH_BC_WALL || m == IMH_BC_FADE)
+                    return fail(IMH_ENOTSUP, "the Caramana-Shashkov-Whalen viscosity runs with "
+                                "circ, const, zero, mirror and flip boundaries");
+            }
+        }
     if (type)
         for (int d = 0; d < c->ndims; ++d)
             for (int s = 0; s < 2; ++s)

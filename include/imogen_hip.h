@@ -160,11 +160,12 @@ int imh_set_radiation(imh_ctx *ctx, int type, double exponent, double initial_ma
 int imh_get_radiation_strength(imh_ctx *ctx, double *strength);
 
 /* run.fluid.viscosity -- fluid/ArtificialViscosity.m (FluidManager.m:42, initialize.m:
- * 395-400): type 0 'empty', 1 'neumann_richtmyer' (ArtificialViscosity.m:73-77) with
+ * 395-400): type 0 'empty', 1 'neumann_richtmyer' (ArtificialViscosity.m:73-77), 2
+ * 'caramana_shashkov_whalen' (ArtificialViscosity.m:80-148, 155-190) with
  * linearViscousStrength and quadraticViscousStrength; its term is added to the pressure
- * of both wFluidFlux calls of every fluid sweep (wFluidFlux.m:24).  Type 2
- * ('caramana_shashkov_whalen') returns IMH_ENOTSUP, as do fade boundaries; batches refuse
- * contexts with viscosity. */
+ * of both wFluidFlux calls of every fluid sweep (wFluidFlux.m:24).  Fade boundaries
+ * return IMH_ENOTSUP, and type 2 also transparent and wall boundaries on a swept
+ * dimension; batches refuse contexts with viscosity. */
 int imh_set_viscosity(imh_ctx *ctx, int type, double linear, double quadratic);
 
 /* ---- time state -------------------------------------------------------------------- */

@@ -485,8 +485,82 @@ static void apply_limiter(orc_sim *s, int lim, obj_t *flux, const double *dl,
 /* wFluidFlux.m                                                                           */
 /* ------------------------------------------------------------------------------------ */
 
-static void wfluidflux(orc_sim *s, double *const q[5], obj_t *momX, obj_t *freeze, int X,
-                       double *w[5]) {
+/* ArtificialViscosity.cswSoundSpeed (ArtificialViscosity.m:155-190): the smaller of the
+ * sound speeds of the two neighbours, every operand a shift of its own object, in the
+ * order the .m text evaluates them. */
+static void csw_sound_speed(orc_sim *s, obj_t *const o[5], int X, double *result) {
+    long n = s->ncell;
+    double G = s->cfg.gamma;
+    double *t1 = alloc_arr(n), *t2 = alloc_arr(n), *t3 = alloc_arr(n), *tm = alloc_arr(n);
+    double *b1 = alloc_arr(n), *b2 = alloc_arr(n), *b3 = alloc_arr(n), *te = alloc_arr(n);
+    double *vel2 = alloc_arr(n), *mag2 = alloc_arr(n), *sp = alloc_arr(n);
+    for (int side = 1; side >= 0; --side) { /* C (shift +1) first, then B (shift -1) */
+        int sh = side ? 1 : -1;
+        do_shift(s, o[1], X, sh, t1);
+        do_shift(s, o[2], X, sh, t2);
+        do_shift(s, o[3], X, sh, t3);
+        do_shift(s, o[0], X, sh, tm);
+        EW_LOOP(n) vel2[c] = t1[c] * t1[c] + t2[c] * t2[c] + t3[c] * t3[c] / (tm[c] * tm[c]);
+        do_shift(s, &s->cmag[0], X, sh, b1);
+        do_shift(s, &s->cmag[1], X, sh, b2);
+        do_shift(s, &s->cmag[2], X, sh, b3);
+        EW_LOOP(n) mag2[c] = b1[c] * b1[c] + b2[c] * b2[c] + b3[c] * b3[c];
+        do_shift(s, o[4], X, sh, te);
+        do_shift(s, o[0], X, sh, tm);
+        EW_LOOP(n) sp[c] = (G - 1.0) * (te[c] - 0.5 * tm[c] * vel2[c]);
+        EW_LOOP(n) sp[c] = sp[c] - (G - 1.0) * 0.5 * mag2[c];
+        do_shift(s, o[0], X, sh, tm);
+        if (side) {
+            EW_LOOP(n) result[c] = sqrt(fabs((G * sp[c] + 2.0 * mag2[c]) / tm[c]));
+        } else {
+            EW_LOOP(n) {
+                double sb = sqrt(fabs((G * sp[c] + 2.0 * mag2[c]) / tm[c]));
+                double r = fmin(sb, result[c]); /* min(sndSpdB, sndSpdC), NaN skipped */
+                result[c] = fmax(r, 0.0);
+            }
+        }
+    }
+    free(t1); free(t2); free(t3); free(tm); free(b1); free(b2); free(b3); free(te);
+    free(vel2); free(mag2); free(sp);
+}
+
+/* ArtificialViscosity.caramanaShashkovWhalenViscosity (ArtificialViscosity.m:80-148), the
+ * averaging form the .m file enables; the result is added to press. */
+static void csw_viscosity(orc_sim *s, obj_t *const o[5], int X, double *press) {
+    long n = s->ncell;
+    double G = s->cfg.gamma, c1 = s->visc_lin, c2 = s->visc_quad;
+    double *mB = alloc_arr(n), *mC = alloc_arr(n), *rho = alloc_arr(n), *snd = alloc_arr(n);
+    double *ta = alloc_arr(n), *tb = alloc_arr(n), *v[4];
+    for (int k = 0; k < 4; ++k) v[k] = alloc_arr(n);
+    do_shift(s, o[0], X, -1, mB);
+    do_shift(s, o[0], X, 1, mC);
+    EW_LOOP(n) rho[c] = 2.0 * mC[c] * mB[c] / (mC[c] + mB[c]);
+    csw_sound_speed(s, o, X, snd);
+    static const int sh[4] = {-2, -1, 1, 2};
+    for (int k = 0; k < 4; ++k) { /* v0..v3 = mom(X).shift ./ mass.shift */
+        do_shift(s, o[X], X, sh[k], ta);
+        do_shift(s, o[0], X, sh[k], tb);
+        double *vk = v[k];
+        EW_LOOP(n) vk[c] = ta[c] / tb[c];
+    }
+    const double q = (G + 1.0) / 4.0;
+    EW_LOOP(n) {
+        double dv = v[2][c] - v[1][c];
+        double rL = (v[3][c] - v[2][c]) / (2.0 * dv);
+        double rR = (v[1][c] - v[0][c]) / (2.0 * dv);
+        double lim = fmax(0.0, fmin(fmin(0.5 * (rL + rR), fmin(2.0 * rL, 2.0 * rR)), 1.0));
+        double A = c2 * (G + 1.0) * fabs(dv) / 4.0;
+        double B = sqrt(c2 * c2 * (q * q) * (dv * dv) + c1 * c1 * (snd[c] * snd[c]));
+        double r = (dv < 0 ? 1.0 : 0.0) * rho[c] * (A + B) * (1.0 - lim) * fabs(dv);
+        press[c] = press[c] + r;
+    }
+    free(mB); free(mC); free(rho); free(snd); free(ta); free(tb);
+    for (int k = 0; k < 4; ++k) free(v[k]);
+}
+
+static void wfluidflux(orc_sim *s, double *const q[5], obj_t *const obj[5], obj_t *freeze,
+                       int X, double *w[5]) {
+    obj_t *momX = obj[X];
     long n = s->ncell;
     int d = X - 1;
     double g = s->cfg.gamma;
@@ -526,6 +600,8 @@ static void wfluidflux(orc_sim *s, double *const q[5], obj_t *momX, obj_t *freez
             press[c] = press[c] + vm * (lin * snd[c] + quad * vm / rho[c]);
         }
         free(sh);
+    } else if (s->visc_type == 2) {
+        csw_viscosity(s, obj, X, press);
     } else {
         EW_LOOP(n) press[c] = press[c] + 0.0; /* 'empty' viscosity returns 0 */
     }
@@ -561,7 +637,8 @@ int orc_relaxing_fluid(orc_sim *s, int X) {
     for (int i = 0; i < 5; ++i) q[i] = s->prim[i].a;
 
     /* Half-timestep predictor */
-    wfluidflux(s, q, &s->prim[X], &s->freeze[d], X, w);
+    obj_t *pobj[5] = {&s->prim[0], &s->prim[1], &s->prim[2], &s->prim[3], &s->prim[4]};
+    wfluidflux(s, q, pobj, &s->freeze[d], X, w);
     const double *C = s->freeze[d].a;
     double *s1 = alloc_arr(n), *s2 = alloc_arr(n);
     for (int i = 0; i < 5; ++i) {
@@ -593,7 +670,8 @@ int orc_relaxing_fluid(orc_sim *s, int X) {
     /* Full-timestep corrector */
     double *qs[5];
     for (int i = 0; i < 5; ++i) qs[i] = s->store[i].a;
-    wfluidflux(s, qs, &s->store[X], &s->freeze_tvd[d], X, w);
+    obj_t *sobj[5] = {&s->store[0], &s->store[1], &s->store[2], &s->store[3], &s->store[4]};
+    wfluidflux(s, qs, sobj, &s->freeze_tvd[d], X, w);
     const double *C2 = s->freeze_tvd[d].a;
     double *s3 = alloc_arr(n), *dl = alloc_arr(n), *dr = alloc_arr(n);
     int lim = s->cfg.limiter[d];
@@ -980,10 +1058,39 @@ int orc_set_radiation(orc_sim *s, int type, double exponent, double initial_maxi
 
 double orc_radiation_strength(const orc_sim *s) { return s->rad_strength; }
 
+/* The viscosity term wFluidFlux adds to the pressure for the current state along X
+ * (the predictor's call: FluidArray objects), written to out; trackers are nudged as in
+ * the sweep. */
+int orc_viscosity_term(orc_sim *s, int X, double *out) {
+    if (!s->have_state || X < 1 || X > 3) return -1;
+    long n = s->ncell;
+    obj_t *pobj[5] = {&s->prim[0], &s->prim[1], &s->prim[2], &s->prim[3], &s->prim[4]};
+    EW_LOOP(n) out[c] = 0.0;
+    if (s->visc_type == 1) {
+        double *press = alloc_arr(n), *snd = alloc_arr(n), *sh = alloc_arr(n);
+        int use[3] = {!s->mag_is_zero[0], !s->mag_is_zero[1], !s->mag_is_zero[2]};
+        pressure_arr(0, n, s->cfg.gamma, s->prim[0].a, s->prim[1].a, s->prim[2].a,
+                     s->prim[3].a, s->prim[4].a, s->cmag[0].a, s->cmag[1].a, s->cmag[2].a, use,
+                     press, snd);
+        do_shift(s, pobj[X], X, 1, sh);
+        const double *qx = pobj[X]->a, *rho = s->prim[0].a;
+        EW_LOOP(n) {
+            double vm = fmin(sh[c] - qx[c], 0.0);
+            out[c] = vm * (s->visc_lin * snd[c] + s->visc_quad * vm / rho[c]);
+        }
+        free(press);
+        free(snd);
+        free(sh);
+    } else if (s->visc_type == 2) {
+        csw_viscosity(s, pobj, X, out);
+    }
+    return 0;
+}
+
 /* run.fluid.viscosity (FluidManager.m:42, initialize.m:395-400): 0 'empty',
- * 1 'neumann_richtmyer'.  The Caramana-Shashkov-Whalen form is not restated. */
+ * 1 'neumann_richtmyer', 2 'caramana_shashkov_whalen'. */
 int orc_set_viscosity(orc_sim *s, int type, double linear, double quadratic) {
-    if (type != 0 && type != 1) {
+    if (type < 0 || type > 2) {
         snprintf(s->err, sizeof(s->err), "viscosity type %d is not supported", type);
         return -1;
     }

@@ -20,6 +20,7 @@
  *   source.m:16-31, classes/ImogenArray.m:189-192, fluid/FluidArray.m:29-34,
  *   gravity/GravityManager.m:104-116, fluid/Radiation.m:61-89   source step
  *   fluid/ArtificialViscosity.m:73-77, fluid/wFluidFlux.m:24   Neumann-Richtmyer viscosity
+ *   fluid/ArtificialViscosity.m:80-148,155-190   Caramana-Shashkov-Whalen viscosity
  *
  * Every array operation keeps MATLAB's left-to-right evaluation order so that the
  * rounding is the one the .m text prescribes.  Build without FP contraction
@@ -122,9 +123,12 @@ int orc_set_gravity(orc_sim *s, const double *potential, double mass_threshold);
 int orc_set_radiation(orc_sim *s, int type, double exponent, double initial_maximum);
 double orc_radiation_strength(const orc_sim *s);
 /* Artificial viscosity (fluid/ArtificialViscosity.m): type 0 'empty', 1
- * 'neumann_richtmyer' with linear / quadratic strengths; the term is added to the
+ * 'neumann_richtmyer', 2 'caramana_shashkov_whalen' (the averaging form the .m file
+ * enables, with cswSoundSpeed) with linear / quadratic strengths; the term is added to the
  * pressure of both wFluidFlux calls of every fluid sweep (wFluidFlux.m:24). */
 int orc_set_viscosity(orc_sim *s, int type, double linear, double quadratic);
+/* The term the viscosity adds to the pressure for the current state along X (unit tests). */
+int orc_viscosity_term(orc_sim *s, int X, double *out);
 int orc_source(orc_sim *s);
 
 /* Time state. */

@@ -90,6 +90,7 @@ def lib():
         L.orc_source.argtypes = [ctypes.c_void_p]
         L.orc_set_viscosity.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
                                         ctypes.c_double]
+        L.orc_viscosity_term.argtypes = [ctypes.c_void_p, ctypes.c_int, P]
         _lib = L
     return _lib
 
@@ -222,9 +223,16 @@ class Oracle:
         return self.L.orc_radiation_strength(self.h)
 
     def set_viscosity(self, kind, linear=0.0, quadratic=0.0):
-        """Artificial viscosity: 'empty' or 'neumann_richtmyer'."""
-        t = {"empty": 0, "neumann_richtmyer": 1}[kind] if isinstance(kind, str) else int(kind)
+        """Artificial viscosity: 'empty', 'neumann_richtmyer' or 'caramana_shashkov_whalen'."""
+        t = ({"empty": 0, "neumann_richtmyer": 1, "caramana_shashkov_whalen": 2}[kind]
+             if isinstance(kind, str) else int(kind))
         self._check(self.L.orc_set_viscosity(self.h, t, float(linear), float(quadratic)))
+
+    def viscosity_term(self, X):
+        """The pressure term of the configured viscosity for the current state along X."""
+        out = np.empty(self.shape, dtype=np.float64)
+        self._check(self.L.orc_viscosity_term(self.h, int(X), _ptr(out)))
+        return out
 
     def source(self):
         self._check(self.L.orc_source(self.h))

@@ -4,7 +4,9 @@ debug-run strengths (run_Debug.m:17-19), 2-D hydro with transparent ends (the mo
 FluidArray's and StorageArray's trackers), 2-D MHD with mirror / wall ends (sign flips of
 the shifted momentum, stored wall edges), and 3-D MHD with gravity and radiation on too.
 Graph replay equals eager iterations, a checkpoint resumes bit for bit, FP32-bulk stays
-close to FP64, and the Caramana-Shashkov-Whalen form is refused."""
+close to FP64.  The Caramana-Shashkov-Whalen form in lockstep too: a Sod tube, 2-D MHD
+with circ / const / mirror / flip / zero ends, 3-D MHD with gravity and radiation, and a
+1-D problem on every candidate tile; it is refused with transparent ends."""
 import numpy as np
 import pytest
 
@@ -99,8 +101,66 @@ def test_fp32_viscosity_close_to_fp64():
         r.close()
 
 
+def csw_lockstep(ini, st, iterations, fields, visc=(0.3, 0.8), tile=None, **kw):
+    orc = make_oracle(ini, st)
+    run = make_gpu(ini, st)
+    if tile:
+        run.set_fluid_tile(*tile)
+    orc.set_viscosity("caramana_shashkov_whalen", *visc)
+    run.set_viscosity("caramana_shashkov_whalen", *visc)
+    if kw.get("grav"):
+        phi = potential(tuple(ini["grid"]))
+        orc.set_gravity(phi)
+        run.set_gravity(phi)
+    if kw.get("rad"):
+        orc.set_radiation("optically_thin", *kw["rad"])
+        run.set_radiation("optically_thin", *kw["rad"])
+    for it in range(iterations):
+        print("iteration", it + 1, "tile", tile, flush=True)
+        orc.iteration()
+        iteration(run)
+        try:
+            compare(run, orc, fields, TOL)
+        except AssertionError as e:
+            raise AssertionError("tile %s iteration %d: %s" % (tile, it + 1, e))
+    run.close()
+    orc.close()
+
+
+def test_csw_sod_lockstep():
+    ini, st = problems.sod(1024, bc="const", iter_max=10 ** 6)
+    csw_lockstep(ini, st, 30, FLUID_FIELDS, visc=(0.1, 0.25))
+
+
+@pytest.mark.parametrize("bc", [
+    {"x": "circ", "y": "const", "z": "circ"},
+    {"x": "mirror", "y": "flip", "z": "circ"},
+    {"x": "zero", "y": "circ", "z": "circ"},
+])
+def test_csw_2d_lockstep(bc):
+    ini, st = box((48, 40, 1), bc, magnet=True)
+    csw_lockstep(ini, st, 5, FLUID_FIELDS + MAG_FIELDS)
+
+
+def test_csw_3d_mhd_with_sources_lockstep():
+    ini, st = box((24, 20, 16), {"x": "circ", "y": "mirror", "z": "const"}, magnet=True)
+    csw_lockstep(ini, st, 3, FLUID_FIELDS + MAG_FIELDS, grav=True, rad=(0.5, 1.0))
+
+
+def test_csw_every_candidate_tile():
+    """1-D pencils of 300 cells on every tile the autotuner may pick (K = 1..8 cells per
+    thread, several pencils per block): the neighbour rows cross chunk edges."""
+    from imogen_amd import autotune
+    ini, st = problems.mhd_riemann(300, seed=3)
+    ini = dict(ini, bcMode={"x": "const", "y": "circ", "z": "circ"})
+    tiles = [t for t in autotune.candidates(autotune._pow2(300), 8, False, 1) if t[0] > 0]
+    assert tiles
+    for bs, k, px in tiles:
+        csw_lockstep(ini, st, 3, FLUID_FIELDS + MAG_FIELDS, tile=(1, bs, k, px))
+
+
 def test_unsupported_viscosity_is_refused():
-    ini, st = box((32, 24, 1), "circ")
+    ini, st = box((32, 24, 1), {"x": "trans", "y": "circ", "z": "circ"})
     run = make_gpu(ini, st)
     with pytest.raises(ImogenError):
         run.set_viscosity("caramana_shashkov_whalen", 0.1, 0.2)

@@ -247,4 +247,91 @@ def test_viscosity_rejects_fade_and_unknown_types():
     o = O.Oracle(O.make_config((32, 24, 1), (0.1, 0.1, 0.1)))
     o.set_state(*uniform((32, 24, 1)))
     with pytest.raises(RuntimeError):
-        o.set_viscosity(2, 0.1, 0.2)
+        o.set_viscosity(3, 0.1, 0.2)
+
+
+def _line_state(v, rho=1.0, P=1.0):
+    n = v.size
+    shape = (1, 1, n)
+    mass = np.full(shape, rho)
+    mom = [(rho * v).reshape(shape), np.zeros(shape), np.zeros(shape)]
+    ener = P / (GAMMA - 1.0) + 0.5 * rho * v.reshape(shape) ** 2
+    return mass, mom, ener, [np.zeros(shape)] * 3
+
+
+def _term(kind, v, c1, c2, bc="const", rho=1.0, P=1.0):
+    n = v.size
+    o = O.Oracle(O.make_config((n, 1, 1), (1.0 / n,) * 3, gamma=GAMMA, bc=bc))
+    o.set_state(*_line_state(v, rho, P))
+    o.set_viscosity(kind, c1, c2)
+    return o.viscosity_term(1).ravel()
+
+
+def test_neumann_richtmyer_term_closed_form():
+    """q = vm (c1 c + c2 vm / rho), vm = min(p(i+1) - p(i), 0), c the fast speed
+    (here the sound speed, no field)."""
+    n, rho, P = 32, 1.3, 0.7
+    x = (np.arange(n) + 0.5) / n
+    v = 0.4 - 0.6 * x
+    c1, c2 = 0.3, 0.8
+    q = _term("neumann_richtmyer", v, c1, c2, rho=rho, P=P)
+    p = rho * v
+    vm = np.minimum(np.append(p[1:], p[-1]) - p, 0.0)  # constant BC: last ghost = p[N-1]
+    Pc = (GAMMA - 1.0) * (P / (GAMMA - 1.0))
+    cs = np.sqrt(GAMMA * Pc / rho)  # interior sound speed (kinetic part cancels)
+    np.testing.assert_allclose(q[:-1], (vm * (c1 * cs + c2 * vm / rho))[:-1], rtol=1e-12)
+    assert q[-1] == 0.0
+
+
+def test_csw_term_closed_form_on_linear_compression():
+    """Caramana, Shashkov & Whalen (J. Comput. Phys. 144, 1998): q = rho (c2 (g+1)/4 |dv|
+    + sqrt(c2^2 ((g+1)/4)^2 dv^2 + c1^2 cs^2)) (1 - psi) |dv| where dv < 0, with the
+    limiter psi = max(0, min(0.5 (rL + rR), 2 rL, 2 rR, 1)).  On v = a - b x the
+    .m text's dv spans two cells and the one-cell ratios are rL = rR = 1/4, so psi = 1/4;
+    rho is the harmonic mean of equal neighbours and cs the smaller neighbour sound
+    speed.  cswSoundSpeed's velSquared is mom1.^2 + mom2.^2 + mom3.^2 ./ mass.^2 (only the
+    last term is divided: MATLAB precedence), so its kinetic part is 0.5 rho (rho v)^2."""
+    n, rho, P = 40, 1.1, 0.9
+    dx = 1.0 / n
+    x = (np.arange(n) + 0.5) * dx
+    b = 0.5
+    v = 0.3 - b * x
+    c1, c2 = 0.4, 1.2
+    q = _term("caramana_shashkov_whalen", v, c1, c2, rho=rho, P=P)
+    dv = -2.0 * b * dx
+    E = P / (GAMMA - 1.0) + 0.5 * rho * v ** 2
+    cj = np.sqrt(np.abs(GAMMA * (GAMMA - 1.0) * (E - 0.5 * rho * (rho * v) ** 2) / rho))
+    cs = np.minimum(cj[1:-3], cj[3:-1])  # neighbours i-1, i+1 of cells 2..n-3
+    g4 = (GAMMA + 1.0) / 4.0
+    psi = 0.25
+    expect = rho * (c2 * g4 * abs(dv) + np.sqrt(c2 ** 2 * g4 ** 2 * dv ** 2 + c1 ** 2 * cs ** 2)) \
+        * (1.0 - psi) * abs(dv)
+    np.testing.assert_allclose(q[2:-2], expect, rtol=1e-9)
+    # expansion: no viscosity anywhere
+    assert np.all(_term("caramana_shashkov_whalen", -v, c1, c2, rho=rho, P=P) == 0.0)
+    # a jump: the limiter is off (psi = 0) at the step, and the term is larger there
+    vs = np.where(x < 0.5, 0.5, -0.5)
+    qs = _term("caramana_shashkov_whalen", vs, c1, c2, rho=rho, P=P)
+    assert qs.max() > 0 and np.argmax(qs) in (n // 2 - 2, n // 2 - 1, n // 2, n // 2 + 1)
+
+
+def test_csw_zero_strengths_is_the_inviscid_run():
+    a, b = _sod_oracle(256), _sod_oracle(256)
+    b.set_viscosity("caramana_shashkov_whalen", 0.0, 0.0)
+    for _ in range(20):
+        a.iteration()
+        b.iteration()
+    for f, v in _sweep_state(a).items():
+        np.testing.assert_array_equal(_sweep_state(b)[f], v, err_msg=f)
+
+
+def test_csw_sod_converges():
+    from tests.test_oracle import sod_exact
+    o = _sod_oracle(256)
+    o.set_viscosity("caramana_shashkov_whalen", 0.1, 0.25)
+    while o.running:
+        o.iteration()
+    xs = (np.arange(256) + 0.5) / 256
+    ex = sod_exact(xs, o.time, x0=(round(256 / 2) - 1) / 256)
+    err = float(np.sum(np.abs(o.get("mass").ravel() - ex)) / np.sum(np.abs(ex)))
+    assert err < 0.05, err
