@@ -212,8 +212,11 @@ int imh_init_problem(imh_ctx *ctx, int kind, const double *params, int nparams);
  * of the flux objects (InitializedArray.m:43-47).  A checkpoint holds those as well, so a
  * run resumed from it continues bit for bit.  Layout: a fixed header (magic, version,
  * precision, grid, sizes), the device time state, the flux-object flags, the 11 field
- * arrays in the context's precision, the edge stores (FP64).  Load into a context created
- * with the same configuration; its captured iterations are dropped. */
+ * arrays in the context's precision, the edge stores (FP64; the potential's and the
+ * momentum stores' trackers included).  Load into a context created with the same
+ * configuration, with the same imh_set_gravity / imh_set_radiation / imh_set_viscosity
+ * settings applied first (settings are not part of a checkpoint); its captured
+ * iterations are dropped. */
 int imh_checkpoint_bytes(imh_ctx *ctx, size_t *bytes);
 int imh_checkpoint_save(imh_ctx *ctx, void *buf, size_t bytes);
 int imh_checkpoint_load(imh_ctx *ctx, const void *buf, size_t bytes);
