@@ -87,6 +87,9 @@ SIGNATURES = {
     "imh_set_radiation": (_I, [_P, _I, ctypes.c_double, ctypes.c_double]),
     "imh_get_radiation_strength": (_I, [_P, _D]),
     "imh_set_viscosity": (_I, [_P, _I, ctypes.c_double, ctypes.c_double]),
+    "imh_batch_set_gravity": (_I, [_P, _I, _D, ctypes.c_size_t, ctypes.c_double]),
+    "imh_batch_set_radiation": (_I, [_P, _I, _I, ctypes.c_double, ctypes.c_double]),
+    "imh_batch_set_viscosity": (_I, [_P, _I, _I, ctypes.c_double, ctypes.c_double]),
     "imh_checkpoint_bytes": (_I, [_P, ctypes.POINTER(ctypes.c_size_t)]),
     "imh_checkpoint_save": (_I, [_P, ctypes.c_void_p, ctypes.c_size_t]),
     "imh_checkpoint_load": (_I, [_P, ctypes.c_void_p, ctypes.c_size_t]),

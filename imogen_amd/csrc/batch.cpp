@@ -39,13 +39,13 @@ using namespace imh;
 namespace {
 
 enum LaunchKind { L_BEGIN, L_CFL_RESET, L_CFL, L_FINALIZE, L_FLUID, L_MAG_ADV, L_MAG_CT, L_CELLMAG, L_END,
-                  L_FORK, L_JOIN };
+                  L_FORK, L_JOIN, L_SOURCE };
 
 constexpr int MAX_LANES = 2;
 
 struct Launch {
     LaunchKind kind;
-    int step;       // finalize: 1 or 2
+    int step;       // finalize: 1 or 2; source: the simulation
     int bs, kmax;   // fluid / magnet variant
     bool mhd;
     bool general;   // fluid: the kernel form with fade boundaries / first assignments
@@ -336,7 +336,7 @@ static int build_plans(imh_batch* b) {
                             const FluidTile& ft = c->ftile[X - 1];
                             const int dX = X - 1;
                             const bool gen = c->cfg.bc[0][dX] == IMH_BC_FADE ||
-                                             c->cfg.bc[1][dX] == IMH_BC_FADE;
+                                             c->cfg.bc[1][dX] == IMH_BC_FADE || c->visc_type;
                             // bs 0: the column march (one thread per pencil)
                             const bool col = ft.column && !gen;
                             const int gbs = col ? 0 : ft.bs, gkm = col ? 0 : ft.kmax;
@@ -427,6 +427,17 @@ static int build_plans(imh_batch* b) {
                         add_launch(p, c2, cm);
                     }
                 }
+            // source(): after each flux call, on the simulation's lane
+            for (int i = 0; i < n; ++i) {
+                const imh_ctx* c = b->ctx[i];
+                if (!c->grav && !c->rad_type) continue;
+                Launch s{};
+                s.kind = L_SOURCE;
+                s.lane = lane[i];
+                s.step = i;
+                s.bytes = double(9) * c->ncell * es;
+                add_launch(p, s, none);
+            }
         }
         fork_join(L_JOIN);
         Launch E{};
@@ -476,6 +487,7 @@ static hipError_t run_launch(imh_batch* b, const Launch& L, hipStream_t st) {
     case L_CELLMAG:
         return launch_cell_centered_batch<T>((const CellMagArgs<T>*)b->cargs, tl, L.ntiles, nb, st);
     case L_END: return launch_end_iteration(b->sims_dev, n, st);
+    case L_SOURCE: return imh_launch_source(b->ctx[L.step], st);
     case L_FORK:
     case L_JOIN: return hipSuccess;  // handled by run_plan
     }
@@ -632,10 +644,6 @@ int imh_batch_download_field(imh_batch* b, int sim, int field, double* out, size
 
 // First iterations of fresh simulations, then (re)build the argument blocks and plans.
 static int batch_prepare(imh_batch* b) {
-    for (imh_ctx* c : b->ctx)
-        if (c->grav || c->rad_type || c->visc_type)
-            return fail(IMH_ENOTSUP, "source terms (gravity, radiation) and artificial "
-                        "viscosity run on single contexts, not in batched launches");
     bool changed = false;
     int rc = run_first_iterations(b, &changed);
     if (rc) return rc;
@@ -715,6 +723,31 @@ int imh_batch_set_fluid_tile(imh_batch* b, int sim, int X, int bs, int kmax, int
     if (rc) return rc;
     free_plans(b);  // argument blocks and plans are rebuilt before the next step
     return IMH_OK;
+}
+
+// Source terms and viscosity of one simulation of the batch (the single-context calls);
+// the plans are rebuilt before the next step.
+int imh_batch_set_gravity(imh_batch* b, int sim, const double* potential, size_t n,
+                          double mass_threshold) {
+    if (!b || sim < 0 || sim >= b->n) return fail(IMH_EINVAL, "bad simulation index");
+    int rc = imh_set_gravity(b->ctx[sim], potential, n, mass_threshold);
+    free_plans(b);
+    return rc;
+}
+
+int imh_batch_set_radiation(imh_batch* b, int sim, int type, double exponent,
+                            double initial_maximum) {
+    if (!b || sim < 0 || sim >= b->n) return fail(IMH_EINVAL, "bad simulation index");
+    int rc = imh_set_radiation(b->ctx[sim], type, exponent, initial_maximum);
+    free_plans(b);
+    return rc;
+}
+
+int imh_batch_set_viscosity(imh_batch* b, int sim, int type, double linear, double quadratic) {
+    if (!b || sim < 0 || sim >= b->n) return fail(IMH_EINVAL, "bad simulation index");
+    int rc = imh_set_viscosity(b->ctx[sim], type, linear, quadratic);
+    free_plans(b);
+    return rc;
 }
 
 int imh_batch_synchronize(imh_batch* b) {

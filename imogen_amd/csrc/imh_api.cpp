@@ -860,8 +860,7 @@ static void drop_graphs(imh_ctx* c) {
     }
 }
 
-int imh_source_internal(imh_ctx* c) {
-    if (!c->grav && c->rad_type == 0) return IMH_OK;
+hipError_t imh_launch_source(imh_ctx* c, hipStream_t st) {
     SourceArgs a;
     memset(&a, 0, sizeof(a));
     a.phi = c->grav;
@@ -891,8 +890,13 @@ int imh_source_internal(imh_ctx* c) {
     a.rad_strength = c->rad_strength;
     a.rad_exponent = c->rad_exponent;
     a.use = {!c->mag_is_zero[0], !c->mag_is_zero[1], !c->mag_is_zero[2]};
+    return launch_source(a, c->f, c->cfg.precision == IMH_FP64, st);
+}
+
+int imh_source_internal(imh_ctx* c) {
+    if (!c->grav && c->rad_type == 0) return IMH_OK;
     ProfScope ps(c, IMH_K_BOOK, 0.0);
-    hipError_t e = launch_source(a, c->f, c->cfg.precision == IMH_FP64, c->st);
+    hipError_t e = imh_launch_source(c, c->st);
     return e == hipSuccess ? IMH_OK : hip_fail(e, "source launch");
 }
 

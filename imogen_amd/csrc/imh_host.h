@@ -181,4 +181,6 @@ template <typename T>
 imh::CflArgs<T> make_cfl_args(const imh_ctx* c);
 int imh_iteration_body(imh_ctx* c, int conditional);
 int imh_source_internal(imh_ctx* c);
+// The source step's kernels of one context on stream st (no check of what is active).
+hipError_t imh_launch_source(imh_ctx* c, hipStream_t st);
 void imh_ctx_free(imh_ctx* c);

@@ -80,6 +80,26 @@ class Ensemble:
         check(lib().imh_batch_set_fluid_tile(self._b, int(i), int(X), int(bs), int(kmax),
                                              int(px)))
 
+    def set_gravity(self, i, potential, mass_threshold=0.0):
+        """Fixed gravitational potential of simulation i (None turns it off)."""
+        if potential is None:
+            check(lib().imh_batch_set_gravity(self._b, int(i), None, 0, float(mass_threshold)))
+            return
+        arr = as_field(potential, self.configs[i].ncell)
+        check(lib().imh_batch_set_gravity(self._b, int(i), dptr(arr), arr.size,
+                                          float(mass_threshold)))
+
+    def set_radiation(self, i, kind, exponent=0.5, initial_maximum=1.0):
+        """Radiation of simulation i ('empty' / 'optically_thin'), after its state upload."""
+        t = {"empty": 0, "optically_thin": 1}[kind]
+        check(lib().imh_batch_set_radiation(self._b, int(i), t, float(exponent),
+                                            float(initial_maximum)))
+
+    def set_viscosity(self, i, kind, linear=0.0, quadratic=0.0):
+        """Artificial viscosity of simulation i."""
+        t = {"empty": 0, "neumann_richtmyer": 1, "caramana_shashkov_whalen": 2}[kind]
+        check(lib().imh_batch_set_viscosity(self._b, int(i), t, float(linear), float(quadratic)))
+
     def synchronize(self):
         check(lib().imh_batch_synchronize(self._b))
 

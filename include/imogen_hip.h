@@ -153,7 +153,7 @@ int imh_run(imh_ctx *ctx, int max_iterations, int *done);
  * with exponent and initialMaximum (initialize.m:403-409); imh_set_radiation computes
  * the strength from the current state as Radiation.initialize does (Radiation.m:61-77),
  * so it is called after the state is in place.  Sources run on single contexts; a batch
- * holding a context with sources returns IMH_ENOTSUP. */
+ * holding a context with sources runs its source step inside the batched iteration. */
 int imh_source(imh_ctx *ctx);
 int imh_set_gravity(imh_ctx *ctx, const double *potential, size_t n, double mass_threshold);
 int imh_set_radiation(imh_ctx *ctx, int type, double exponent, double initial_maximum);
@@ -165,7 +165,7 @@ int imh_get_radiation_strength(imh_ctx *ctx, double *strength);
  * linearViscousStrength and quadraticViscousStrength; its term is added to the pressure
  * of both wFluidFlux calls of every fluid sweep (wFluidFlux.m:24).  Fade boundaries
  * return IMH_ENOTSUP, and type 2 also transparent and wall boundaries on a swept
- * dimension; batches refuse contexts with viscosity. */
+ * dimension. */
 int imh_set_viscosity(imh_ctx *ctx, int type, double linear, double quadratic);
 
 /* ---- time state -------------------------------------------------------------------- */
@@ -242,6 +242,14 @@ int imh_batch_step(imh_batch *b, int iterations);
 int imh_batch_get_time(imh_batch *b, int sim, double *time, double *dtime, int *iteration);
 int imh_batch_synchronize(imh_batch *b);
 int imh_batch_memory_bytes(imh_batch *b, size_t *bytes);
+/* imh_set_gravity / imh_set_radiation / imh_set_viscosity for simulation `sim` of the
+ * batch: its sweeps then run in the general kernel form and its source step follows each
+ * flux call inside the batched iteration. */
+int imh_batch_set_gravity(imh_batch *b, int sim, const double *potential, size_t n,
+                          double mass_threshold);
+int imh_batch_set_radiation(imh_batch *b, int sim, int type, double exponent,
+                            double initial_maximum);
+int imh_batch_set_viscosity(imh_batch *b, int sim, int type, double linear, double quadratic);
 
 /* ---- serving loop (continuous batching of solve requests on one device) ------------ */
 /* A request is one simulation run while TimeManager.running holds (to its TIMEMAX or
