@@ -19,6 +19,7 @@ Printed as one JSON line:
                    around each launch on the batch stream (an eager pass of the same
                    launch plan), algorithmic bytes = compulsory field reads + writes
   plain_roofline   the same for the plain FP64 path on a 256^3 hydro grid
+  source_roofline  the source step (gravity + radiation) on that grid, FP64
   cpu_baseline     the CPU oracle (oracle/, C FP64 restatement), single thread, on a
                    bounded sample of the ensemble
   configs          Sod 1024, Orszag-Tang 512^2 and Sedov 256^3 one at a time on the plain
@@ -258,9 +259,10 @@ def field_errors(pairs):
     return out
 
 
-def roofline(prof, label):
-    """Dominant kernel kind of a profile {kind: (ms, bytes, launches)}."""
-    kind = max((k for k in prof if k != "bookkeeping"), key=lambda k: prof[k][0])
+def roofline(prof, label, kind=None):
+    """Dominant kernel kind of a profile {kind: (ms, bytes, launches)}, or `kind`."""
+    if kind is None:
+        kind = max((k for k in prof if k != "bookkeeping"), key=lambda k: prof[k][0])
     ms, by, cnt = prof[kind]
     avg_s = ms / 1e3 / max(cnt, 1)
     achieved = by / max(cnt, 1) / avg_s / 1e9
@@ -492,6 +494,32 @@ def large_plain_roofline(args):
     return roofline(prof, "sedov 256^3 hydro FP64, plain path")
 
 
+def source_roofline(args):
+    """The source step (gravity from a fixed potential + optically thin radiation) on the
+    plain FP64 path, Sedov 256^3: HIP-event time of its launches (ghost + cell kernels)."""
+    import numpy as np
+
+    import imogen_amd
+    from imogen_amd import problems
+    ini, st = problems.sedov(256)
+    run = imogen_amd.initialize(ini, "fp64")
+    imogen_amd.create_arrays(run, st)
+    del st
+    x = (np.arange(256) + 0.5) / 256
+    Z, Y, X = np.meshgrid(x, x, x, indexing="ij")
+    run.set_gravity(((X - 0.5) ** 2 + (Y - 0.5) ** 2 + (Z - 0.5) ** 2).reshape(-1))
+    del X, Y, Z
+    run.set_radiation("optically_thin", 0.5, 1e-3)
+    run.run_iterations(1)
+    prof = run.profile_iterations(2)
+    run.close()
+    out = roofline(prof, "sedov 256^3 FP64 with gravity and radiation, source step",
+                   kind="source")
+    out["bytes_rule"] = ("per cell: mass, momentum, energy, potential (8 B) and cell-centred "
+                         "B read; momentum and energy written")
+    return out
+
+
 def main():
     args = parse()
     if args.pmc_traffic:
@@ -559,6 +587,8 @@ def main():
         log("plain fp64 sedov 256^3")
         out["plain_roofline"] = large_plain_roofline(args)
         out["plain_roofline"]["traffic"] = measured_traffic(out["plain_roofline"]["workload"])
+        log("source step, sedov 256^3")
+        out["source_roofline"] = source_roofline(args)
 
     if not args.no_serve:
         log("serving stream")

@@ -120,8 +120,8 @@ SIGNATURES = {
 
 # Tuning and measurement entry points (include/imogen_hip_ext.h; not part of the drop-in
 # boundary).
-NKIND = 6
-KERNEL_KINDS = ["cfl", "fluid", "mag_adv", "mag_ct", "cellmag", "bookkeeping"]
+NKIND = 7
+KERNEL_KINDS = ["cfl", "fluid", "mag_adv", "mag_ct", "cellmag", "bookkeeping", "source"]
 EXTRA = {
     "imh_set_fluid_tile": (_I, [_P, _I, _I, _I, _I]),
     "imh_get_fluid_tile": (_I, [_P, _I, ctypes.POINTER(_I), ctypes.POINTER(_I),

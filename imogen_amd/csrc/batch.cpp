@@ -63,6 +63,7 @@ int launch_kind_class(LaunchKind k) {
     case L_MAG_ADV: return IMH_K_MAG_ADV;
     case L_MAG_CT: return IMH_K_MAG_CT;
     case L_CELLMAG: return IMH_K_CELLMAG;
+    case L_SOURCE: return IMH_K_SOURCE;
     default: return IMH_K_BOOK;
     }
 }
@@ -435,7 +436,7 @@ static int build_plans(imh_batch* b) {
                 s.kind = L_SOURCE;
                 s.lane = lane[i];
                 s.step = i;
-                s.bytes = double(9) * c->ncell * es;
+                s.bytes = source_bytes(c);
                 add_launch(p, s, none);
             }
         }

@@ -893,9 +893,18 @@ hipError_t imh_launch_source(imh_ctx* c, hipStream_t st) {
     return launch_source(a, c->f, c->cfg.precision == IMH_FP64, st);
 }
 
+// Algorithmic HBM bytes of one source step: mass, momentum and energy read, the
+// potential read with gravity, cell-centred B read with radiation; momentum and energy
+// written with gravity, energy alone otherwise.
+double source_bytes(const imh_ctx* c) {
+    const double es = c->cfg.precision == IMH_FP64 ? 8.0 : 4.0;
+    double per = 5 * es + (c->grav ? 8.0 + 4 * es : es) + (c->rad_type ? 3 * es : 0.0);
+    return per * double(c->ncell);
+}
+
 int imh_source_internal(imh_ctx* c) {
     if (!c->grav && c->rad_type == 0) return IMH_OK;
-    ProfScope ps(c, IMH_K_BOOK, 0.0);
+    ProfScope ps(c, IMH_K_SOURCE, source_bytes(c));
     hipError_t e = imh_launch_source(c, c->st);
     return e == hipSuccess ? IMH_OK : hip_fail(e, "source launch");
 }

@@ -183,4 +183,5 @@ int imh_iteration_body(imh_ctx* c, int conditional);
 int imh_source_internal(imh_ctx* c);
 // The source step's kernels of one context on stream st (no check of what is active).
 hipError_t imh_launch_source(imh_ctx* c, hipStream_t st);
+double source_bytes(const imh_ctx* c);
 void imh_ctx_free(imh_ctx* c);

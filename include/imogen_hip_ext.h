@@ -21,7 +21,8 @@ extern "C" {
 #define IMH_K_MAG_CT 3    /* relaxingMagnet constrained-transport step */
 #define IMH_K_CELLMAG 4   /* MagnetArray.updateCellCentered */
 #define IMH_K_BOOK 5      /* per-iteration time bookkeeping */
-#define IMH_NKIND 6
+#define IMH_K_SOURCE 6    /* source step (gravity, radiation) */
+#define IMH_NKIND 7
 
 /* Pin the fluid sweep tile along dim X (1..3): block size, registers per thread (KMAX)
  * and pencils per block; px = 0 restores the automatic choice.  Drops captured graphs. */
