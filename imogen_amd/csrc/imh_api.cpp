@@ -974,13 +974,18 @@ int imh_set_gravity(imh_ctx* c, const double* potential, size_t n, double mass_t
     if (potential && n != (size_t)c->ncell)
         return fail(IMH_EINVAL, "potential holds %zu values, the grid %lld", n, c->ncell);
     if (potential) {
-        for (int d = 0; d < c->ndims; ++d) {
-            if (c->n[d] < 3)
-                return fail(IMH_ENOTSUP, "gravity needs >= 3 cells along dim %d", d + 1);
-            for (int s = 0; s < 2; ++s)
-                if (c->cfg.bc[s][d] == IMH_BC_FADE)
+        for (int d = 0; d < c->ndims; ++d)
+            for (int s = 0; s < 2; ++s) {
+                const int m = c->cfg.bc[s][d];
+                if (m == IMH_BC_FADE)
                     return fail(IMH_ENOTSUP, "gravity with a fade boundary is not supported");
-        }
+                // a dimension of 1 or 2 cells (a 1-D grid's y): only shifts that stay in
+                // the pencil (the reference's other shift rules index outside it)
+                if (c->n[d] < 3 && !(m == IMH_BC_CIRC || m == IMH_BC_CONST ||
+                                     m == IMH_BC_FLIP || m == IMH_BC_ZERO))
+                    return fail(IMH_ENOTSUP, "gravity along a dimension of < 3 cells needs "
+                                "circ, const, flip or zero boundaries there");
+            }
     }
     hipError_t e = hipStreamSynchronize(c->st);
     if (e != hipSuccess) return hip_fail(e, "sync");

@@ -1007,8 +1007,13 @@ int orc_set_gravity(orc_sim *s, const double *potential, double mass_threshold) 
     if (!potential) return 0;
     for (int d = 0; d < s->ndims; ++d)
         for (int side = 0; side < 2; ++side) {
-            if (s->n[d] < 3) {
-                snprintf(s->err, sizeof(s->err), "gravity needs >= 3 cells along dim %d", d + 1);
+            /* fewer than 3 cells: only the modes whose shifts are index maps clamped or
+             * wrapped into the pencil (the others index outside it, as the .m files do) */
+            const int m = s->cfg.bc[side][d];
+            if (s->n[d] < 3 && !(m == ORC_BC_CIRC || m == ORC_BC_CONST || m == ORC_BC_FLIP ||
+                                 m == ORC_BC_ZERO)) {
+                snprintf(s->err, sizeof(s->err),
+                         "gravity along dim %d of < 3 cells needs circ/const/flip/zero", d + 1);
                 return -1;
             }
             if (s->cfg.bc[side][d] == ORC_BC_FADE) {

@@ -97,6 +97,23 @@ def test_gravity_and_radiation_3d_mhd_lockstep():
     orc.close()
 
 
+def test_gravity_1d_lockstep():
+    """Sod with a linear potential: y has one cell (shifted, circular)."""
+    ini, st = problems.sod(512, bc="const", iter_max=10 ** 6)
+    ini = dict(ini, bcMode={"x": "const", "y": "circ", "z": "circ"})
+    orc = make_oracle(ini, st)
+    run = make_gpu(ini, st)
+    phi = (0.5 * (np.arange(512) + 0.5) / 512).reshape(1, 1, 512)
+    orc.set_gravity(phi)
+    run.set_gravity(phi)
+    for it in range(20):
+        orc.iteration()
+        iteration(run)
+        compare(run, orc, FLUID_FIELDS, TOL)
+    run.close()
+    orc.close()
+
+
 def test_direct_source_call():
     ini, st = box((32, 24, 1), {"x": "trans", "y": "circ", "z": "circ"})
     orc = make_oracle(ini, st)
