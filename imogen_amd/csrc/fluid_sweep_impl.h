@@ -214,7 +214,7 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
     constexpr int K = KMAX;
     // chunk boundary values of the group: [field*4 + q], q = 0 first FR, 1 last FR,
     // 2 first FL, 3 last FL
-    __shared__ T s_x[4 * G][BS];
+    __shared__ __attribute__((aligned(16))) T s_x[4 * G][BS];
     __shared__ double s_ends[2][G][PXMAX][6];   // FR / FL pencil ends (a[0..2], a[N-3..N-1])
     __shared__ double s_gh[PXMAX][G][4];        // ghosts: FR lower, FL lower, FR upper, FL upper
     __shared__ T s_red[2][(BS / 64) * PXMAX];
@@ -509,19 +509,24 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
         if constexpr (4 * G < 5) fn(FieldIdx<(4 * G < 5 ? 4 * G : 0)>{});
     };
 
-    // press = press + viscosity.solve(...) (wFluidFlux.m:24), Neumann-Richtmyer
-    // (ArtificialViscosity.m:73-77): vm = min(p(i+1) - p(i), 0) of the sweep-direction
-    // momentum of state Q, result vm .* (c1 .* c_f + c2 .* vm ./ rho).  The shift's upper
-    // ghost follows the BC of object obj (the FluidArray in the predictor, its
-    // StorageArray in the corrector) and nudges that object's tracker; `first` = the
-    // object's first assignment, whose tracker starts at the current end value.
-    // Caramana-Shashkov-Whalen (ArtificialViscosity.m:80-148 with cswSoundSpeed, :155-190)
-    // for boundaries that are index maps (circ, const, zero, mirror, flip; the others are
-    // refused at set-up): every shifted array is then a cell row at a wrapped, clamped or
-    // reflected index, mom(X) changing sign at mirror and flip ends.  Three per-cell rows
-    // carry it: S (cswSoundSpeed of one cell, whose velSquared divides only the third
-    // momentum by mass^2, as the .m text does), m (mass) and V (mom(X) ./ mass).
-    auto csw = [&](const T (&Q)[5][K]) __attribute__((always_inline)) {
+    // press = press + viscosity.solve(...) (wFluidFlux.m:24).  The objects of the five
+    // fields are obj + 0..4: the FluidArrays (OBJ_PRIM) in the predictor, their
+    // StorageArrays (OBJ_FSTORE) in the corrector; `first` = the corrector of the run's
+    // first sweep, the StorageArrays' first assignment.
+    // Neumann-Richtmyer (ArtificialViscosity.m:73-77): vm = min(p(i+1) - p(i), 0) of the
+    // sweep-direction momentum, result vm .* (c1 .* c_f + c2 .* vm ./ rho); the shift's
+    // upper ghost follows that momentum object's BC and nudges its tracker.
+    // Caramana-Shashkov-Whalen (ArtificialViscosity.m:80-148 with cswSoundSpeed,
+    // :155-190): inside the pencil every shifted array is a cell row, so three per-cell
+    // rows carry it: S (cswSoundSpeed of one cell, whose velSquared divides only the third
+    // momentum by mass^2, as the .m text does), m (mass) and V (mom(X) ./ mass).  Past a
+    // circ / const / zero / mirror / flip end the shifts are index maps (mom(X) changing
+    // sign at mirror and flip ends).  Past a transparent or wall end every shift call has
+    // its own ghost from its object's tracker (about 30 calls per wFluidFlux, mass,
+    // momentum, energy and cell-centred B objects): one lane per pencil replays the calls
+    // of each side in the .m text's order and leaves the five values the cells read there
+    // (massB/C, sndB/C, v0, v1 / v2, v3), staged in the exchange rows s_x (unused here).
+    auto csw = [&](const T (&Q)[5][K], int obj) __attribute__((always_inline)) {
         const T G = T(g);
         T Sr[K], Mr[K], Vr[K];
 #pragma unroll
@@ -543,6 +548,13 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
         s_cw[5][tid] = Vr[K > 1 ? 1 : 0];
         s_cw[6][tid] = Vr[K > 1 ? K - 2 : 0];
         s_cw[7][tid] = Vr[K - 1];
+        // tracked ends: the 8 objects' end cells (mass, mom 1..3, ener, cellMag 1..3) and
+        // the lane's results, in the exchange rows
+        const bool trk_any = trk_lo || trk_hi;
+        double* const xe = reinterpret_cast<double*>(&s_x[0][0]);
+        constexpr int XE_ENDS = 48, XE_OUT = 10;
+        static_assert(sizeof(s_x) >= sizeof(double) * PXMAX * (XE_ENDS + XE_OUT),
+                      "exchange rows too small for the CSW pencil ends");
         if (pen_ok) {
 #pragma unroll
             for (int k = 0; k < K; ++k) {
@@ -554,15 +566,105 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
                     s_cend[px][0][slot] = double(Sr[k]);
                     s_cend[px][1][slot] = double(Mr[k]);
                     s_cend[px][2][slot] = double(Vr[k]);
+                    if (trk_any) {
+                        double* e = xe + px * XE_ENDS;
+#pragma unroll
+                        for (int v = 0; v < 5; ++v) e[v * 6 + slot] = double(Q[v][k]);
+#pragma unroll
+                        for (int b = 0; b < 3; ++b) e[(5 + b) * 6 + slot] = double(bm[b][k]);
+                    }
                 }
             }
         }
         __syncthreads();
-        // row value (0 S, 1 m, 2 V) at pencil index i0 + k + o, |o| <= 2
+        if (trk_any && tid < PX && pen0 + tid < a.npencil) {
+            const int p = tid;
+            const double* e = xe + p * XE_ENDS;
+            double* out = xe + PXMAX * XE_ENDS + p * XE_OUT;
+            const double Gd = g;
+            for (int up = 0; up < 2; ++up) {
+                const int mode = up ? mhi : mlo;
+                if (mode != IMH_BC_TRANS && mode != IMH_BC_WALL) continue;
+                // the side's trackers: rows 0..4 obj + v, rows 5..7 OBJ_CMAG + b
+                double tv[8];
+                double* tp[8];
+                for (int r = 0; r < 8; ++r) {
+                    const int o = r < 5 ? obj + r : OBJ_CMAG + (r - 5);
+                    tp[r] = a.edges + a.eoff[(o * 2 + up) * 3 + d] + pen0 + p;
+                    tv[r] = *tp[r];
+                }
+                // one shift call of row r by m cells on this side: ghost(s) g0 (, g1)
+                auto call = [&](int r, int m, double& g0, double& g1) {
+                    GhostSpec sp;
+                    sp.mode = mode;
+                    sp.upper = up;
+                    sp.flip_sign = (r == 1 + d) || (r == 5 + d);
+                    sp.zero_ghost = 0;
+                    sp.active_linear = a.active_lin;
+                    sp.bc_infinity = f;
+                    const double* x = e + r * 6;
+                    const Ends en = {x[0], x[1], x[2], x[3], x[4], x[5]};
+                    ghost_values(sp, en, N, m, &tv[r], g0, g1);
+                };
+                double g0, g1, u0, u1;
+                call(0, 1, g0, g1);                       // massB / massC
+                const double mEnd = g0;
+                double t1, t2, t3, tm;                    // cswSoundSpeed, this side
+                call(1, 1, t1, g1);
+                call(2, 1, t2, g1);
+                call(3, 1, t3, g1);
+                call(0, 1, tm, g1);
+                const double vel2 = t1 * t1 + t2 * t2 + t3 * t3 / (tm * tm);
+                double b1, b2, b3;
+                call(5, 1, b1, g1);
+                call(6, 1, b2, g1);
+                call(7, 1, b3, g1);
+                const double mag2 = b1 * b1 + b2 * b2 + b3 * b3;
+                double te, tm2, tm3;
+                call(4, 1, te, g1);
+                call(0, 1, tm2, g1);
+                double sp = (Gd - 1.0) * (te - 0.5 * tm2 * vel2);
+                sp = sp - (Gd - 1.0) * 0.5 * mag2;
+                call(0, 1, tm3, g1);
+                const double snd = sqrt(fabs((Gd * sp + 2.0 * mag2) / tm3));
+                double* o = out + up * 5;
+                o[0] = mEnd;
+                o[1] = snd;
+                if (!up) {
+                    call(1 + d, 2, g0, g1);               // v0 = mom.shift(-2) ./ mass.shift(-2)
+                    call(0, 2, u0, u1);
+                    o[2] = g0 / u0;
+                    o[3] = g1 / u1;
+                    call(1 + d, 1, g0, g1);               // v1
+                    call(0, 1, u0, u1);
+                    o[4] = g0 / u0;
+                } else {
+                    call(1 + d, 1, g0, g1);               // v2
+                    call(0, 1, u0, u1);
+                    o[2] = g0 / u0;
+                    call(1 + d, 2, g0, g1);               // v3 = mom.shift(2) ./ mass.shift(2)
+                    call(0, 2, u0, u1);
+                    o[3] = g0 / u0;
+                    o[4] = g1 / u1;
+                }
+                for (int r = 0; r < 8; ++r)
+                    if (mode == IMH_BC_TRANS) *tp[r] = tv[r];
+            }
+        }
+        if (trk_any) __syncthreads();
+        // row value (0 S, 1 m, 2 V) of the array shifted by -o at cell i0 + k, |o| <= 2
         auto at = [&](int row, const T (&R)[K], int k, int o) __attribute__((always_inline)) -> T {
-            const int j = i0 + k + o;
+            const int i = i0 + k;
+            const int j = i + o;
             if (j < 0 || j >= N) {
                 const int mode = j < 0 ? mlo : mhi;
+                if (mode == IMH_BC_TRANS || mode == IMH_BC_WALL) {
+                    const double* ov = xe + PXMAX * XE_ENDS + px * XE_OUT + (j < 0 ? 0 : 5);
+                    if (row == 1) return T(ov[0]);
+                    if (row == 0) return T(ov[1]);
+                    if (j < 0) return T(o == -1 ? ov[4] : (i == 0 ? ov[2] : ov[3]));
+                    return T(o == 1 ? ov[2] : (i == N - 2 ? ov[3] : ov[4]));
+                }
                 int jj;
                 bool neg = false;
                 if (mode == IMH_BC_CIRC) {
@@ -604,13 +706,14 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
             const T B = sqrt(c2 * c2 * (q4 * q4) * (dv * dv) + c1 * c1 * (snd * snd));
             P[k] = P[k] + (dv < T(0) ? T(1) : T(0)) * rho * (A + B) * (T(1) - lim) * fabs(dv);
         }
+        // the group loop that follows writes the exchange rows
+        if (trk_any) __syncthreads();
     };
-
     auto viscosity = [&](const T (&Q)[5][K], int obj, bool first) __attribute__((always_inline)) {
         if constexpr (GEN) {
             if (a.visc_type == 0) return;
             if (a.visc_type == 2) {
-                csw(Q);
+                csw(Q, obj);
                 return;
             }
             T mX[K];
@@ -629,7 +732,7 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
             if (tid < PX && pen0 + tid < a.npencil) {
                 const double* e = s_vend[tid];
                 const bool trk = (mhi == IMH_BC_TRANS || mhi == IMH_BC_WALL);
-                double* tp = trk ? a.edges + a.eoff[(obj * 2 + 1) * 3 + d] + pen0 + tid : nullptr;
+                double* tp = trk ? a.edges + a.eoff[((obj + X) * 2 + 1) * 3 + d] + pen0 + tid : nullptr;
                 double tv = trk ? (first ? e[5] : *tp) : 0.0;
                 GhostSpec sp;
                 sp.mode = mhi;
@@ -658,7 +761,7 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
 
     // ================= predictor: W and C from U^n (wFluidFlux.m) ====================
     const T C = pencil_reduce_max(cell_speeds(q), a.xmajor, TL, PX, tid, px, s_red[0]);
-    viscosity(q, OBJ_PRIM + X, false);
+    viscosity(q, OBJ_PRIM, false);
     const T rC = M::rcp(C);
     const T hfC = T(hf) * C;
     T qh[5][K];
@@ -699,38 +802,40 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
             }
         });
     });
+    // First assignment of the StorageArrays (InitializedArray.m:43-47): their edges, read
+    // only by the viscosity's shifts, come from this half step (the mass store's from the
+    // array before its MINMASS floor, which is its second assignment).
+    if constexpr (GEN) {
+        if (a.init_edges) {
+            auto init_store = [&](auto VC) __attribute__((always_inline)) {
+                constexpr int v = decltype(VC)::value;
+                init_faces(OBJ_FSTORE + v, qh[v]);
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    const int gi = i0 + k;
+                    if (!valid(k)) continue;
+                    if (gi == 0 && trk_lo)
+                        a.edges[a.eoff[((OBJ_FSTORE + v) * 2 + 0) * 3 + d] + pen] = double(qh[v][k]);
+                    if (gi == N - 1 && trk_hi)
+                        a.edges[a.eoff[((OBJ_FSTORE + v) * 2 + 1) * 3 + d] + pen] = double(qh[v][k]);
+                }
+            };
+            init_store(FieldIdx<0>{});
+            init_store(FieldIdx<1>{});
+            init_store(FieldIdx<2>{});
+            init_store(FieldIdx<3>{});
+            init_store(FieldIdx<4>{});
+        }
+    }
     // mass.store.array = max(mass.store.array, MINMASS)
 #pragma unroll
     for (int k = 0; k < K; ++k) qh[0][k] = fmax(qh[0][k], T(a.minmass));
 #pragma unroll
     for (int v = 0; v < 5; ++v) pad_fill(qh[v]);
 
-    // First assignment of the momentum StorageArrays (InitializedArray.m:43-47): their
-    // edges, read only by the viscosity's shifts, come from this half step.
-    if constexpr (GEN) {
-        if (a.init_edges) {
-            auto init_store = [&](auto CC) __attribute__((always_inline)) {
-                constexpr int cc = decltype(CC)::value;
-                init_faces(OBJ_FSTORE + cc, qh[1 + cc]);
-#pragma unroll
-                for (int k = 0; k < K; ++k) {
-                    const int gi = i0 + k;
-                    if (!valid(k)) continue;
-                    if (gi == 0 && trk_lo)
-                        a.edges[a.eoff[((OBJ_FSTORE + cc) * 2 + 0) * 3 + d] + pen] = double(qh[1 + cc][k]);
-                    if (gi == N - 1 && trk_hi)
-                        a.edges[a.eoff[((OBJ_FSTORE + cc) * 2 + 1) * 3 + d] + pen] = double(qh[1 + cc][k]);
-                }
-            };
-            init_store(FieldIdx<0>{});
-            init_store(FieldIdx<1>{});
-            init_store(FieldIdx<2>{});
-        }
-    }
-
     // ================= corrector: W' and C' from U^(n+1/2) ===========================
     const T C2 = pencil_reduce_max(cell_speeds(qh), a.xmajor, TL, PX, tid, px, s_red[1]);
-    viscosity(qh, OBJ_FSTORE + d, a.init_edges != 0);
+    viscosity(qh, OBJ_FSTORE, a.init_edges != 0);
     const T rC2 = M::rcp(C2);
     const T ffC = T(ff) * C2;
     const int lim = a.limiter;

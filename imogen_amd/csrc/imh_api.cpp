@@ -547,6 +547,23 @@ static int cellmag(imh_ctx* c, int k, int force) {
                                         : launch_cellmag<float>(c, k, force);
 }
 
+// The cellMag InitializedArrays take their edge stores from their first array, the one
+// the MagnetArray constructor computes (InitializedArray.m:43-47).
+static int cellmag_edges(imh_ctx* c) {
+    for (int k = 0; k < 3; ++k)
+        for (int s = 0; s < 2; ++s)
+            for (int d = 0; d < c->ndims; ++d) {
+                const int m = c->cfg.bc[s][d];
+                if (m != IMH_BC_TRANS && m != IMH_BC_WALL) continue;
+                hipError_t e = launch_edges_from_field(c->f[IMH_F_CMAGX + k],
+                                                       c->cfg.precision == IMH_FP64, c->n, d, s,
+                                                       c->edges + c->eoff[OBJ_CMAG + k][s][d],
+                                                       c->nface[d], c->st);
+                if (e != hipSuccess) return hip_fail(e, "cellMag edges");
+            }
+    return IMH_OK;
+}
+
 extern "C" int imh_upload_state(imh_ctx* c, const double* mass, const double* const mom[3],
                                 const double* ener, const double* const mag[3], size_t n) {
     if (!c || !mass || !mom || !ener || !mag) return fail(IMH_EINVAL, "null argument");
@@ -593,6 +610,7 @@ extern "C" int imh_upload_state(imh_ctx* c, const double* mass, const double* co
         int rc = cellmag(c, k, 1);
         if (rc) return rc;
     }
+    if (int rc = cellmag_edges(c)) return rc;
     e = hipStreamSynchronize(c->st);
     if (e != hipSuccess) return hip_fail(e, "cellMag");
     c->have_state = true;
@@ -1065,16 +1083,6 @@ int imh_set_radiation(imh_ctx* c, int type, double exponent, double initial_maxi
 int imh_set_viscosity(imh_ctx* c, int type, double linear, double quadratic) {
     if (!c) return fail(IMH_EINVAL, "null context");
     if (type < 0 || type > 2) return fail(IMH_EINVAL, "viscosity type %d", type);
-    if (type == 2)
-        for (int d = 0; d < c->ndims; ++d) {
-            if (c->n[d] < 3) continue;  // never swept (flux.m:39)
-            for (int s = 0; s < 2; ++s) {
-                const int m = c->cfg.bc[s][d];
-                if (m == IMH_BC_TRANS || m == IMH_BC_WALL || m == IMH_BC_FADE)
-                    return fail(IMH_ENOTSUP, "the Caramana-Shashkov-Whalen viscosity runs with "
-                                "circ, const, zero, mirror and flip boundaries");
-            }
-        }
     if (type)
         for (int d = 0; d < c->ndims; ++d)
             for (int s = 0; s < 2; ++s)
@@ -1312,6 +1320,7 @@ extern "C" int imh_init_problem(imh_ctx* c, int kind, const double* params, int 
         int rc = cellmag(c, k, 1);
         if (rc) return rc;
     }
+    if (int rc = cellmag_edges(c)) return rc;
     e = hipStreamSynchronize(c->st);
     if (e != hipSuccess) return hip_fail(e, "cellMag");
     c->have_state = true;

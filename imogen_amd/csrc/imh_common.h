@@ -31,8 +31,9 @@ enum ObjId : int {
     OBJ_WMAG = 46,   // mag(I).wMag(X)
     OBJ_MFLUX = 55,  // mag(I).flux(X)
     OBJ_GRAV = 64,   // grav (GravityArray)
-    OBJ_FSTORE = 65, // mom(1..3).store (StorageArray), shifted by the artificial viscosity
-    NOBJ = 68
+    OBJ_FSTORE = 65, // mass, mom(1..3), ener .store (StorageArray), shifted by the viscosity
+    OBJ_CMAG = 70,   // mag(1..3).cellMag (InitializedArray), shifted by the CSW viscosity
+    NOBJ = 73
 };
 
 // FluxArray.fluxType for zero_shift.m.

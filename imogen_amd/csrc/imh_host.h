@@ -162,6 +162,9 @@ struct SourceArgs {
 namespace imh {
 hipError_t launch_source(const SourceArgs& a, void* const f[IMH_NFIELDS], bool fp64,
                          hipStream_t st);
+// edges[f] = field at the lower (side 0) or upper (side 1) end of pencil f along d
+hipError_t launch_edges_from_field(const void* field, bool fp64, const int n[3], int d,
+                                   int side, double* edges, long long nface, hipStream_t st);
 }  // namespace imh
 
 // internal entry points (force = run regardless of the loop's running flag)

@@ -114,7 +114,32 @@ __global__ void __launch_bounds__(256) source_cell_kernel(SourceArgs a, T* mass,
     ener[c] = T(E);
 }
 
+template <typename T>
+__global__ void __launch_bounds__(256) edges_from_field_kernel(const T* field, int n0, int n1,
+                                                               int n2, int d, int side,
+                                                               double* edges, long long nface) {
+    const long long f = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nface) return;
+    const int n[3] = {n0, n1, n2};
+    long long base, stride;
+    pencil_base(n, d, f, base, stride);
+    edges[f] = double(field[base + (side ? (long long)(n[d] - 1) : 0) * stride]);
+}
+
 }  // namespace
+
+hipError_t launch_edges_from_field(const void* field, bool fp64, const int n[3], int d,
+                                   int side, double* edges, long long nface, hipStream_t st) {
+    if (nface <= 0) return hipSuccess;
+    const dim3 grid(unsigned((nface + 255) / 256));
+    if (fp64)
+        hipLaunchKernelGGL(edges_from_field_kernel<double>, grid, dim3(256), 0, st,
+                           (const double*)field, n[0], n[1], n[2], d, side, edges, nface);
+    else
+        hipLaunchKernelGGL(edges_from_field_kernel<float>, grid, dim3(256), 0, st,
+                           (const float*)field, n[0], n[1], n[2], d, side, edges, nface);
+    return hipGetLastError();
+}
 
 hipError_t launch_source(const SourceArgs& a, void* const f[IMH_NFIELDS], bool fp64,
                          hipStream_t st) {

@@ -164,8 +164,7 @@ int imh_get_radiation_strength(imh_ctx *ctx, double *strength);
  * 'caramana_shashkov_whalen' (ArtificialViscosity.m:80-148, 155-190) with
  * linearViscousStrength and quadraticViscousStrength; its term is added to the pressure
  * of both wFluidFlux calls of every fluid sweep (wFluidFlux.m:24).  Fade boundaries
- * return IMH_ENOTSUP, and type 2 also transparent and wall boundaries on a swept
- * dimension. */
+ * return IMH_ENOTSUP. */
 int imh_set_viscosity(imh_ctx *ctx, int type, double linear, double quadratic);
 
 /* ---- time state -------------------------------------------------------------------- */

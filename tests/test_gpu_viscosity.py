@@ -6,7 +6,8 @@ the shifted momentum, stored wall edges), and 3-D MHD with gravity and radiation
 Graph replay equals eager iterations, a checkpoint resumes bit for bit, FP32-bulk stays
 close to FP64.  The Caramana-Shashkov-Whalen form in lockstep too: a Sod tube, 2-D MHD
 with circ / const / mirror / flip / zero ends, 3-D MHD with gravity and radiation, and a
-1-D problem on every candidate tile; it is refused with transparent ends."""
+1-D problem on every candidate tile, and with transparent and wall ends (the trackers of
+every shifted object); fade ends are refused."""
 import numpy as np
 import pytest
 
@@ -159,9 +160,52 @@ def test_csw_every_candidate_tile():
         csw_lockstep(ini, st, 3, FLUID_FIELDS + MAG_FIELDS, tile=(1, bs, k, px))
 
 
+@pytest.mark.parametrize("bc", [
+    {"x": "trans", "y": "circ", "z": "circ"},
+    {"x": "wall", "y": "trans", "z": "circ"},
+    {"x": ["trans", "mirror"], "y": ["const", "wall"], "z": "circ"},
+])
+def test_csw_tracked_ends_lockstep(bc):
+    """Transparent and wall ends: every shift call of the CSW term has its own ghost from
+    its object's tracker (mass, momentum, energy and cell-centred B, FluidArrays in the
+    predictor and StorageArrays in the corrector)."""
+    ini, st = box((48, 40, 1), bc, magnet=True)
+    csw_lockstep(ini, st, 6, FLUID_FIELDS + MAG_FIELDS)
+
+
+def test_csw_transparent_sod_and_3d():
+    ini, st = problems.sod(1024, iter_max=10 ** 6)  # transparent ends
+    csw_lockstep(ini, st, 30, FLUID_FIELDS, visc=(0.1, 0.25))
+    ini, st = box((24, 20, 16), {"x": "trans", "y": "wall", "z": "trans"}, magnet=True)
+    csw_lockstep(ini, st, 3, FLUID_FIELDS + MAG_FIELDS, grav=True, rad=(0.5, 1.0))
+
+
+def test_csw_transparent_checkpoint_resume():
+    ini, st = box((48, 40, 1), {"x": "trans", "y": "trans", "z": "circ"}, magnet=True)
+    a = make_gpu(ini, st)
+    a.set_viscosity("caramana_shashkov_whalen", 0.3, 0.8)
+    for _ in range(8):
+        iteration(a)
+    b = make_gpu(ini, st)
+    b.set_viscosity("caramana_shashkov_whalen", 0.3, 0.8)
+    assert b.run_iterations(4) == 4
+    ck = b.checkpoint()
+    b.close()
+    c = make_gpu(ini, st)
+    c.set_viscosity("caramana_shashkov_whalen", 0.3, 0.8)
+    c.restore(ck)
+    assert c.run_iterations(4) == 4
+    for f in FLUID_FIELDS + MAG_FIELDS:
+        np.testing.assert_array_equal(c.download(f), a.download(f), err_msg=f)
+    a.close()
+    c.close()
+
+
 def test_unsupported_viscosity_is_refused():
-    ini, st = box((32, 24, 1), {"x": "trans", "y": "circ", "z": "circ"})
+    ini, st = box((32, 24, 1), {"x": "fade", "y": "circ", "z": "circ"})
+    ini = dict(ini, bcInfinity=8)
     run = make_gpu(ini, st)
-    with pytest.raises(ImogenError):
-        run.set_viscosity("caramana_shashkov_whalen", 0.1, 0.2)
+    for kind in ("neumann_richtmyer", "caramana_shashkov_whalen"):
+        with pytest.raises(ImogenError):
+            run.set_viscosity(kind, 0.1, 0.2)
     run.close()
