@@ -211,8 +211,8 @@ int imh_init_problem(imh_ctx *ctx, int kind, const double *params, int nparams);
  * of the flux objects (InitializedArray.m:43-47).  A checkpoint holds those as well, so a
  * run resumed from it continues bit for bit.  Layout: a fixed header (magic, version,
  * precision, grid, sizes), the device time state, the flux-object flags, the 11 field
- * arrays in the context's precision, the edge stores (FP64; the potential's and the
- * momentum stores' trackers included).  Load into a context created with the same
+ * arrays in the context's precision, the edge stores (FP64; the trackers of the
+ * potential, the StorageArrays and the cell-centred B included).  Load into a context created with the same
  * configuration, with the same imh_set_gravity / imh_set_radiation / imh_set_viscosity
  * settings applied first (settings are not part of a checkpoint); its captured
  * iterations are dropped. */
