@@ -903,6 +903,7 @@ hipError_t imh_launch_source(imh_ctx* c, hipStream_t st) {
     a.bc_inf = c->cfg.bc_infinity;
     a.rad_type = c->rad_type;
     a.sim = c->sim;
+    a.fade = c->fade_dev;
     a.mass_threshold = c->mass_threshold;
     a.gamma = c->cfg.gamma;
     a.rad_strength = c->rad_strength;
@@ -995,8 +996,6 @@ int imh_set_gravity(imh_ctx* c, const double* potential, size_t n, double mass_t
         for (int d = 0; d < c->ndims; ++d)
             for (int s = 0; s < 2; ++s) {
                 const int m = c->cfg.bc[s][d];
-                if (m == IMH_BC_FADE)
-                    return fail(IMH_ENOTSUP, "gravity with a fade boundary is not supported");
                 // a dimension of 1 or 2 cells (a 1-D grid's y): only shifts that stay in
                 // the pencil (the reference's other shift rules index outside it)
                 if (c->n[d] < 3 && !(m == IMH_BC_CIRC || m == IMH_BC_CONST ||

@@ -152,6 +152,7 @@ struct SourceArgs {
     int bc_inf;
     int rad_type;
     const imh::SimDev* sim;
+    const double* fade;       // fade_shift.m coefficients (device), bcInfinity long
     double mass_threshold;
     double dgrid[3];
     double gamma;

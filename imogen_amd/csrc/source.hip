@@ -93,6 +93,21 @@ __global__ void __launch_bounds__(256) source_cell_kernel(SourceArgs a, T* mass,
                 s1 = p < N - 1 ? a.phi[c + st] : g[2];
                 b1 = p >= 1 ? a.phi[c - st] : g[3];
                 b2 = p >= 2 ? a.phi[c - 2 * st] : g[4 + p];
+                if (a.modes[0][d] == IMH_BC_FADE || a.modes[1][d] == IMH_BC_FADE) {
+                    // fade_shift.m: the constant shift, then both ends of the shifted array
+                    // blended toward its end values over bcInfinity cells
+                    const double* row = a.phi + (c - (long long)p * st);
+                    auto at = [&](int i) { return row[(long long)(i < 0 ? 0 : (i > N - 1 ? N - 1 : i)) * st]; };
+                    auto faded = [&](int m, int upper, double r) {
+                        if (a.modes[upper][d] != IMH_BC_FADE) return r;
+                        const double v1 = at(upper ? m : -m), vN = at(upper ? N - 1 + m : N - 1 - m);
+                        return fade_blend(at(upper ? p + m : p - m), p, N, v1, vN, a.fade, a.bc_inf);
+                    };
+                    s2 = faded(2, 1, s2);
+                    s1 = faded(1, 1, s1);
+                    b1 = faded(1, 0, b1);
+                    b2 = faded(2, 0, b2);
+                }
             }
             const double d5 = (((-s2 + 8.0 * s1) - 8.0 * b1) + b2) / (12.0 * a.dgrid[d]);
             const double ms = (dt * thr) * d5;

@@ -147,7 +147,7 @@ int imh_run(imh_ctx *ctx, int max_iterations, int *done);
  * (column-major, n cells, FP64; NULL turns gravity off) and sets run.fluid.MASS_THRESHOLD
  * (initialize.m:72-77); the potential's edge stores start at zero, as a new GravityArray's
  * do.  Dimensions of fewer than 3 cells (a 1-D grid's y) need circ, const, flip or zero
- * boundaries there; fade boundaries are not supported with gravity.  The Poisson solvers stay out (a solved potential is uploaded
+ * boundaries there.  The Poisson solvers stay out (a solved potential is uploaded
  * with imh_set_gravity).
  * Radiation: run.fluid.radiation (Radiation.m): type 0 'empty', 1 'optically_thin'
  * with exponent and initialMaximum (initialize.m:403-409); imh_set_radiation computes

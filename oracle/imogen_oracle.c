@@ -1016,10 +1016,6 @@ int orc_set_gravity(orc_sim *s, const double *potential, double mass_threshold) 
                          "gravity along dim %d of < 3 cells needs circ/const/flip/zero", d + 1);
                 return -1;
             }
-            if (s->cfg.bc[side][d] == ORC_BC_FADE) {
-                snprintf(s->err, sizeof(s->err), "gravity with a fade BC is not supported");
-                return -1;
-            }
         }
     s->grav_potential = dup_arr(potential, s->ncell);
     grav_create(s);

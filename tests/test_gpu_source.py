@@ -61,6 +61,7 @@ def lockstep(ini, st, iterations, fields, grav=True, threshold=0.0, rad=None):
     {"x": "circ", "y": "trans", "z": "circ"},
     {"x": "trans", "y": "wall", "z": "circ"},
     {"x": "mirror", "y": "const", "z": "circ"},
+    {"x": "fade", "y": ["trans", "fade"], "z": "circ"},
 ])
 def test_gravity_2d_hydro_lockstep(bc):
     ini, st = box((48, 40, 1), bc)

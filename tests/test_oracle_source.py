@@ -163,15 +163,23 @@ def test_gravity_on_a_1d_grid():
     assert np.max(np.abs(o.get("momy"))) < 1e-15
 
 
-def test_gravity_rejects_short_dimension_and_fade():
+def test_gravity_short_dimension_rule_and_fade():
     o = O.Oracle(O.make_config((16, 2, 1), (0.1, 0.1, 0.1), bc="trans"))
     o.set_state(*uniform((16, 2, 1)))
     with pytest.raises(RuntimeError):
         o.set_gravity(np.zeros((1, 2, 16)))
-    o = O.Oracle(O.make_config((32, 24, 1), (0.1, 0.1, 0.1), bc="fade", bc_infinity=8))
-    o.set_state(*uniform((32, 24, 1)))
-    with pytest.raises(RuntimeError):
-        o.set_gravity(np.zeros((1, 24, 32)))
+    # fade ends: the shifts blend toward their end values; a linear potential keeps the
+    # interior force and the force changes only within bcInfinity cells of the ends
+    n, dt, g = 32, 1e-3, 0.5
+    o = O.Oracle(O.make_config((n, 24, 1), (1.0 / n,) * 3, bc="fade", bc_infinity=8))
+    o.set_state(*uniform((n, 24, 1)))
+    x = (np.arange(n) + 0.5) / n
+    o.set_gravity(np.broadcast_to(g * x, (1, 24, n)).copy())
+    o.dtime = dt
+    o.source()
+    mx = o.get("momx")[0, 0]
+    np.testing.assert_allclose(mx[8:-8], -dt * g, rtol=1e-9)
+    assert np.max(np.abs(mx[:8] + dt * g)) > 1e-6
 
 
 # ---- artificial viscosity (ArtificialViscosity.m:73-77) -----------------------------------
