@@ -752,7 +752,10 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
             const T gh = T(s_vgh[px]);
 #pragma unroll
             for (int k = 0; k < K; ++k) {
-                const T sh = upper_of(mX, k, nb, gh);
+                T sh = upper_of(mX, k, nb, gh);
+                if (mhi == IMH_BC_FADE && i0 + k < N)  // fade_shift.m: both ends blended,
+                                                       // v1 = a[1], vN = a[N-1]
+                    sh = fade_blend(sh, i0 + k, N, T(s_vend[px][1]), T(s_vend[px][5]), a.fade, f);
                 const T vm = fmin(sh - mX[k], T(0));  // MATLAB min(.., 0) skips NaN
                 P[k] = P[k] + vm * (T(a.visc_lin) * CS[k] + T(a.visc_quad) * vm / Q[0][k]);
             }

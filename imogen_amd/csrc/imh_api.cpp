@@ -1082,11 +1082,12 @@ int imh_set_radiation(imh_ctx* c, int type, double exponent, double initial_maxi
 int imh_set_viscosity(imh_ctx* c, int type, double linear, double quadratic) {
     if (!c) return fail(IMH_EINVAL, "null context");
     if (type < 0 || type > 2) return fail(IMH_EINVAL, "viscosity type %d", type);
-    if (type)
+    if (type == 2)  // the fade blend reaches into every one of its shifted arrays
         for (int d = 0; d < c->ndims; ++d)
             for (int s = 0; s < 2; ++s)
                 if (c->cfg.bc[s][d] == IMH_BC_FADE)
-                    return fail(IMH_ENOTSUP, "viscosity with a fade boundary is not supported");
+                    return fail(IMH_ENOTSUP, "the Caramana-Shashkov-Whalen viscosity with a "
+                                "fade boundary is not supported");
     hipError_t e = hipStreamSynchronize(c->st);
     if (e != hipSuccess) return hip_fail(e, "sync");
     drop_graphs(c);

@@ -163,8 +163,8 @@ int imh_get_radiation_strength(imh_ctx *ctx, double *strength);
  * 395-400): type 0 'empty', 1 'neumann_richtmyer' (ArtificialViscosity.m:73-77), 2
  * 'caramana_shashkov_whalen' (ArtificialViscosity.m:80-148, 155-190) with
  * linearViscousStrength and quadraticViscousStrength; its term is added to the pressure
- * of both wFluidFlux calls of every fluid sweep (wFluidFlux.m:24).  Fade boundaries
- * return IMH_ENOTSUP. */
+ * of both wFluidFlux calls of every fluid sweep (wFluidFlux.m:24).  Type 2 with fade
+ * boundaries returns IMH_ENOTSUP. */
 int imh_set_viscosity(imh_ctx *ctx, int type, double linear, double quadratic);
 
 /* ---- time state -------------------------------------------------------------------- */

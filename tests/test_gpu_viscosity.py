@@ -7,7 +7,7 @@ Graph replay equals eager iterations, a checkpoint resumes bit for bit, FP32-bul
 close to FP64.  The Caramana-Shashkov-Whalen form in lockstep too: a Sod tube, 2-D MHD
 with circ / const / mirror / flip / zero ends, 3-D MHD with gravity and radiation, and a
 1-D problem on every candidate tile, and with transparent and wall ends (the trackers of
-every shifted object); fade ends are refused."""
+every shifted object); with fade ends it is refused (Neumann-Richtmyer runs with them)."""
 import numpy as np
 import pytest
 
@@ -201,11 +201,16 @@ def test_csw_transparent_checkpoint_resume():
     c.close()
 
 
+def test_neumann_richtmyer_fade_lockstep():
+    """Fade ends: the momentum shift is blended toward its end values (fade_shift.m)."""
+    ini, st = box((48, 40, 1), {"x": "fade", "y": ["const", "fade"], "z": "circ"})
+    lockstep(ini, st, 6, FLUID_FIELDS, visc=(0.3, 0.6))
+
+
 def test_unsupported_viscosity_is_refused():
     ini, st = box((32, 24, 1), {"x": "fade", "y": "circ", "z": "circ"})
     ini = dict(ini, bcInfinity=8)
     run = make_gpu(ini, st)
-    for kind in ("neumann_richtmyer", "caramana_shashkov_whalen"):
-        with pytest.raises(ImogenError):
-            run.set_viscosity(kind, 0.1, 0.2)
+    with pytest.raises(ImogenError):
+        run.set_viscosity("caramana_shashkov_whalen", 0.1, 0.2)
     run.close()

@@ -264,8 +264,9 @@ def test_viscosity_acts_in_compression_and_sod_still_converges():
 def test_viscosity_rejects_fade_and_unknown_types():
     o = O.Oracle(O.make_config((32, 24, 1), (0.1, 0.1, 0.1), bc="fade", bc_infinity=8))
     o.set_state(*uniform((32, 24, 1)))
+    o.set_viscosity("neumann_richtmyer", 0.1, 0.2)
     with pytest.raises(RuntimeError):
-        o.set_viscosity("neumann_richtmyer", 0.1, 0.2)
+        o.set_viscosity("caramana_shashkov_whalen", 0.1, 0.2)
     o = O.Oracle(O.make_config((32, 24, 1), (0.1, 0.1, 0.1)))
     o.set_state(*uniform((32, 24, 1)))
     with pytest.raises(RuntimeError):
