@@ -92,6 +92,75 @@ __device__ __noinline__ void fluid_init_face(double* edges, const long long* eof
     }
 }
 
+// Caramana-Shashkov-Whalen past a transparent or wall end (side `up` of a pencil): the
+// side's 15 shift calls in the .m text's order (ArtificialViscosity.m:80-148, 155-190),
+// each a ghost of its object (rows 0 mass, 1..3 momentum, 4 energy, 5..7 cell-centred B;
+// ends e[row * 6 + 0..5] = a[0..2], a[N-3..N-1]) with that object's tracker *tp[row];
+// out = massB/C, sndB/C, then v0 (two cells), v1 on the lower side or v2, v3 (two cells)
+// on the upper.  Out of line: it runs on one lane per pencil.
+__device__ __noinline__ void csw_side_ghosts(int mode, int up, int d, int N, int active_lin,
+                                             int f, double G, double* const tp[8],
+                                             const double* e, double* out) {
+    double tv[8];
+    for (int r = 0; r < 8; ++r) tv[r] = *tp[r];
+    auto call = [&](int r, int m, double& g0, double& g1) {
+        GhostSpec sp;
+        sp.mode = mode;
+        sp.upper = up;
+        sp.flip_sign = (r == 1 + d) || (r == 5 + d);
+        sp.zero_ghost = 0;
+        sp.active_linear = active_lin;
+        sp.bc_infinity = f;
+        const double* x = e + r * 6;
+        const Ends en = {x[0], x[1], x[2], x[3], x[4], x[5]};
+        if (m == 2)
+            ghost_values<2>(sp, en, N, 2, &tv[r], g0, g1);
+        else
+            ghost_values<1>(sp, en, N, 1, &tv[r], g0, g1);
+    };
+    double g0, g1, u0, u1;
+    call(0, 1, g0, g1);  // massB / massC
+    const double mEnd = g0;
+    double t1, t2, t3, tm;  // cswSoundSpeed, this side
+    call(1, 1, t1, g1);
+    call(2, 1, t2, g1);
+    call(3, 1, t3, g1);
+    call(0, 1, tm, g1);
+    const double vel2 = t1 * t1 + t2 * t2 + t3 * t3 / (tm * tm);
+    double b1, b2, b3;
+    call(5, 1, b1, g1);
+    call(6, 1, b2, g1);
+    call(7, 1, b3, g1);
+    const double mag2 = b1 * b1 + b2 * b2 + b3 * b3;
+    double te, tm2, tm3;
+    call(4, 1, te, g1);
+    call(0, 1, tm2, g1);
+    double sp = (G - 1.0) * (te - 0.5 * tm2 * vel2);
+    sp = sp - (G - 1.0) * 0.5 * mag2;
+    call(0, 1, tm3, g1);
+    out[0] = mEnd;
+    out[1] = sqrt(fabs((G * sp + 2.0 * mag2) / tm3));
+    if (!up) {
+        call(1 + d, 2, g0, g1);  // v0 = mom.shift(-2) ./ mass.shift(-2)
+        call(0, 2, u0, u1);
+        out[2] = g0 / u0;
+        out[3] = g1 / u1;
+        call(1 + d, 1, g0, g1);  // v1
+        call(0, 1, u0, u1);
+        out[4] = g0 / u0;
+    } else {
+        call(1 + d, 1, g0, g1);  // v2
+        call(0, 1, u0, u1);
+        out[2] = g0 / u0;
+        call(1 + d, 2, g0, g1);  // v3 = mom.shift(2) ./ mass.shift(2)
+        call(0, 2, u0, u1);
+        out[3] = g0 / u0;
+        out[4] = g1 / u1;
+    }
+    if (mode == IMH_BC_TRANS)
+        for (int r = 0; r < 8; ++r) *tp[r] = tv[r];
+}
+
 // Max over each pencil of the block (NaN-ignoring): wave shuffles, then one LDS pass.
 // Takes the tile geometry by value: a reference to the argument block here would keep
 // the block's private copy in scratch memory.
@@ -527,6 +596,9 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
     // of each side in the .m text's order and leaves the five values the cells read there
     // (massB/C, sndB/C, v0, v1 / v2, v3), staged in the exchange rows s_x (unused here).
     auto csw = [&](const T (&Q)[5][K], int obj) __attribute__((always_inline)) {
+      // the body exists in the general form only: left in the common form it changes that
+      // form's register allocation although it is never called there
+      if constexpr (GEN) {
         const T G = T(g);
         T Sr[K], Mr[K], Vr[K];
 #pragma unroll
@@ -579,76 +651,16 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
         __syncthreads();
         if (trk_any && tid < PX && pen0 + tid < a.npencil) {
             const int p = tid;
-            const double* e = xe + p * XE_ENDS;
-            double* out = xe + PXMAX * XE_ENDS + p * XE_OUT;
-            const double Gd = g;
             for (int up = 0; up < 2; ++up) {
                 const int mode = up ? mhi : mlo;
                 if (mode != IMH_BC_TRANS && mode != IMH_BC_WALL) continue;
-                // the side's trackers: rows 0..4 obj + v, rows 5..7 OBJ_CMAG + b
-                double tv[8];
                 double* tp[8];
                 for (int r = 0; r < 8; ++r) {
                     const int o = r < 5 ? obj + r : OBJ_CMAG + (r - 5);
                     tp[r] = a.edges + a.eoff[(o * 2 + up) * 3 + d] + pen0 + p;
-                    tv[r] = *tp[r];
                 }
-                // one shift call of row r by m cells on this side: ghost(s) g0 (, g1)
-                auto call = [&](int r, int m, double& g0, double& g1) {
-                    GhostSpec sp;
-                    sp.mode = mode;
-                    sp.upper = up;
-                    sp.flip_sign = (r == 1 + d) || (r == 5 + d);
-                    sp.zero_ghost = 0;
-                    sp.active_linear = a.active_lin;
-                    sp.bc_infinity = f;
-                    const double* x = e + r * 6;
-                    const Ends en = {x[0], x[1], x[2], x[3], x[4], x[5]};
-                    ghost_values(sp, en, N, m, &tv[r], g0, g1);
-                };
-                double g0, g1, u0, u1;
-                call(0, 1, g0, g1);                       // massB / massC
-                const double mEnd = g0;
-                double t1, t2, t3, tm;                    // cswSoundSpeed, this side
-                call(1, 1, t1, g1);
-                call(2, 1, t2, g1);
-                call(3, 1, t3, g1);
-                call(0, 1, tm, g1);
-                const double vel2 = t1 * t1 + t2 * t2 + t3 * t3 / (tm * tm);
-                double b1, b2, b3;
-                call(5, 1, b1, g1);
-                call(6, 1, b2, g1);
-                call(7, 1, b3, g1);
-                const double mag2 = b1 * b1 + b2 * b2 + b3 * b3;
-                double te, tm2, tm3;
-                call(4, 1, te, g1);
-                call(0, 1, tm2, g1);
-                double sp = (Gd - 1.0) * (te - 0.5 * tm2 * vel2);
-                sp = sp - (Gd - 1.0) * 0.5 * mag2;
-                call(0, 1, tm3, g1);
-                const double snd = sqrt(fabs((Gd * sp + 2.0 * mag2) / tm3));
-                double* o = out + up * 5;
-                o[0] = mEnd;
-                o[1] = snd;
-                if (!up) {
-                    call(1 + d, 2, g0, g1);               // v0 = mom.shift(-2) ./ mass.shift(-2)
-                    call(0, 2, u0, u1);
-                    o[2] = g0 / u0;
-                    o[3] = g1 / u1;
-                    call(1 + d, 1, g0, g1);               // v1
-                    call(0, 1, u0, u1);
-                    o[4] = g0 / u0;
-                } else {
-                    call(1 + d, 1, g0, g1);               // v2
-                    call(0, 1, u0, u1);
-                    o[2] = g0 / u0;
-                    call(1 + d, 2, g0, g1);               // v3 = mom.shift(2) ./ mass.shift(2)
-                    call(0, 2, u0, u1);
-                    o[3] = g0 / u0;
-                    o[4] = g1 / u1;
-                }
-                for (int r = 0; r < 8; ++r)
-                    if (mode == IMH_BC_TRANS) *tp[r] = tv[r];
+                csw_side_ghosts(mode, up, d, N, a.active_lin, f, g, tp, xe + p * XE_ENDS,
+                                xe + PXMAX * XE_ENDS + p * XE_OUT + up * 5);
             }
         }
         if (trk_any) __syncthreads();
@@ -708,6 +720,7 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
         }
         // the group loop that follows writes the exchange rows
         if (trk_any) __syncthreads();
+      }
     };
     auto viscosity = [&](const T (&Q)[5][K], int obj, bool first) __attribute__((always_inline)) {
         if constexpr (GEN) {

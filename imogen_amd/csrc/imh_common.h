@@ -186,8 +186,12 @@ __host__ __device__ inline double end_at(const Ends& e, int i, int N) {
 
 // One ghost value at position t (0 <= t < m).  `tracker` is nudged by the caller once
 // per shift call (see ghost_values); here ev is the nudged edge (or 0).
-__host__ __device__ inline double ghost_at(const GhostSpec& s, const Ends& e, int N, int m, int t,
+// M > 0 fixes the shift length at compile time: a separate instantiation, so callers of
+// the run-time form (all with m = 1 in a translation unit) keep their code.
+template <int M = 0>
+__host__ __device__ inline double ghost_at(const GhostSpec& s, const Ends& e, int N, int m_, int t,
                                            double stored, double ev) {
+    const int m = M > 0 ? M : m_;
     const int up = s.upper;
     switch (s.mode) {
     case IMH_BC_CIRC: {
@@ -241,14 +245,16 @@ __host__ __device__ inline double ghost_at(const GhostSpec& s, const Ends& e, in
 
 // The m ghost values of one shift call; nudges the transparent tracker once
 // (Edges.getEdge is called once per transparent_shift call).
-__host__ __device__ inline void ghost_values(const GhostSpec& s, const Ends& e, int N, int m, double* tracker, double& g0,
+template <int M = 0>
+__host__ __device__ inline void ghost_values(const GhostSpec& s, const Ends& e, int N, int m_, double* tracker, double& g0,
                   double& g1) {
+    const int m = M > 0 ? M : m_;
     double ev = 0.0;
     if (s.mode == IMH_BC_TRANS && s.active_linear)
         ev = edge_nudge(*tracker, s.upper ? end_at(e, N - 1, N) : end_at(e, 0, N));
     double stored = (s.mode == IMH_BC_WALL) ? *tracker : 0.0;
-    g0 = ghost_at(s, e, N, m, 0, stored, ev);
-    g1 = (m > 1) ? ghost_at(s, e, N, m, 1, stored, ev) : 0.0;
+    g0 = ghost_at<M>(s, e, N, m, 0, stored, ev);
+    g1 = (m > 1) ? ghost_at<M>(s, e, N, m, 1, stored, ev) : 0.0;
 }
 
 // Fade blend of a constant-shifted value r at position i (fade_shift.m:31-45).

@@ -60,7 +60,8 @@ __global__ void __launch_bounds__(256) source_ghost_kernel(SourceArgs a) {
     for (int k = 0; k < 6; ++k) out[k] = g[k];
 }
 
-template <typename T>
+// FADE: some boundary of the potential is a fade one (the blend is compiled only then)
+template <typename T, bool FADE>
 __global__ void __launch_bounds__(256) source_cell_kernel(SourceArgs a, T* mass, T* mx, T* my,
                                                           T* mz, T* ener, const T* bx,
                                                           const T* by, const T* bz) {
@@ -93,7 +94,7 @@ __global__ void __launch_bounds__(256) source_cell_kernel(SourceArgs a, T* mass,
                 s1 = p < N - 1 ? a.phi[c + st] : g[2];
                 b1 = p >= 1 ? a.phi[c - st] : g[3];
                 b2 = p >= 2 ? a.phi[c - 2 * st] : g[4 + p];
-                if (a.modes[0][d] == IMH_BC_FADE || a.modes[1][d] == IMH_BC_FADE) {
+                if (FADE && (a.modes[0][d] == IMH_BC_FADE || a.modes[1][d] == IMH_BC_FADE)) {
                     // fade_shift.m: the constant shift, then both ends of the shifted array
                     // blended toward its end values over bcInfinity cells
                     const double* row = a.phi + (c - (long long)p * st);
@@ -164,14 +165,29 @@ hipError_t launch_source(const SourceArgs& a, void* const f[IMH_NFIELDS], bool f
                            0, st, a);
     const long long ncell = (long long)a.n[0] * a.n[1] * a.n[2];
     const dim3 grid(unsigned((ncell + 255) / 256));
-    if (fp64)
-        hipLaunchKernelGGL(source_cell_kernel<double>, grid, dim3(256), 0, st, a,
+    bool fade = false;
+    for (int d = 0; d < a.ndims; ++d)
+        fade = fade || a.modes[0][d] == IMH_BC_FADE || a.modes[1][d] == IMH_BC_FADE;
+    if (fp64 && fade)
+        hipLaunchKernelGGL((source_cell_kernel<double, true>), grid, dim3(256), 0, st, a,
                            (double*)f[IMH_F_MASS], (double*)f[IMH_F_MOMX], (double*)f[IMH_F_MOMY],
                            (double*)f[IMH_F_MOMZ], (double*)f[IMH_F_ENER],
                            (const double*)f[IMH_F_CMAGX], (const double*)f[IMH_F_CMAGY],
                            (const double*)f[IMH_F_CMAGZ]);
+    else if (fp64)
+        hipLaunchKernelGGL((source_cell_kernel<double, false>), grid, dim3(256), 0, st, a,
+                           (double*)f[IMH_F_MASS], (double*)f[IMH_F_MOMX], (double*)f[IMH_F_MOMY],
+                           (double*)f[IMH_F_MOMZ], (double*)f[IMH_F_ENER],
+                           (const double*)f[IMH_F_CMAGX], (const double*)f[IMH_F_CMAGY],
+                           (const double*)f[IMH_F_CMAGZ]);
+    else if (fade)
+        hipLaunchKernelGGL((source_cell_kernel<float, true>), grid, dim3(256), 0, st, a,
+                           (float*)f[IMH_F_MASS], (float*)f[IMH_F_MOMX], (float*)f[IMH_F_MOMY],
+                           (float*)f[IMH_F_MOMZ], (float*)f[IMH_F_ENER],
+                           (const float*)f[IMH_F_CMAGX], (const float*)f[IMH_F_CMAGY],
+                           (const float*)f[IMH_F_CMAGZ]);
     else
-        hipLaunchKernelGGL(source_cell_kernel<float>, grid, dim3(256), 0, st, a,
+        hipLaunchKernelGGL((source_cell_kernel<float, false>), grid, dim3(256), 0, st, a,
                            (float*)f[IMH_F_MASS], (float*)f[IMH_F_MOMX], (float*)f[IMH_F_MOMY],
                            (float*)f[IMH_F_MOMZ], (float*)f[IMH_F_ENER],
                            (const float*)f[IMH_F_CMAGX], (const float*)f[IMH_F_CMAGY],
