@@ -47,7 +47,8 @@
  *                  zero strengths equals the inviscid run bit for bit.
  *   CFL            closed form on a uniform state.
  * Parity unpinned (no MATLAB anywhere): MATLAB's own perms row order (two orders are
- * offered, see orc_config.perms_order) and MATLAB pchip's last-bit rounding.
+ * offered, see orc_config.perms_order), MATLAB pchip's last-bit rounding, and the last bit
+ * of MATLAB's .^ in the radiation sink (C pow here).
  */
 #ifndef IMOGEN_ORACLE_H
 #define IMOGEN_ORACLE_H
