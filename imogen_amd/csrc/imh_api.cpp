@@ -996,12 +996,14 @@ int imh_set_gravity(imh_ctx* c, const double* potential, size_t n, double mass_t
         for (int d = 0; d < c->ndims; ++d)
             for (int s = 0; s < 2; ++s) {
                 const int m = c->cfg.bc[s][d];
-                // a dimension of 1 or 2 cells (a 1-D grid's y): only shifts that stay in
-                // the pencil (the reference's other shift rules index outside it)
-                if (c->n[d] < 3 && !(m == IMH_BC_CIRC || m == IMH_BC_CONST ||
-                                     m == IMH_BC_FLIP || m == IMH_BC_ZERO))
-                    return fail(IMH_ENOTSUP, "gravity along a dimension of < 3 cells needs "
-                                "circ, const, flip or zero boundaries there");
+                // the 5-point derivative shifts by two cells: on a one-cell dimension (a
+                // 1-D grid's y) every shift rule of the reference indexes element 0 and
+                // MATLAB errors; on two cells only circ / const / flip / zero stay inside
+                if (c->n[d] == 1 || (c->n[d] == 2 && !(m == IMH_BC_CIRC || m == IMH_BC_CONST ||
+                                                       m == IMH_BC_FLIP || m == IMH_BC_ZERO)))
+                    return fail(IMH_ENOTSUP, "gravity along dimension %d of %d cells: the "
+                                "reference's shift by two cells indexes outside the array",
+                                d + 1, c->n[d]);
             }
     }
     hipError_t e = hipStreamSynchronize(c->st);

@@ -146,8 +146,8 @@ int imh_run(imh_ctx *ctx, int max_iterations, int *done);
  * matching energy change (source.m:16-25).  imh_set_gravity copies the potential
  * (column-major, n cells, FP64; NULL turns gravity off) and sets run.fluid.MASS_THRESHOLD
  * (initialize.m:72-77); the potential's edge stores start at zero, as a new GravityArray's
- * do.  Dimensions of fewer than 3 cells (a 1-D grid's y) need circ, const, flip or zero
- * boundaries there.  The Poisson solvers stay out (a solved potential is uploaded
+ * do.  A one-cell dimension below ndims (a 1-D grid's y) is refused, as the reference's
+ * shift by two cells errors there; two-cell dimensions need circ, const, flip or zero.  The Poisson solvers stay out (a solved potential is uploaded
  * with imh_set_gravity).
  * Radiation: run.fluid.radiation (Radiation.m): type 0 'empty', 1 'optically_thin'
  * with exponent and initialMaximum (initialize.m:403-409); imh_set_radiation computes

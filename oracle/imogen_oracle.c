@@ -1007,11 +1007,12 @@ int orc_set_gravity(orc_sim *s, const double *potential, double mass_threshold) 
     if (!potential) return 0;
     for (int d = 0; d < s->ndims; ++d)
         for (int side = 0; side < 2; ++side) {
-            /* fewer than 3 cells: only the modes whose shifts are index maps clamped or
-             * wrapped into the pencil (the others index outside it, as the .m files do) */
+            /* the 5-point derivative shifts by two cells: on a one-cell dimension every
+             * *_shift.m indexes element 0 (MATLAB errors); on two cells only the index
+             * maps of circ / const / flip / zero stay inside the array */
             const int m = s->cfg.bc[side][d];
-            if (s->n[d] < 3 && !(m == ORC_BC_CIRC || m == ORC_BC_CONST || m == ORC_BC_FLIP ||
-                                 m == ORC_BC_ZERO)) {
+            if (s->n[d] == 1 || (s->n[d] == 2 && !(m == ORC_BC_CIRC || m == ORC_BC_CONST ||
+                                                  m == ORC_BC_FLIP || m == ORC_BC_ZERO))) {
                 snprintf(s->err, sizeof(s->err),
                          "gravity along dim %d of < 3 cells needs circ/const/flip/zero", d + 1);
                 return -1;

@@ -327,6 +327,10 @@ class Sim:
         self.time = 0.0
         self.dtime = 0.0
         self.iteration = 1
+        self.grav = None          # GravityArray with the fixed potential, or None
+        self.threshold = 0.0      # run.fluid.MASS_THRESHOLD
+        self.rad = None           # (exponent, strength) of the optically thin sink
+        self.visc = None          # ("nr" | "csw", linear, quadratic)
 
     def set_state(self, mass, mom, ener, mag):
         comp = [0, 1, 2, 3, 0]
@@ -377,6 +381,9 @@ class Sim:
                 b2 = b2 + self.cmag[i].a * self.cmag[i].a
         r = (g - 1.0) * (ener - 0.5 * rho * v2)
         aux = None
+        if mode == "gas":
+            r = r - (g - 1.0) * 0.5 * b2
+            return np.where(r < 0, 0.0, r), None
         if mode == "totsnd":
             aux = np.sqrt(np.abs((g * (r - (g - 1.0) * 0.5 * b2) + 2.0 * b2) / rho))
             r = r + 0.5 * (2.0 - g) * b2
@@ -386,13 +393,49 @@ class Sim:
         r = np.where(r < 0, 0.0, r)
         return r, aux
 
-    def wfluidflux(self, q, freeze, X):
+    def viscosity(self, objs, snd, X):
+        """ArtificialViscosity.m: Neumann-Richtmyer (:73-77) or Caramana-Shashkov-Whalen
+        (:80-148, cswSoundSpeed :155-190) on the objects of the five fields."""
+        kind, c1, c2 = self.visc
+        X1 = X + 1
+        mass, mom, ener = objs[0], objs[1:4], objs[4]
+        if kind == "nr":
+            vm = np.fmin(mom[X].shift(X1, 1) - mom[X].a, 0)
+            return vm * (c1 * snd + c2 * vm / mass.a)
+        g = self.gamma
+        massB = mass.shift(X1, -1)
+        massC = mass.shift(X1, 1)
+        rho = 2 * massC * massB / (massC + massB)
+        sides = []
+        for n in (1, -1):
+            v2 = (mom[0].shift(X1, n) ** 2 + mom[1].shift(X1, n) ** 2
+                  + mom[2].shift(X1, n) ** 2 / mass.shift(X1, n) ** 2)
+            b2 = (self.cmag[0].shift(X1, n) ** 2 + self.cmag[1].shift(X1, n) ** 2
+                  + self.cmag[2].shift(X1, n) ** 2)
+            sp = (g - 1.0) * (ener.shift(X1, n) - 0.5 * mass.shift(X1, n) * v2)
+            sp = sp - (g - 1.0) * 0.5 * b2
+            sides.append(np.sqrt(np.abs((g * sp + 2.0 * b2) / mass.shift(X1, n))))
+        cs = np.fmax(np.fmin(sides[1], sides[0]), 0)
+        v0 = mom[X].shift(X1, -2) / mass.shift(X1, -2)
+        v1 = mom[X].shift(X1, -1) / mass.shift(X1, -1)
+        v2 = mom[X].shift(X1, 1) / mass.shift(X1, 1)
+        v3 = mom[X].shift(X1, 2) / mass.shift(X1, 2)
+        dv = v2 - v1
+        rL = (v3 - v2) / (2 * dv)
+        rR = (v1 - v0) / (2 * dv)
+        lim = np.fmax(0, np.fmin(np.fmin(0.5 * (rL + rR), np.fmin(2 * rL, 2 * rR)), 1))
+        return ((dv < 0) * rho * (c2 * (g + 1) * np.abs(dv) / 4
+                                  + np.sqrt(c2 ** 2 * ((g + 1) / 4) ** 2 * dv ** 2
+                                            + c1 ** 2 * cs ** 2))
+                * (1 - lim) * np.abs(dv))
+
+    def wfluidflux(self, q, freeze, X, objs=None):
         rho, mom, E = q[0], q[1:4], q[4]
         press, snd = self.pressure("totsnd", rho, mom, E)
         vel = mom[X] / rho
         spd = np.fmax.reduce(np.abs(vel) + snd, axis=X, keepdims=True)
         freeze.assign(np.broadcast_to(spd, rho.shape).copy())
-        press = press + 0
+        press = press + (self.viscosity(objs, snd, X) if self.visc else 0)
         C = freeze.a
         bc = [c.a for c in self.cmag]
         w = [None] * 5
@@ -408,7 +451,7 @@ class Sim:
         X = X1 - 1
         ff = self.dtime / self.dgrid[X]
         q = [o.a for o in self.prim]
-        w = self.wfluidflux(q, self.freeze[X], X)
+        w = self.wfluidflux(q, self.freeze[X], X, self.prim)
         C = self.freeze[X].a
         for i in range(5):
             self.sfr[i].assign(0.5 * (q[i] + w[i]))
@@ -417,7 +460,7 @@ class Sim:
                                                         + self.sfl[i].a - self.sfl[i].shift(X1, 1)))
         self.store[0].assign(np.fmax(self.store[0].a, self.minmass))
         qs = [o.a for o in self.store]
-        w = self.wfluidflux(qs, self.freeze_tvd[X], X)
+        w = self.wfluidflux(qs, self.freeze_tvd[X], X, self.store)
         C = self.freeze_tvd[X].a
         lim = LIMITERS[self.limiter[X]]
         for i in range(5):
@@ -496,10 +539,52 @@ class Sim:
                     self.magnet_flux(X, mi[X])
                 self.relaxing_fluid(X)
 
+    def set_gravity(self, potential, threshold):
+        """GravityArray: zeros first (its edge stores), then the fixed potential."""
+        self.grav = Obj(self, "init", 0)
+        self.grav.assign(np.zeros(self.shape))
+        self.grav.assign(potential.copy())
+        self.threshold = threshold
+
+    def set_radiation(self, exponent, initial_maximum):
+        """Radiation.initialize with the constructor's strength of 1."""
+        rho = self.prim[0].a
+        mom = [o.a for o in self.prim[1:4]]
+        pg, _ = self.pressure("gas", rho, mom, self.prim[4].a)
+        unscaled = 1 * rho ** (2 - exponent) * pg ** exponent
+        ke = 0.5 * (mom[0] * mom[0] + mom[1] * mom[1] + mom[2] * mom[2]) / rho
+        me = 0.5 * (self.mag[0].a * self.mag[0].a + self.mag[1].a * self.mag[1].a
+                    + self.mag[2].a * self.mag[2].a)
+        self.rad = (exponent, (initial_maximum / 100) * np.nanmin((self.prim[4].a - ke - me) / unscaled))
+
+    def source(self):
+        """source.m: gravity from the fixed potential, then radiation."""
+        dt = self.dtime
+        rho = self.prim[0].a
+        if self.grav is not None:
+            thr = rho * (rho > self.threshold) if self.threshold > 0 else rho
+            es = np.zeros(self.shape)
+            for i in range(3):
+                D1 = i + 1
+                d5 = (-self.grav.shift(D1, 2) + 8 * self.grav.shift(D1, 1)
+                      - 8 * self.grav.shift(D1, -1) + self.grav.shift(D1, -2)) / (12 * self.dgrid[i])
+                ms = dt * thr * d5
+                es = es + ms * self.prim[i + 1].a / rho
+                self.prim[i + 1].a = self.prim[i + 1].a - ms
+            self.prim[4].a = self.prim[4].a - es
+        if self.rad is not None:
+            a, strength = self.rad
+            pg, _ = self.pressure("gas", rho, [o.a for o in self.prim[1:4]], self.prim[4].a)
+            self.prim[4].a = self.prim[4].a - dt * (strength * rho ** (2 - a) * pg ** a)
+        else:
+            self.prim[4].a = self.prim[4].a - dt * 0
+
     def iterate(self):
         self.time_update()
         self.flux(1)
+        self.source()
         self.flux(-1)
+        self.source()
         self.iteration += 1
 
 
@@ -542,6 +627,33 @@ CASES = [
 ]
 
 
+# source terms and viscosity: name, grid, dgrid, bc, limiters, magnet, iterations, extras
+# (gravity: potential of the cell centres with a mass threshold; radiation: exponent,
+# initialMaximum; viscosity: "nr" / "csw" with linear, quadratic strengths)
+EXTRA_CASES = [
+    ("hydro2d_grav_rad_nr", (12, 10, 1), (1 / 12, 1 / 10, 1 / 12),
+     [["trans", "wall", "circ"], ["const", "trans", "circ"]], ["vanleer", "superbee", "vanleer"],
+     False, 4, {"grav": 0.9, "rad": (0.5, 1.0), "visc": ("nr", 0.3, 0.6)}),
+    ("mhd2d_csw_trans_mirror", (11, 10, 1), (0.09, 0.1, 0.09),
+     [["trans", "mirror", "circ"], ["wall", "trans", "circ"]], ["minmod"] * 3,
+     True, 3, {"grav": 0.0, "visc": ("csw", 0.3, 0.8)}),
+    ("hydro2d_two_cell_nr_grav_fade", (24, 2, 1), (1 / 24, 1 / 24, 1 / 24),
+     [["fade", "circ", "circ"], ["trans", "circ", "circ"]], ["vanleer"] * 3,
+     False, 5, {"grav": 0.0, "visc": ("nr", 0.1, 0.25), "bc_infinity": 8}),
+    ("hydro3d_csw_grav", (8, 7, 6), (0.1, 0.1, 0.1),
+     [["trans", "circ", "mirror"], ["wall", "circ", "const"]], ["minmod"] * 3,
+     False, 2, {"grav": 0.0, "rad": (0.5, 2.0), "visc": ("csw", 0.2, 0.5)}),
+]
+
+
+def potential(grid):
+    """The fixed potential of the extra cases, (nz, ny, nx), from the cell centres."""
+    nx, ny, nz = grid
+    z, y, x = np.meshgrid((np.arange(nz) + 0.5) / nz, (np.arange(ny) + 0.5) / ny,
+                          (np.arange(nx) + 0.5) / nx, indexing="ij")
+    return 0.8 * np.sin(2 * np.pi * x) * np.cos(np.pi * y) + 0.3 * (x - 0.5) ** 2 + 0.1 * z
+
+
 def random_state(grid, seed, magnet):
     rng = np.random.default_rng(seed)
     nx, ny, nz = grid
@@ -558,12 +670,28 @@ def random_state(grid, seed, magnet):
 
 def main():
     out = {}
-    for k, (name, grid, dgrid, bc, lims, magnet, iters) in enumerate(CASES):
+    cases = [c + ({},) for c in CASES] + EXTRA_CASES
+    for k, (name, grid, dgrid, bc, lims, magnet, iters, extra) in enumerate(cases):
         mass, mom, ener, mag = random_state(grid, 100 + k, magnet)
-        sim = Sim(grid, dgrid, 5.0 / 3.0, bc, lims, magnet, 0.35 if magnet else 0.7)
+        sim = Sim(grid, dgrid, 5.0 / 3.0, bc, lims, magnet, 0.35 if magnet else 0.7,
+                  bc_infinity=extra.get("bc_infinity", 20))
         shp = sim.shape
         sim.set_state(to_ref(mass, shp), [to_ref(m, shp) for m in mom], to_ref(ener, shp),
                       [to_ref(b, shp) for b in mag])
+        keep = {}
+        if "grav" in extra:
+            phi = potential(grid)
+            sim.set_gravity(to_ref(phi, shp), extra["grav"])
+            keep.update(potential=phi, threshold=np.array(extra["grav"]))
+        if "rad" in extra:
+            sim.set_radiation(*extra["rad"])
+            keep.update(rad=np.array(extra["rad"], dtype=float))
+        if "visc" in extra:
+            sim.visc = extra["visc"]
+            keep.update(visc_kind=np.array(extra["visc"][0]),
+                        visc=np.array(extra["visc"][1:], dtype=float))
+        if "bc_infinity" in extra:
+            keep.update(bc_infinity=np.array(extra["bc_infinity"]))
         states = []
         times = []
         for _ in range(iters):
@@ -576,7 +704,7 @@ def main():
                             grid=np.array(grid), dgrid=np.array(dgrid, dtype=float),
                             bc=np.array(bc), limiter=np.array(lims), magnet=np.array(magnet),
                             mass=mass, mom=np.stack(mom), ener=ener, mag=np.stack(mag),
-                            states=np.stack(states), times=np.array(times))
+                            states=np.stack(states), times=np.array(times), **keep)
         print("wrote", name, flush=True)
     # pchip and fade vectors
     x = np.array([1.0, 2.0, 3.0, 20.0, 21.0])

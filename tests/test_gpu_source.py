@@ -98,13 +98,14 @@ def test_gravity_and_radiation_3d_mhd_lockstep():
     orc.close()
 
 
-def test_gravity_1d_lockstep():
-    """Sod with a linear potential: y has one cell (shifted, circular)."""
-    ini, st = problems.sod(512, bc="const", iter_max=10 ** 6)
+def test_gravity_two_cell_dimension_lockstep():
+    """Sod two cells deep in y (circular there) with a linear potential in x; a one-cell
+    dimension is refused as the reference's shift by two errors on it."""
+    ini, st = problems.sod(512, ny=2, bc="const", iter_max=10 ** 6)
     ini = dict(ini, bcMode={"x": "const", "y": "circ", "z": "circ"})
     orc = make_oracle(ini, st)
     run = make_gpu(ini, st)
-    phi = (0.5 * (np.arange(512) + 0.5) / 512).reshape(1, 1, 512)
+    phi = np.broadcast_to(0.5 * (np.arange(512) + 0.5) / 512, (1, 2, 512)).copy()
     orc.set_gravity(phi)
     run.set_gravity(phi)
     for it in range(20):
@@ -113,6 +114,11 @@ def test_gravity_1d_lockstep():
         compare(run, orc, FLUID_FIELDS, TOL)
     run.close()
     orc.close()
+    ini1, st1 = problems.sod(64, bc="const")
+    r1 = make_gpu(ini1, st1)
+    with pytest.raises(Exception):
+        r1.set_gravity(np.zeros((1, 1, 64)))
+    r1.close()
 
 
 def test_direct_source_call():

@@ -36,10 +36,19 @@ def test_oracle_matches_numpy_restatement(path):
     grid = [int(v) for v in z["grid"]]
     magnet = bool(z["magnet"])
     bc = [[str(m) for m in row] for row in z["bc"]]
-    cfg = O.make_config(grid, z["dgrid"], 5.0 / 3.0, 1e-5, 0.35 if magnet else 0.7, 20, bc,
+    binf = int(z["bc_infinity"]) if "bc_infinity" in z.files else 20
+    cfg = O.make_config(grid, z["dgrid"], 5.0 / 3.0, 1e-5, 0.35 if magnet else 0.7, binf, bc,
                         [str(l) for l in z["limiter"]], True, magnet, 1, 100, 1e9, 0)
     o = O.Oracle(cfg)
     o.set_state(z["mass"], list(z["mom"]), z["ener"], list(z["mag"]))
+    # source terms and viscosity of the extra cases (source.m, ArtificialViscosity.m)
+    if "potential" in z.files:
+        o.set_gravity(z["potential"], float(z["threshold"]))
+    if "rad" in z.files:
+        o.set_radiation("optically_thin", *[float(v) for v in z["rad"]])
+    if "visc_kind" in z.files:
+        kind = {"nr": "neumann_richtmyer", "csw": "caramana_shashkov_whalen"}[str(z["visc_kind"])]
+        o.set_viscosity(kind, *[float(v) for v in z["visc"]])
     for it, ref in enumerate(z["states"]):
         o.iteration()
         got = np.stack([o.get(f) for f in O.FIELDS])

@@ -149,16 +149,16 @@ def test_empty_sources_change_nothing():
         np.testing.assert_array_equal(after[f], before[f])
 
 
-def test_gravity_on_a_1d_grid():
-    """A 1-D grid (y of one cell, still shifted: ndims is 2): a linear potential gives
-    the force -dt rho g on interior cells; the y shifts return the cell itself, so the y
-    derivative is the rounding residue of the 5-point expression only."""
+def test_gravity_on_a_two_cell_dimension():
+    """A grid two cells deep in y: a linear potential in x gives the force -dt rho g on
+    interior cells; the y shifts by two wrap onto the cell itself (circ)."""
     n, dt, g = 64, 1e-3, 0.7
     x = (np.arange(n) + 0.5) / n
-    o = sim((n, 1, 1), (1.0 / n,) * 3, {"x": "const", "y": "circ"},
-            potential=(g * x).reshape(1, 1, n), state=uniform((n, 1, 1), rho=1.2), dt=dt)
+    o = sim((n, 2, 1), (1.0 / n,) * 3, {"x": "const", "y": "circ"},
+            potential=np.broadcast_to(g * x, (1, 2, n)).copy(), state=uniform((n, 2, 1), rho=1.2),
+            dt=dt)
     o.source()
-    mx = o.get("momx").ravel()
+    mx = o.get("momx")[0, 0]
     np.testing.assert_allclose(mx[2:-2], -dt * 1.2 * g, rtol=1e-9)
     assert np.max(np.abs(o.get("momy"))) < 1e-15
 
@@ -168,6 +168,10 @@ def test_gravity_short_dimension_rule_and_fade():
     o.set_state(*uniform((16, 2, 1)))
     with pytest.raises(RuntimeError):
         o.set_gravity(np.zeros((1, 2, 16)))
+    o = O.Oracle(O.make_config((16, 1, 1), (0.1, 0.1, 0.1), bc="circ"))
+    o.set_state(*uniform((16, 1, 1)))
+    with pytest.raises(RuntimeError):  # the reference errors: a shift by two of one cell
+        o.set_gravity(np.zeros((1, 1, 16)))
     # fade ends: the shifts blend toward their end values; a linear potential keeps the
     # interior force and the force changes only within bcInfinity cells of the ends
     n, dt, g = 32, 1e-3, 0.5
