@@ -43,8 +43,11 @@
  *                  mass threshold; dimensions beyond ndims; radiation strength and sink
  *                  closed forms on uniform states.
  *   viscosity      zero on expanding or uniform flow; on a linear compression the added
- *                  pressure is vm*(c1*c_f + c2*vm/rho) with vm = p(i+1) - p(i); a run with
- *                  zero strengths equals the inviscid run bit for bit.
+ *                  pressure is vm*(c1*c_f + c2*vm/rho) with vm = p(i+1) - p(i) (NR) and the
+ *                  published Caramana-Shashkov-Whalen form with limiter 1/4 (CSW); a run
+ *                  with zero strengths equals the inviscid run bit for bit.
+ *   both           golden vectors of the NumPy restatement with gravity, radiation and
+ *                  both viscosities over transparent, wall, mirror and fade ends.
  *   CFL            closed form on a uniform state.
  * Parity unpinned (no MATLAB anywhere): MATLAB's own perms row order (two orders are
  * offered, see orc_config.perms_order), MATLAB pchip's last-bit rounding, and the last bit
