@@ -527,8 +527,9 @@ def main():
     if args.trace_summary:
         return trace_summary(*args.trace_summary)
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (--gpus %d "
-              "requested); run with --gpus 1" % args.gpus, file=sys.stderr)
+        print("bench.py measures one GPU: the library runs each simulation on one device and "
+              "has no multi-GPU path (--gpus %d requested); run with --gpus 1" % args.gpus,
+              file=sys.stderr)
         return 2
     import imogen_amd._lib as L
     L.lib()  # fails loudly if the HIP library is missing
