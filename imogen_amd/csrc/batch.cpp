@@ -99,6 +99,16 @@ struct imh_batch {
     hipEvent_t join_ev[MAX_LANES] = {};
 };
 
+// Wait until nothing of the batch is in flight: the batch stream (graph replays join
+// their lane branches back into it) and the lane streams.  Every call that changes a
+// simulation's settings or state, or drops the captured graphs, waits here first.
+static int batch_quiesce(imh_batch* b) {
+    hipError_t e = b->st ? hipStreamSynchronize(b->st) : hipSuccess;
+    for (int l = 0; l < MAX_LANES && e == hipSuccess; ++l)
+        if (b->lane_st[l]) e = hipStreamSynchronize(b->lane_st[l]);
+    return e == hipSuccess ? IMH_OK : hip_fail(e, "batch sync");
+}
+
 static void free_plans(imh_batch* b) {
     for (int p = 0; p < 6; ++p) {
         if (b->graph[p]) hipGraphExecDestroy(b->graph[p]);
@@ -630,8 +640,9 @@ int imh_batch_destroy(imh_batch* b) {
 int imh_batch_upload_state(imh_batch* b, int sim, const double* mass, const double* const mom[3],
                            const double* ener, const double* const mag[3], size_t n) {
     if (!b || sim < 0 || sim >= b->n) return fail(IMH_EINVAL, "bad simulation index");
-    hipStreamSynchronize(b->st);
-    int rc = imh_upload_state(b->ctx[sim], mass, mom, ener, mag, n);
+    int rc = batch_quiesce(b);
+    if (rc) return rc;
+    rc = imh_upload_state(b->ctx[sim], mass, mom, ener, mag, n);
     free_plans(b);
     return rc;
 }
@@ -718,20 +729,25 @@ int imh_batch_get_time(imh_batch* b, int sim, double* time, double* dtime, int* 
 
 int imh_batch_set_fluid_tile(imh_batch* b, int sim, int X, int bs, int kmax, int px) {
     if (!b || sim < 0 || sim >= b->n) return fail(IMH_EINVAL, "bad simulation index");
-    hipError_t e = hipStreamSynchronize(b->st);
-    if (e != hipSuccess) return hip_fail(e, "batch sync");
-    int rc = imh_set_fluid_tile(b->ctx[sim], X, bs, kmax, px);
+    int rc = batch_quiesce(b);
+    if (rc) return rc;
+    rc = imh_set_fluid_tile(b->ctx[sim], X, bs, kmax, px);
     if (rc) return rc;
     free_plans(b);  // argument blocks and plans are rebuilt before the next step
     return IMH_OK;
 }
 
-// Source terms and viscosity of one simulation of the batch (the single-context calls);
-// the plans are rebuilt before the next step.
+// Source terms and viscosity of one simulation of the batch (the single-context calls,
+// after the batch's work in flight has finished); the plans are rebuilt before the next
+// step.
 int imh_batch_set_gravity(imh_batch* b, int sim, const double* potential, size_t n,
                           double mass_threshold) {
     if (!b || sim < 0 || sim >= b->n) return fail(IMH_EINVAL, "bad simulation index");
-    int rc = imh_set_gravity(b->ctx[sim], potential, n, mass_threshold);
+    // the setters write device memory that in-flight source and sweep kernels read, and
+    // free_plans destroys graphs that may still run
+    int rc = batch_quiesce(b);
+    if (rc) return rc;
+    rc = imh_set_gravity(b->ctx[sim], potential, n, mass_threshold);
     free_plans(b);
     return rc;
 }
@@ -739,14 +755,22 @@ int imh_batch_set_gravity(imh_batch* b, int sim, const double* potential, size_t
 int imh_batch_set_radiation(imh_batch* b, int sim, int type, double exponent,
                             double initial_maximum) {
     if (!b || sim < 0 || sim >= b->n) return fail(IMH_EINVAL, "bad simulation index");
-    int rc = imh_set_radiation(b->ctx[sim], type, exponent, initial_maximum);
+    // the setters write device memory that in-flight source and sweep kernels read, and
+    // free_plans destroys graphs that may still run
+    int rc = batch_quiesce(b);
+    if (rc) return rc;
+    rc = imh_set_radiation(b->ctx[sim], type, exponent, initial_maximum);
     free_plans(b);
     return rc;
 }
 
 int imh_batch_set_viscosity(imh_batch* b, int sim, int type, double linear, double quadratic) {
     if (!b || sim < 0 || sim >= b->n) return fail(IMH_EINVAL, "bad simulation index");
-    int rc = imh_set_viscosity(b->ctx[sim], type, linear, quadratic);
+    // the setters write device memory that in-flight source and sweep kernels read, and
+    // free_plans destroys graphs that may still run
+    int rc = batch_quiesce(b);
+    if (rc) return rc;
+    rc = imh_set_viscosity(b->ctx[sim], type, linear, quadratic);
     free_plans(b);
     return rc;
 }

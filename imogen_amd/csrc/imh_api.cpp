@@ -66,7 +66,7 @@ int validate_config(const imh_config* c) {
     if (c->grid[2] > 1 && (c->grid[0] == 1 || c->grid[1] == 1))
         return fail(IMH_ENOTSUP,
                     "a grid with nz > 1 and a singleton x or y is re-ordered by squeeze() "
-                    "in the reference (FluidArray.m:174)");
+                    "in the reference (FluidArray.m:48)");
     for (int d = 0; d < 3; ++d)
         if (c->dgrid[d] <= 0) return fail(IMH_EINVAL, "dgrid[%d] must be > 0", d);
     if (c->precision != IMH_FP64 && c->precision != IMH_FP32_BULK)
@@ -77,7 +77,7 @@ int validate_config(const imh_config* c) {
             int m = c->bc[s][d];
             if (m < IMH_BC_CIRC || m > IMH_BC_ZERO)
                 return fail(IMH_EINVAL, "Imogen:UnknownType: Unknown BC mode %d", m);
-            if (d >= ndims) continue;  // every dim <= ndims is shifted (MagnetArray.m:131)
+            if (d >= ndims) continue;  // every dim <= ndims is shifted (MagnetArray.m:36-37,103)
             if (m == IMH_BC_FADE &&
                 (c->bc_infinity < 4 || c->bc_infinity > MAX_FADE || c->grid[d] < c->bc_infinity))
                 return fail(IMH_ENOTSUP,
@@ -605,7 +605,7 @@ extern "C" int imh_upload_state(imh_ctx* c, const double* mass, const double* co
         if (c->graph[i]) hipGraphExecDestroy(c->graph[i]);
         c->graph[i] = nullptr;
     }
-    // cellMag from updateCellCentered in the MagnetArray constructor (MagnetArray.m:197)
+    // cellMag from updateCellCentered in the MagnetArray constructor (MagnetArray.m:102-103)
     for (int k = 0; k < 3; ++k) {
         int rc = cellmag(c, k, 1);
         if (rc) return rc;
@@ -1318,7 +1318,7 @@ extern "C" int imh_init_problem(imh_ctx* c, int kind, const double* params, int 
         if (c->graph[i]) hipGraphExecDestroy(c->graph[i]);
         c->graph[i] = nullptr;
     }
-    for (int k = 0; k < 3; ++k) {  // MagnetArray.m:197
+    for (int k = 0; k < 3; ++k) {  // MagnetArray.m:102-103
         int rc = cellmag(c, k, 1);
         if (rc) return rc;
     }
@@ -1334,7 +1334,7 @@ extern "C" int imh_init_problem(imh_ctx* c, int kind, const double* params, int 
 namespace {
 struct CkptHeader {
     char magic[8];            // "IMHCKPT\0"
-    int version;              // 1
+    int version;              // 2
     int precision;
     int grid[3];
     int nfields;
@@ -1347,8 +1347,38 @@ struct CkptHeader {
     int mhd_kernel;
     int mag_is_zero[3];
     int pad;
+    unsigned long long cfg_hash;  // run settings that shape the evolution (ckpt_cfg_hash)
 };
 const char kCkptMagic[8] = {'I', 'M', 'H', 'C', 'K', 'P', 'T', 0};
+constexpr int kCkptVersion = 2;
+
+// FNV-1a over the settings a resumed run must share with the saved one to continue bit
+// for bit: boundary modes, limiters, gamma, spacing, bcInfinity, MINMASS, CFL, the
+// fluid / magnet flags, the time-update mode and the perms order (the iteration and time
+// limits may change between the save and the resume).
+unsigned long long ckpt_cfg_hash(const imh_config& c) {
+    unsigned long long h = 1469598103934665603ull;
+    auto mix = [&](const void* p, size_t n) {
+        const unsigned char* b = (const unsigned char*)p;
+        for (size_t i = 0; i < n; ++i) {
+            h ^= b[i];
+            h *= 1099511628211ull;
+        }
+    };
+    for (int s = 0; s < 2; ++s)
+        for (int d = 0; d < 3; ++d) mix(&c.bc[s][d], sizeof(int));
+    for (int d = 0; d < 3; ++d) mix(&c.limiter[d], sizeof(int));
+    for (int d = 0; d < 3; ++d) mix(&c.dgrid[d], sizeof(double));
+    mix(&c.gamma, sizeof(double));
+    mix(&c.minmass, sizeof(double));
+    mix(&c.cfl, sizeof(double));
+    mix(&c.bc_infinity, sizeof(int));
+    mix(&c.fluid_active, sizeof(int));
+    mix(&c.magnet_active, sizeof(int));
+    mix(&c.update_mode, sizeof(int));
+    mix(&c.perms_order, sizeof(int));
+    return h;
+}
 
 size_t ckpt_bytes(const imh_ctx* c) {
     return sizeof(CkptHeader) + sizeof(SimDev) + NOBJ + (size_t)IMH_NFIELDS * c->ncell * esize(c) +
@@ -1372,8 +1402,9 @@ extern "C" int imh_checkpoint_save(imh_ctx* c, void* buf, size_t bytes) {
     CkptHeader h;
     memset(&h, 0, sizeof(h));
     memcpy(h.magic, kCkptMagic, 8);
-    h.version = 1;
+    h.version = kCkptVersion;
     h.precision = c->cfg.precision;
+    h.cfg_hash = ckpt_cfg_hash(c->cfg);
     for (int d = 0; d < 3; ++d) h.grid[d] = c->n[d];
     h.nfields = IMH_NFIELDS;
     h.nobj = NOBJ;
@@ -1403,13 +1434,17 @@ extern "C" int imh_checkpoint_load(imh_ctx* c, const void* buf, size_t bytes) {
     const char* p = (const char*)buf;
     CkptHeader h;
     memcpy(&h, p, sizeof(h));
-    if (memcmp(h.magic, kCkptMagic, 8) != 0 || h.version != 1)
+    if (memcmp(h.magic, kCkptMagic, 8) != 0 || h.version != kCkptVersion)
         return fail(IMH_EINVAL, "not a checkpoint of this library");
     if (h.precision != c->cfg.precision || h.grid[0] != c->n[0] || h.grid[1] != c->n[1] ||
         h.grid[2] != c->n[2] || h.nfields != IMH_NFIELDS || h.nobj != NOBJ ||
         h.ncell != c->ncell || h.edges_len != (long long)c->edges_len)
         return fail(IMH_EINVAL, "checkpoint of another configuration (precision, grid or "
                     "boundary layout differ)");
+    if (h.cfg_hash != ckpt_cfg_hash(c->cfg))
+        return fail(IMH_EINVAL, "checkpoint of another configuration (boundary modes, "
+                    "limiters, gamma, spacing, bcInfinity, MINMASS, CFL, fluid / magnet flags, "
+                    "update mode or perms order differ)");
     if (bytes < ckpt_bytes(c)) return fail(IMH_EINVAL, "checkpoint truncated");
     hipError_t e = hipStreamSynchronize(c->st);
     if (e != hipSuccess) return hip_fail(e, "stream sync");

@@ -17,7 +17,7 @@ namespace imh {
 // Reference array objects that carry edge state (shifting/Edges.m).  Each object of the
 // reference (FluidArray, its StorageArray/FluxArray children, MagnetArray and the
 // magnet's per-(I,X) velGrid/store.fluxR/wMag/flux arrays) keeps its own stored edges,
-// nudged on every transparent shift (Edges.m:292-313).
+// nudged on every transparent shift (Edges.m:115-140).
 // ------------------------------------------------------------------------------------
 enum ObjId : int {
     OBJ_SFR = 0,     // v(i).store.fluxR, i = 0..4 (mass, mom1..3, ener)

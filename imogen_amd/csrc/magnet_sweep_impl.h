@@ -231,8 +231,8 @@ __device__ __forceinline__ void magnet_advect_body(const MagnetArgs<T>& ap, long
         pencil_coords(nn, dX, pen, pc);
     }
     const double dt = a.sim->dt;
-    const double ff = 0.5 * dt / a.dxX;       // relaxingMagnet.m:47
-    const double ff2 = 2.0 * ff;              // relaxingMagnet.m:64
+    const double ff = 0.5 * dt / a.dxX;       // relaxingMagnet.m:15
+    const double ff2 = 2.0 * ff;              // relaxingMagnet.m:32
 
     const int mlo = mode_at(a.modes, 0, dX), mhi = mode_at(a.modes, 1, dX);
     const int oVG = OBJ_VG + 3 * dI + dX, oSFR = OBJ_MSFR + 3 * dI + dX;
@@ -507,7 +507,7 @@ __device__ __forceinline__ void magnet_advect_body(const MagnetArgs<T>& ap, long
 
 // ------------------------------------------------------------------------------------
 // Constrained transport: flux = flux - flux.shift(I,1); flux = flux.shift(X,-1);
-// mag(X) = mag(X) - dt/DGRID{I} .* flux   (relaxingMagnet.m:88-92).
+// mag(X) = mag(X) - dt/DGRID{I} .* flux   (relaxingMagnet.m:56-60).
 // ------------------------------------------------------------------------------------
 template <typename T, int KMAX>
 __device__ __forceinline__ void magnet_ct_body(const MagnetArgs<T>& ap, long long tile) {

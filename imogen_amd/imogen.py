@@ -143,6 +143,7 @@ class ImogenManager:
                                ctypes.byref(h)))
         self._ctx = h
         self._arrays = None
+        self.gravity_potential = None
 
     def close(self):
         if getattr(self, "_ctx", None):
@@ -209,9 +210,13 @@ class ImogenManager:
         array of the grid's shape; None turns gravity off) and run.fluid.MASS_THRESHOLD."""
         if potential is None:
             check(lib().imh_set_gravity(self._ctx, None, 0, float(mass_threshold)))
+            self.gravity_potential = None
             return
         arr = as_field(potential, self.config.ncell)
         check(lib().imh_set_gravity(self._ctx, dptr(arr), arr.size, float(mass_threshold)))
+        # grav.array as the slice saves write it (resultsHandler.m:81-84); the device copy
+        # is the one the source step reads
+        self.gravity_potential = arr.reshape(self.config.shape).copy()
 
     RADIATION = {"empty": 0, "optically_thin": 1}
 

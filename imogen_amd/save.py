@@ -4,7 +4,8 @@ Slices follow save/resultsHandler.m:26-113: one MATLAB v5 file per slice type,
 `<dir>/<dim>_<label>_<suffix>.mat`, holding one struct variable `sx_<label>_<suffix>` with
 the fields gamma, time (TimeManager.toStruct, time/TimeManager.m:183-191), about, ver, iter,
 mass, momX, momY, momZ, ener, magX, magY, magZ (empty without a magnetic field), grav
-(empty: no gravity here), dGrid (a 1x3 cell) and dim.  The slice of an array is
+(the slice of the fixed potential when gravity is on, else empty: resultsHandler.m:81-84),
+dGrid (a 1x3 cell) and dim.  The slice of an array is
 SaveManager.getSaveSlice (save/SaveManager.m:230-253): types 1-3 are the x, y, z lines
 through SLICEINDEX, 4-6 the xy, xz, yz planes, 7 the whole grid, each squeezed as MATLAB
 squeezes.  The suffix is 'START' for iteration 0, 'FINAL' for the end of a run, else the
@@ -78,7 +79,10 @@ def slice_struct(state, kind, slice_index, gamma, dgrid, time, dt, iteration, it
         out[key] = sl(name)
     for key, name in (("magX", "magx"), ("magY", "magy"), ("magZ", "magz")):
         out[key] = sl(name) if magnet else np.zeros((0, 0))
-    out["grav"] = np.zeros((0, 0))
+    # run.gravity.ACTIVE && numel(grav.array) > 1
+    grav = state.get("grav")
+    out["grav"] = (get_save_slice(to_xyz(grav), kind, slice_index)
+                   if grav is not None and np.size(grav) > 1 else np.zeros((0, 0)))
     cell = np.empty((1, 3), dtype=object)
     for d in range(3):
         cell[0, d] = float(dgrid[d])
@@ -120,6 +124,8 @@ def save_run_slices(run, directory, kinds=(7,), slice_index=None, final=False, a
                                     padded_number(iteration, len(str(rc.iter_max))))
     names = ["mass", "momx", "momy", "momz", "ener", "magx", "magy", "magz"]
     state = {f: run.download(f) for f in names}
+    if getattr(run, "gravity_potential", None) is not None:
+        state["grav"] = run.gravity_potential
     return save_state_slices(directory, state, kinds, slice_index, suffix,
                              gamma=rc.gamma, dgrid=rc.dgrid, time=run.time.time,
                              dt=run.time.dTime, iteration=iteration, iter_max=rc.iter_max,

@@ -108,7 +108,7 @@ int imh_destroy(imh_ctx *ctx);
 
 /* ---- state ------------------------------------------------------------------------- */
 /* Initial state; builds what the FluidArray/MagnetArray constructors build
- * (imogen.m:34-42, FluidArray.m:170-194, MagnetArray.m:135-143): the data, the edge
+ * (imogen.m:34-42, FluidArray.m:44-54, MagnetArray.m:41-46,88-104): the data, the edge
  * stores taken from the IC, MagnetArray.isZero and the initial cell-centred B.
  * n = grid[0]*grid[1]*grid[2] elements per array. */
 int imh_upload_state(imh_ctx *ctx, const double *mass, const double *const mom[3],

@@ -10,7 +10,7 @@
  * store (shifting/Edges.m).  Every `obj.shift(D,n)` of the .m text is one call of
  * do_shift(), which dispatches on the object's BC mode exactly as ImogenArray.shift
  * (classes/ImogenArray.m:156-166) and, for the transparent mode, nudges the object's
- * stored edge as Edges.getEdge does (shifting/Edges.m:292-313).  Array expressions are
+ * stored edge as Edges.getEdge does (shifting/Edges.m:115-140).  Array expressions are
  * evaluated whole-array, left to right, as MATLAB does.
  */
 #include "imogen_oracle.h"
@@ -248,7 +248,7 @@ static void obj_cleanup(obj_t *o) {
 }
 
 /* Edges.getEdge for the transparent mode: nudge the stored edge toward the current edge
- * cell by at most TOLERANCE and return it (Edges.m:296-313). */
+ * cell by at most TOLERANCE and return it (Edges.m:121-136). */
 static inline double edge_nudge(double *stored, double newedge) {
     double old = *stored;
     double diff = newedge - old;
@@ -1014,7 +1014,8 @@ int orc_set_gravity(orc_sim *s, const double *potential, double mass_threshold) 
             if (s->n[d] == 1 || (s->n[d] == 2 && !(m == ORC_BC_CIRC || m == ORC_BC_CONST ||
                                                   m == ORC_BC_FLIP || m == ORC_BC_ZERO))) {
                 snprintf(s->err, sizeof(s->err),
-                         "gravity along dim %d of < 3 cells needs circ/const/flip/zero", d + 1);
+                         "gravity along dimension %d of %d cells: the reference's shift by two "
+                         "cells indexes outside the array", d + 1, s->n[d]);
                 return -1;
             }
         }
@@ -1212,7 +1213,7 @@ orc_sim *orc_create(const orc_config *cfg, char *err, int errlen) {
             int m = cfg->bc[sd][d];
             if (m < ORC_BC_CIRC || m > ORC_BC_ZERO) FAIL("unknown BC mode %d", m);
             /* Every dim <= ndims is shifted at least once: MagnetArray's constructor
-             * calls updateCellCentered, which shifts mag(k) along k (MagnetArray.m:131). */
+             * calls updateCellCentered, which shifts mag(k) along k (MagnetArray.m:36-37,103). */
             if (d >= ndims) continue;
             if (m == ORC_BC_FADE && (cfg->bc_infinity < 4 || cfg->grid[d] < cfg->bc_infinity))
                 FAIL("fade BC needs bcInfinity >= 4 and >= bcInfinity cells along dim %d", d + 1);

@@ -5,7 +5,8 @@ This is a second, separately written restatement of the same .m files as the C o
 (oracle/imogen_oracle.c), kept close to the MATLAB text: whole arrays indexed with
 MATLAB's index vectors (the *_shift.m files), one object per reference array object
 with its own Edges state, and scipy's PchipInterpolator for MATLAB's pchip.  The C
-oracle is checked against these fixtures in tests/test_oracle_golden.py; agreement pins
+oracle is checked against these fixtures in tests/test_oracle.py and
+tests/test_oracle_source.py; agreement pins
 the oracle's transcription (index maps, evaluation order, edge-tracker sequence).
 
 Arrays here use the reference's dimension order: shape (nx, ny) or (nx, ny, nz).

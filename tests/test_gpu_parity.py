@@ -243,6 +243,13 @@ def test_checkpoint_resume_is_bit_exact(case):
         with pytest.raises(ImogenError):
             c.restore(ck)
         c.close()
+        # same grid and tracker layout, other settings: refused too (wall for trans keeps
+        # the layout; so do another limiter and another gamma)
+        for change in ({"bcMode": "wall"}, {"fluxLimiter": "superbee"}, {"gamma": 5.0 / 3.0}):
+            d = make_gpu(dict(ini, **change), st)
+            with pytest.raises(ImogenError, match="another configuration"):
+                d.restore(ck)
+            d.close()
     a.close()
     b.close()
 

@@ -256,3 +256,42 @@ def test_batch_with_sources_and_viscosity_matches_alone():
             assert e <= TOL, (i, f, e)
         run.close()
     ens.close()
+
+
+def test_batch_settings_changed_between_steps_match_alone():
+    """Gravity and radiation set on a simulation of a running batch right after a step
+    (the setters wait for the batch's work in flight) give what a single context doing
+    the same sequence gives."""
+    from imogen_amd.ensemble import Ensemble
+    sims = [box((40, 32, 1), {"x": "circ", "y": "trans", "z": "circ"}, seed=11),
+            box((32, 24, 1), "circ", magnet=True, seed=12),
+            problems.sod(384, bc="const", iter_max=10 ** 6)]
+    ens = Ensemble(sims, "fp64")
+    ens.step(3)
+    # no synchronisation here: the setters themselves wait for the batch
+    ens.set_gravity(0, potential(tuple(sims[0][0]["grid"])))
+    ens.set_radiation(1, "optically_thin", 0.5, 1.5)
+    ens.step(3)
+    ens.set_gravity(0, 0.5 * potential(tuple(sims[0][0]["grid"])))
+    ens.step(2)
+    for i, (ini, st) in enumerate(sims):
+        run = make_gpu(ini, st)
+        for _ in range(4):  # the batch's own first iteration + 3 batched ones
+            iteration(run)
+        if i == 0:
+            run.set_gravity(potential(tuple(ini["grid"])))
+        if i == 1:
+            run.set_radiation("optically_thin", 0.5, 1.5)
+        for _ in range(3):
+            iteration(run)
+        if i == 0:
+            run.set_gravity(0.5 * potential(tuple(ini["grid"])))
+        for _ in range(2):
+            iteration(run)
+        assert ens.time(i)[2] == run.time.iteration, i
+        fields = FLUID_FIELDS + (MAG_FIELDS if ini["mode"]["magnet"] else [])
+        for f in fields:
+            e = rel_linf(ens.download(i, f), run.download(f))
+            assert e <= TOL, (i, f, e)
+        run.close()
+    ens.close()

@@ -281,7 +281,7 @@ def test_uniform_state_is_preserved(bc):
 
 def test_uniform_state_is_preserved_1d_transparent():
     """1-D hydro with transparent ends: every flux object's edges come from an x sweep, so
-    a uniform state is stationary.  (With the magnet on it is not: relaxingMagnet.m:90-91
+    a uniform state is stationary.  (With the magnet on it is not: relaxingMagnet.m:58-59
     reuses mag(I).flux(X) for G = F - F(I+1), whose x-edge store still tracks F.)"""
     shape = (1, 1, 40)
     st = {"mass": np.full(shape, 1.3), "mom": [np.full(shape, 0.2), np.full(shape, -0.1),

@@ -81,6 +81,19 @@ def test_hydro_saves_empty_magnet_fields(tmp_path):
     assert s["magX"].size == 0 and s["grav"].size == 0
 
 
+def test_gravity_potential_slice_reads_back(tmp_path):
+    """With gravity on, grav holds the potential's slice (resultsHandler.m:81-84)."""
+    from scipy.io import loadmat
+    st = state_of((3, 4, 5))
+    st["grav"] = np.arange(60, dtype=float).reshape(3, 4, 5)
+    paths = save.save_state_slices(str(tmp_path), st, kinds=(2, 7), slice_index=(2, 1, 3),
+                                   suffix="0012", **meta())
+    line_y = loadmat(paths[0])["sx_Y_0012"][0, 0]["grav"]
+    assert line_y.shape == (4, 1) and np.array_equal(line_y[:, 0], st["grav"][2, :, 1])
+    whole = loadmat(paths[1])["sx_XYZ_0012"][0, 0]["grav"]
+    assert whole.shape == (5, 4, 3) and whole[4, 3, 2] == st["grav"][2, 3, 4]
+
+
 @pytest.mark.gpu
 def test_run_slices_from_the_device(tmp_path):
     """resultsHandler-format slices of a GPU run hold the downloaded fields."""
