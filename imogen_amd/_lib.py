@@ -79,6 +79,7 @@ SIGNATURES = {
     "imh_set_iteration": (_I, [_P, _I]),
     "imh_synchronize": (_I, [_P]),
     "imh_conservation_sums": (_I, [_P, _D]),
+    "imh_conservation_sums_noalloc": (_I, [ctypes.POINTER(_P), _I, _I, ctypes.c_size_t, _P, _P]),
     "imh_memory_bytes": (_I, [_P, ctypes.POINTER(ctypes.c_size_t)]),
     "imh_memory_estimate": (_I, [ctypes.POINTER(ImhConfig), ctypes.POINTER(ctypes.c_size_t)]),
     "imh_init_problem": (_I, [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_int]),

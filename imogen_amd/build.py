@@ -25,7 +25,7 @@ COMMON = [
 ]
 
 SOURCES = ["magnet_dispatch.hip", "cfl.hip", "fluid_dispatch.cpp", "fluid_column.hip",
-           "initial.hip", "source.hip", "imh_api.cpp", "batch.cpp", "serve.cpp"]
+           "initial.hip", "source.hip", "imh_api.cpp", "batch.cpp", "serve.cpp", "noalloc.hip"]
 VARIANT_SRC = "fluid_sweep_variant.hip"
 MAG_VARIANT_SRC = "magnet_sweep_variant.hip"
 

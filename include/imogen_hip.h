@@ -178,6 +178,20 @@ int imh_synchronize(imh_ctx *ctx);
  * reduced in a fixed order (the result does not depend on launch shape). */
 int imh_conservation_sums(imh_ctx *ctx, double out[8]);
 
+/* Allocation-free form of imh_conservation_sums.  It has no counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under stream capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  fields[0..nfields) are device arrays
+ * of ncell elements (double for IMH_FP64, float for IMH_FP32_BULK) and out is device
+ * memory for nfields doubles; the sums are queued on `stream` (a hipStream_t, NULL for the
+ * default stream) and the call returns without waiting.  Same kernels and reduction order
+ * as imh_conservation_sums, so the same bits.  The block partials live in one fixed
+ * 128 KiB device array per GPU, held until the process ends: at most
+ * IMH_NOALLOC_MAX_FIELDS fields per call (more is refused with IMH_EINVAL), and calls that
+ * overlap on one GPU race on that array (order them on one stream). */
+#define IMH_NOALLOC_MAX_FIELDS 8
+int imh_conservation_sums_noalloc(const void *const *fields, int nfields, int precision,
+                                  size_t ncell, double *out, void *stream);
+
 /* Device bytes held by the context. */
 int imh_memory_bytes(imh_ctx *ctx, size_t *bytes);
 /* Device bytes a context with this configuration would hold (validates cfg first). */
