@@ -61,9 +61,10 @@ def parse():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle leg")
     ap.add_argument("--no-large", action="store_true", help="skip the 256^3 plain roofline")
-    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
-                    help="also run one device copy of this many MiB (the reference kernel "
-                         "for scaling HBM counters)")
+    ap.add_argument("--copy-probe", type=int, default=1024, metavar="MIB",
+                    help="device copy of this many MiB (measured copy bandwidth beside the "
+                         "8 TB/s peak, and the reference kernel for scaling HBM counters; "
+                         "0 skips it)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving stream")
     ap.add_argument("--no-configs", action="store_true",
@@ -419,51 +420,101 @@ def serving_leg(args):
     return out
 
 
-# (name, generator, GPU iterations, CPU iterations)
+# (name, generator, GPU run: iterations or ("t_end", t), CPU iterations from the IC)
 CONFIGS = [("sod 1024 (vanLeer, trans)", lambda p: p.sod(1024), 200, 200),
-           ("orszag-tang 512^2 MHD (superbee, periodic)", lambda p: p.orszag_tang(512), 100, 2),
-           ("sedov 256^3 hydro (minmod, trans)", lambda p: p.sedov(256), 10, 1)]
+           ("orszag-tang 512^2 MHD (superbee, periodic)", lambda p: p.orszag_tang(512), 1000, 2),
+           ("sedov 256^3 hydro (minmod, trans)", lambda p: p.sedov(256), ("t_end", 0.05), 1)]
+
+
+def host_cpu():
+    """The host CPU: model name (from /proc/cpuinfo) and the logical cores this process
+    may use."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"model": model, "nproc": usable, "cpu_count": os.cpu_count()}
+
+
+def _advance(run, plan):
+    """Advance a run by a fixed number of iterations, or to its end time (timeMax set in
+    its settings); returns the iterations done."""
+    if not isinstance(plan, tuple):
+        run.run_iterations(plan)
+        return plan
+    done = 0
+    while True:
+        d = run.run_iterations(1000)
+        done += d
+        if d < 1000:
+            return done
 
 
 def configs_leg(args):
-    """Each config on the plain path: FP64 and FP32-bulk rates over the same iterations
-    (after one warm iteration), every field's FP32 error against FP64, conservation sums,
-    and the oracle on 1 core and on all cores (at most 16; a bounded number of
-    iterations from the initial state)."""
+    """Each config on the plain path as stated: Sod 200 iterations, Orszag-Tang 1000,
+    Sedov to t = 0.05.  FP64 and FP32-bulk rates over the whole run (after one warm
+    iteration from the same initial state, which restarts from the IC), every field's
+    FP32 error against FP64 at the end, FP64 conservation sums before / after (for Sedov
+    of both precisions at t = 0.05), and the oracle on 1 core and on all cores (at most
+    16; a bounded number of iterations from the initial state)."""
     import imogen_amd
     from imogen_amd import problems
     sys.path.insert(0, os.path.join(HERE, "tests"))
     from helpers import make_oracle
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = max(1, min(16, host_cpu()["nproc"] or 1))
     out = []
-    for name, gen, n_gpu, n_cpu in CONFIGS:
+    for name, gen, plan, n_cpu in CONFIGS:
         ini, st = gen(problems)
-        ini = dict(ini, iterMax=10 ** 8, timeMax=1e9)
+        if isinstance(plan, tuple):
+            ini = dict(ini, iterMax=10 ** 8, timeMax=plan[1])
+        else:
+            ini = dict(ini, iterMax=10 ** 8, timeMax=1e9)
         cells = ncells(ini)
-        e = {"config": name, "cells": cells, "gpu_iterations": n_gpu}
+        e = {"config": name, "cells": cells}
+        if isinstance(plan, tuple):
+            e["t_end"] = plan[1]
         res = {}
         for prec in ("fp64", "fp32"):
+            # warm-up (graph capture, first-assignment iteration) on a throw-away context
+            run = imogen_amd.initialize(ini, prec)
+            imogen_amd.create_arrays(run, st)
+            run.run_iterations(2)
+            run.synchronize()
+            run.close()
             run = imogen_amd.initialize(ini, prec)
             imogen_amd.create_arrays(run, st)
             s0 = run.conservation_sums()
-            run.run_iterations(1)
             run.synchronize()
             t0 = time.perf_counter()
-            run.run_iterations(n_gpu)
+            n_it = _advance(run, plan)
             run.synchronize()
             dt = time.perf_counter() - t0
-            e["gpu_%s_mcell_updates_per_s" % prec] = round(2.0 * cells * n_gpu / dt / 1e6, 1)
+            t_now, it_now = run.time.time, run.time.iteration
+            e["gpu_%s_iterations" % prec] = int(it_now)
+            e["gpu_%s_seconds" % prec] = round(dt, 3)
+            e["gpu_%s_mcell_updates_per_s" % prec] = round(2.0 * cells * n_it / dt / 1e6, 1)
+            if isinstance(plan, tuple):
+                e["gpu_%s_time_reached" % prec] = float("%.12g" % t_now)
             res[prec] = {f: run.download(f) for f in FIELDS}
-            if prec == "fp64":
-                s1 = run.conservation_sums()
-                e["fp64_sums"] = {"before": [float("%.12e" % v) for v in s0[:5]],
-                                  "after": [float("%.12e" % v) for v in s1[:5]],
-                                  "order": "mass, mom x/y/z, energy"}
+            s1 = run.conservation_sums()
+            if prec == "fp64" or isinstance(plan, tuple):
+                e["%s_sums" % prec] = {"before": [float("%.12e" % v) for v in s0[:5]],
+                                       "after": [float("%.12e" % v) for v in s1[:5]],
+                                       "order": "mass, mom x/y/z, energy (FP64, compensated)"}
             run.close()
         e["fp32_error_vs_fp64"] = field_errors([(res["fp32"], res["fp64"])])
         del res
-        log("config %s: gpu fp64 %.1f, fp32 %.1f Mcell-updates/s" % (
-            name, e["gpu_fp64_mcell_updates_per_s"], e["gpu_fp32_mcell_updates_per_s"]))
+        log("config %s: gpu fp64 %.1f (%d it), fp32 %.1f (%d it) Mcell-updates/s" % (
+            name, e["gpu_fp64_mcell_updates_per_s"], e["gpu_fp64_iterations"],
+            e["gpu_fp32_mcell_updates_per_s"], e["gpu_fp32_iterations"]))
         if not args.no_cpu:
             for label, th in (("cpu_1core", 1), ("cpu_all_cores", threads)):
                 orc = make_oracle(ini, st, threads=th)  # no warm-up: plain C, nothing to JIT
@@ -619,6 +670,7 @@ def main():
         crate, done, t_cpu, n_iter = cpu_leg(sims, args)
         out["cpu_baseline"] = {
             "value": round(crate, 3), "unit": "Mcell-updates/s", "cores": 1, "kind": "port",
+            "host_cpu": host_cpu(),
             "sample": "%d of the ensemble's simulations (%d cells), %d iterations each, "
                       "%.1f s on one host core" % (len(done), sum(ncells(sims[i][0]) for i, _ in done),
                                                    n_iter, t_cpu)}

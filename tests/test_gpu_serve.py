@@ -122,3 +122,36 @@ def test_device_generated_requests():
     assert srv.status(hid)[0] == DONE
     srv.release(hid)
     srv.close()
+
+
+def test_stated_sizes_alone_equals_mix():
+    """The serving stream's size range: 2-D grids of 64^2, 128^2 and 1024^2 and Sedov
+    blasts of 64^3 and 192^3, admitted while others run (four slots), each equal to the
+    same request run alone (FP32 bulk, exactly)."""
+    reqs = limited([problems.sedov(64, time_max=1.0),
+                    problems.kelvin_helmholtz(128, seed=3),
+                    problems.orszag_tang(128),
+                    problems.kelvin_helmholtz(1024, seed=8),
+                    problems.sedov(192, time_max=1.0),
+                    problems.kelvin_helmholtz(64, seed=1)], 12)
+    srv = Server("fp32", max_active=4)
+    ids = [srv.submit(*reqs[i]) for i in range(2)]
+    srv.step(3)
+    ids += [srv.submit(*r) for r in reqs[2:]]
+    for _ in range(200):
+        running, queued = srv.step(2)
+        if running == 0 and queued == 0:
+            break
+    for rid, (ini, st) in zip(ids, reqs):
+        status, t, it, ev = srv.status(rid)
+        assert status == DONE, (ini["grid"], status)
+        run = make_gpu(ini, st, "fp32")
+        run.run_iterations(10 ** 6)
+        assert it == run.time.iteration and t == run.time.time, ini["grid"]
+        for f in ("mass", "momx", "momy", "ener", "magx"):
+            e = rel_linf(srv.download(rid, f), run.download(f))
+            assert e == 0.0, (ini["grid"], f, e)
+        run.close()
+        srv.release(rid)
+    assert srv.stats()["used_bytes"] == 0
+    srv.close()
