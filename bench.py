@@ -76,6 +76,9 @@ def parse():
     ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--serve-rates", default="10,40,160", help="Poisson arrival rates (req/s)")
     ap.add_argument("--profile-steps", type=int, default=6)
+    ap.add_argument("--only-plain", action="store_true",
+                    help="run only the plain FP64 256^3 roofline leg (a short program for "
+                         "counter passes) and print its JSON")
     ap.add_argument("--pmc-traffic", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "BENCH_LOG"),
                     help="offline: HBM bytes per launch from two rocprofv3 --pmc passes")
     ap.add_argument("--trace-summary", nargs=2, metavar=("KERNEL_TRACE_CSV", "BENCH_LOG"),
@@ -584,6 +587,9 @@ def main():
         return 2
     import imogen_amd._lib as L
     L.lib()  # fails loudly if the HIP library is missing
+    if args.only_plain:
+        print(json.dumps({"plain_roofline": large_plain_roofline(args)}))
+        return 0
 
     sims = ensemble(args.seed, args.riemann, args.kh)
     cells = np.array([ncells(ini) for ini, _ in sims], dtype=np.int64)

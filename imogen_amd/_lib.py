@@ -128,6 +128,8 @@ EXTRA = {
     "imh_get_fluid_tile": (_I, [_P, _I, ctypes.POINTER(_I), ctypes.POINTER(_I),
                                 ctypes.POINTER(_I)]),
     "imh_set_graphs": (_I, [_P, _I]),
+    "imh_set_fluid_kernel": (_I, [_P, _I]),
+    "imh_get_fluid_kernel": (_I, [_P, ctypes.POINTER(_I)]),
     "imh_batch_set_fluid_tile": (_I, [_P, _I, _I, _I, _I, _I]),
     "imh_profile_iterations": (_I, [_P, _I, _D, _D, ctypes.POINTER(_I)]),
     "imh_batch_profile_step": (_I, [_P, _I, _D, _D, ctypes.POINTER(_I)]),

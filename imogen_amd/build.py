@@ -27,6 +27,7 @@ COMMON = [
 SOURCES = ["magnet_dispatch.hip", "cfl.hip", "fluid_dispatch.cpp", "fluid_column.hip",
            "initial.hip", "source.hip", "imh_api.cpp", "batch.cpp", "serve.cpp", "noalloc.hip"]
 VARIANT_SRC = "fluid_sweep_variant.hip"
+WAVE_VARIANT_SRC = "fluid_wave_variant.hip"
 MAG_VARIANT_SRC = "magnet_sweep_variant.hip"
 
 
@@ -45,6 +46,14 @@ def magnet_variants():
     return [tuple(int(v) for v in t) for t in re.findall(r"X\((\d+), (\d+)\)", m.group(1))]
 
 
+def wave_variants():
+    """(element size, block size, fast FP64) of every wave-spread sweep variant
+    (IMH_WAVE_VARIANTS in imh_kernels.h)."""
+    text = open(os.path.join(CSRC, "imh_kernels.h")).read()
+    m = re.search(r"#define IMH_WAVE_VARIANTS\(X\)((?:[^\n]*\\\n)*[^\n]*)", text)
+    return [tuple(int(v) for v in t) for t in re.findall(r"X\((\d+), (\d+), (\d+)\)", m.group(1))]
+
+
 def units():
     """(source, object name, extra flags) of every translation unit."""
     out = [(s, os.path.splitext(s)[0], []) for s in SOURCES]
@@ -54,6 +63,13 @@ def units():
     for e, b, k in fluid_variants():
         out.append((VARIANT_SRC, "fluid_sweep_v%d_%d_%d" % (e, b, k),
                     ["-DIMH_FLUID_ESZ=%d" % e, "-DIMH_FLUID_BS=%d" % b, "-DIMH_FLUID_KMAX=%d" % k]
+                    + (fp32 if e == 4 else [])))
+    for e, b, f in wave_variants():
+        # FP64 units keep contraction off: the exact ones for bit parity with the
+        # restatement, the fast ones because contracted builds of this kernel gave wrong
+        # MHD sweeps along y (their arithmetic gain is the reciprocals)
+        out.append((WAVE_VARIANT_SRC, "fluid_wave_v%d_%d_%d" % (e, b, f),
+                    ["-DIMH_WAVE_ESZ=%d" % e, "-DIMH_WAVE_BS=%d" % b, "-DIMH_WAVE_FAST=%d" % f]
                     + (fp32 if e == 4 else [])))
     for e, k in magnet_variants():
         out.append((MAG_VARIANT_SRC, "magnet_sweep_v%d_%d" % (e, k),
@@ -97,7 +113,7 @@ def build(verbose=False, jobs=None, incremental=False, out=None, defines=()):
     os.makedirs(build_dir, exist_ok=True)
     us = units()
     # the long fluid variants first, so they overlap with the short units
-    us.sort(key=lambda u: 0 if u[0] in (VARIANT_SRC, MAG_VARIANT_SRC) else 1)
+    us.sort(key=lambda u: 0 if u[0] in (VARIANT_SRC, MAG_VARIANT_SRC, WAVE_VARIANT_SRC) else 1)
     jobs = jobs or min(len(us), max(1, (os.cpu_count() or 2)))
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         results = list(ex.map(lambda u: _compile(u, verbose, incremental, defines, build_dir), us))

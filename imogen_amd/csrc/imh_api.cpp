@@ -690,7 +690,21 @@ static int fluid_sweep_t(imh_ctx* c, int d, int force) {
     if (!da) return IMH_EHIP;
     ProfScope ps(c, IMH_K_FLUID, double(10 + (c->mhd_kernel ? 3 : 0)) * c->ncell * sizeof(T));
     hipError_t e;
-    if (t.column && !fluid_needs_general(a)) {
+    int wbs = 0, wpx = 0, wtl = 0;
+    size_t wlds = 0;
+    if (c->sweep_kind != IMH_SWEEP_GENERAL && !fluid_needs_general(a) &&
+        wave_bc_ok(a.modes, d) && wave_tile(c->n[d], d, (int)sizeof(T), &wbs, &wpx, &wtl, &wlds)) {
+        // the wave-spread sweep: its own tile in the argument block
+        a.PX = wpx;
+        a.TL = wtl;
+        a.K = 4;
+        a.xmajor = d == 0;
+        const void* dw = imh_stage_args(c, &a, sizeof(a));
+        if (!dw) return IMH_EHIP;
+        const int nt = int((a.npencil + wpx - 1) / wpx);
+        e = launch_fluid_wave((const FluidArgs<T>*)dw, nullptr, nt, nt, wbs,
+                              c->sweep_kind == IMH_SWEEP_WAVE_FAST, c->mhd_kernel, wlds, c->st);
+    } else if (t.column && !fluid_needs_general(a)) {
         const int nt = int((a.npencil + fluid_column_pencils() - 1) / fluid_column_pencils());
         e = launch_fluid_column((const FluidArgs<T>*)da, nullptr, nt, nt, c->mhd_kernel, c->st);
     } else {
@@ -1527,6 +1541,26 @@ int imh_set_fluid_tile(imh_ctx* c, int X, int bs, int kmax, int px) {
         if (c->graph[i]) hipGraphExecDestroy(c->graph[i]);
         c->graph[i] = nullptr;
     }
+    return IMH_OK;
+}
+
+int imh_set_fluid_kernel(imh_ctx* c, int kind) {
+    if (!c) return fail(IMH_EINVAL, "null context");
+    if (kind != IMH_SWEEP_GENERAL && kind != IMH_SWEEP_WAVE && kind != IMH_SWEEP_WAVE_FAST)
+        return fail(IMH_EINVAL, "unknown sweep kind %d", kind);
+    if (kind == IMH_SWEEP_WAVE_FAST && c->cfg.precision != IMH_FP64)
+        return fail(IMH_EINVAL, "the fast FP64 sweep needs an FP64 context");
+    c->sweep_kind = kind;
+    for (int i = 0; i < 6; ++i) {
+        if (c->graph[i]) hipGraphExecDestroy(c->graph[i]);
+        c->graph[i] = nullptr;
+    }
+    return IMH_OK;
+}
+
+int imh_get_fluid_kernel(imh_ctx* c, int* kind) {
+    if (!c || !kind) return fail(IMH_EINVAL, "null argument");
+    *kind = c->sweep_kind;
     return IMH_OK;
 }
 

@@ -76,6 +76,7 @@ struct imh_ctx {
     double* csum_out = nullptr;
     bool obj_init[imh::NOBJ] = {};
     bool fluid_init = false;
+    int sweep_kind = 0;             // IMH_SWEEP_GENERAL / _WAVE / _WAVE_FAST
     bool have_state = false;
     bool mhd_kernel = false;
     bool no_graphs = false;

@@ -125,6 +125,58 @@ inline bool fluid_needs_general(const FluidArgs<T>& a) {
 IMH_FLUID_VARIANTS(IMH_FLUID_DECLARE)
 bool fluid_variant_compiled(int esz, int bs, int kmax);
 
+// Wave-spread fluid sweep variants X(element size, block size, fast FP64 arithmetic)
+// (fluid_wave_impl.h): four cells per lane, pencils of 4 * TL cells (TL a power of two).
+#define IMH_WAVE_VARIANTS(X) \
+    X(8, 256, 0) X(8, 512, 0) X(8, 256, 1) X(8, 512, 1) X(4, 256, 0) X(4, 512, 0)
+#define IMH_WAVE_LAUNCHER_(E, B, F) launch_fluid_wave_v##E##_##B##_##F
+#define IMH_WAVE_LAUNCHER(E, B, F) IMH_WAVE_LAUNCHER_(E, B, F)
+#define IMH_WAVE_DECLARE(E, B, F)                                                          \
+    hipError_t IMH_WAVE_LAUNCHER(E, B, F)(const void* args, const BatchTile* tiles, int ntiles, \
+                                          int nblocks, bool mhd, size_t lds, hipStream_t st);
+IMH_WAVE_VARIANTS(IMH_WAVE_DECLARE)
+
+// LDS of one block of the wave-spread sweep (dynamic shared memory), byte offsets: nw
+// waves, px pencils per block, dl = lane distance of chunk neighbours, multi = a pencil
+// spans waves (wave-edge halo slots needed), esz = element size.
+constexpr int WAVE_PXMAX = 32;
+struct WaveLds {
+    int red, e1, e2, end1, end2, trk, gin, gout, total;
+};
+__host__ __device__ inline WaveLds wave_lds_layout(int nw, int px, int dl, int multi, int esz) {
+    WaveLds L;
+    int o = 0;
+    auto take = [&](int bytes) {
+        const int at = o;
+        o += (bytes + 15) & ~15;
+        return at;
+    };
+    L.red = take(2 * nw * WAVE_PXMAX * esz);          // pencil maxima per wave, 2 stages
+    L.e1 = take(multi ? nw * 2 * dl * 10 * esz : 0);  // predictor halo: q, w*C of 1 cell
+    L.e2 = take(multi ? nw * 2 * dl * 20 * esz : 0);  // corrector halo: of 2 cells
+    L.end1 = take(px * 2 * 10 * esz);                 // circular ends, predictor
+    L.end2 = take(px * 2 * 20 * esz);                 // circular ends, corrector
+    L.trk = take(px * 2 * 15 * 8);                    // edge trackers of the pencil ends
+    L.gin = take(px * 2 * 5 * 2 * 4 * 8);             // ghost inputs
+    L.gout = take(px * 2 * 5 * 4 * 8);                // ghost values
+    L.total = o;
+    return L;
+}
+
+// Tile of the wave-spread sweep for pencils of N cells along dim d: block size, pencils
+// per block and lanes per pencil, with its dynamic LDS bytes; false when the sweep does
+// not fit it (N != 4 * TL with TL a power of two, or too few lanes per pencil).
+bool wave_tile(int N, int d, int esz, int* bs, int* px, int* tl, size_t* lds);
+// whether the boundaries of dim d allow the wave-spread sweep (both ends circular or
+// neither)
+inline bool wave_bc_ok(int modes, int d) {
+    return (mode_at(modes, 0, d) == IMH_BC_CIRC) == (mode_at(modes, 1, d) == IMH_BC_CIRC);
+}
+template <typename T>
+hipError_t launch_fluid_wave(const FluidArgs<T>* args, const BatchTile* tiles, int ntiles,
+                             int nblocks, int bs, bool fast, bool mhd, size_t lds,
+                             hipStream_t st);
+
 // One magnetFlux sub-step (component I advected along X) of one simulation
 // (magnet_sweep_impl.h).
 template <typename T>

@@ -255,6 +255,13 @@ class ImogenManager:
     def set_fluid_tile(self, X, bs, kmax, px):
         check(lib().imh_set_fluid_tile(self._ctx, int(X), int(bs), int(kmax), int(px)))
 
+    SWEEP_KINDS = {"general": 0, "wave": 1, "wave_fast": 2}
+
+    def set_fluid_kernel(self, kind):
+        """The fluid sweep kernel: 'general', 'wave' (wave-spread, the precision's own
+        arithmetic) or 'wave_fast' (wave-spread, fast FP64 arithmetic)."""
+        check(lib().imh_set_fluid_kernel(self._ctx, self.SWEEP_KINDS[kind]))
+
     def get_fluid_tile(self, X):
         bs, km, px = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         check(lib().imh_get_fluid_tile(self._ctx, int(X), ctypes.byref(bs), ctypes.byref(km),
