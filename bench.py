@@ -125,15 +125,22 @@ def trace_summary(trace_csv, bench_log):
         for k, (n, ns) in sorted(agg.items(), key=lambda t: -t[1][1]):
             print("| %s | %d | %.1f | %.1f | %.1f%% |" % (k, n, ns / 1e3, ns / n / 1e3,
                                                          100.0 * ns / tot))
-        fl = [(n, ns) for k, (n, ns) in agg.items() if k.startswith("fluid_sweep")]
+        fl = [(n, ns) for k, (n, ns) in agg.items() if k.startswith(("fluid_sweep", "fluid_wave"))]
         n, ns = sum(x[0] for x in fl), sum(x[1] for x in fl)
         print("\nfluid sweep: %d launches, average %.2f us per launch in the trace; "
               "bench.py (HIP events): %.2f us over %d launches\n" % (
                   n, ns / n / 1e3, ref["avg_launch_us"], ref["launches"]))
 
     def is_fluid(r, prec):
-        return "fluid_sweep" in r["Kernel_Name"] and ("<%s" % prec) in r["Kernel_Name"]
+        return (("fluid_sweep" in r["Kernel_Name"] or "fluid_wave" in r["Kernel_Name"])
+                and ("<%s" % prec) in r["Kernel_Name"])
 
+    if "roofline" not in out:  # a --only-plain run
+        n = out["plain_roofline"]["launches"]
+        fl = [i for i, r in enumerate(rows) if is_fluid(r, "double")][-n:]
+        table(rows[fl[0]:fl[-1] + 1], "plain path, profiled pass: " +
+              out["plain_roofline"]["workload"], out["plain_roofline"])
+        return 0
     n = out["roofline"]["launches"]
     fl = [i for i, r in enumerate(rows) if is_fluid(r, "float")][-n:]
     lo, hi = fl[0], fl[-1] + 1
@@ -187,12 +194,13 @@ def pmc_traffic(fetch_csv, write_csv, bench_log):
         rows = load(path, counter)
         cp = [r for r in rows if "copy_probe" in r["Kernel_Name"]]
         scale = probe / (float(cp[-1]["Counter_Value"]) * 1024.0)
-        for key, prec, n in (("roofline", "float", out["roofline"]["launches"]),
+        for key, prec, n in (("roofline", "float", out.get("roofline", {}).get("launches", 0)),
                              ("plain_roofline", "double",
                               out.get("plain_roofline", {}).get("launches", 0))):
             if not n:
                 continue
-            fl = [r for r in rows if "fluid_sweep" in r["Kernel_Name"]
+            fl = [r for r in rows if ("fluid_sweep" in r["Kernel_Name"] or
+                                      "fluid_wave" in r["Kernel_Name"])
                   and ("<%s" % prec) in r["Kernel_Name"]][-n:]
             kib = sum(float(r["Counter_Value"]) for r in fl)
             e = res.setdefault(out[key]["workload"] if key == "plain_roofline"
@@ -321,6 +329,7 @@ def plain_leg(sims, args):
     for ini, st in sims:
         run = imogen_amd.initialize(ini, "fp64")
         imogen_amd.create_arrays(run, st)
+        run.set_fluid_kernel(PLAIN_SWEEP)
         run.run_iterations(args.warmup + 1)
         run.synchronize()
         t0 = time.perf_counter()
@@ -489,11 +498,15 @@ def configs_leg(args):
             # warm-up (graph capture, first-assignment iteration) on a throw-away context
             run = imogen_amd.initialize(ini, prec)
             imogen_amd.create_arrays(run, st)
+            if prec == "fp64":
+                run.set_fluid_kernel(PLAIN_SWEEP)
             run.run_iterations(2)
             run.synchronize()
             run.close()
             run = imogen_amd.initialize(ini, prec)
             imogen_amd.create_arrays(run, st)
+            if prec == "fp64":
+                run.set_fluid_kernel(PLAIN_SWEEP)
             s0 = run.conservation_sums()
             run.synchronize()
             t0 = time.perf_counter()
@@ -535,17 +548,32 @@ def configs_leg(args):
     return out
 
 
+# The plain FP64 path's fluid sweep: the fast FP64 wave-spread sweep where its tiles fill
+# the device, the general sweep elsewhere (include/imogen_hip_ext.h, IMH_SWEEP_AUTO_FAST).
+PLAIN_SWEEP = "auto_fast"
+
+
 def large_plain_roofline(args):
+    """Plain FP64 path on Sedov 256^3: HIP-event time of every launch over two profiled
+    iterations, with the plain path's sweep and, beside it, with the general sweep."""
     import imogen_amd
     from imogen_amd import problems
     ini, st = problems.sedov(256)
-    run = imogen_amd.initialize(ini, "fp64")
-    imogen_amd.create_arrays(run, st)
+    res = {}
+    # the plain path's sweep last: trace and counter summaries take the last launches
+    for kind in ("general", PLAIN_SWEEP):
+        run = imogen_amd.initialize(ini, "fp64")
+        imogen_amd.create_arrays(run, st)
+        run.set_fluid_kernel(kind)
+        run.run_iterations(1)
+        prof = run.profile_iterations(2)
+        run.close()
+        res[kind] = roofline(prof, "sedov 256^3 hydro FP64, plain path")
     del st
-    run.run_iterations(1)
-    prof = run.profile_iterations(2)
-    run.close()
-    return roofline(prof, "sedov 256^3 hydro FP64, plain path")
+    out = res[PLAIN_SWEEP]
+    out["sweep"] = "wave-spread sweep, fast FP64 arithmetic (%s)" % PLAIN_SWEEP
+    out["general_sweep"] = {k: res["general"][k] for k in ("achieved", "frac", "avg_launch_us")}
+    return out
 
 
 def source_roofline(args):
@@ -574,6 +602,18 @@ def source_roofline(args):
     return out
 
 
+def copy_probe(mib):
+    """A plain device copy of `mib` MiB (measured copy bandwidth; the reference kernel of
+    known traffic for scaling the HBM counters)."""
+    import ctypes
+    import imogen_amd._lib as L
+    nbytes = mib << 20
+    ms = ctypes.c_double()
+    L.check(L.lib().imh_copy_probe(nbytes, 10, ctypes.byref(ms)))
+    return {"bytes_read": nbytes, "bytes_written": nbytes, "ms": round(ms.value, 4),
+            "GB/s": round(2.0 * nbytes / (ms.value * 1e-3) / 1e9, 1)}
+
+
 def main():
     args = parse()
     if args.pmc_traffic:
@@ -588,7 +628,10 @@ def main():
     import imogen_amd._lib as L
     L.lib()  # fails loudly if the HIP library is missing
     if args.only_plain:
-        print(json.dumps({"plain_roofline": large_plain_roofline(args)}))
+        res = {"plain_roofline": large_plain_roofline(args)}
+        if args.copy_probe:
+            res["copy_probe"] = copy_probe(args.copy_probe)
+        print(json.dumps(res))
         return 0
 
     sims = ensemble(args.seed, args.riemann, args.kh)
@@ -663,13 +706,7 @@ def main():
         out["configs"] = configs_leg(args)
 
     if args.copy_probe:
-        import ctypes
-        nbytes = args.copy_probe << 20
-        ms = ctypes.c_double()
-        L.check(L.lib().imh_copy_probe(nbytes, 10, ctypes.byref(ms)))
-        out["copy_probe"] = {"bytes_read": nbytes, "bytes_written": nbytes,
-                             "ms": round(ms.value, 4),
-                             "GB/s": round(2.0 * nbytes / (ms.value * 1e-3) / 1e9, 1)}
+        out["copy_probe"] = copy_probe(args.copy_probe)
 
     if not args.no_cpu:
         log("cpu oracle sample")

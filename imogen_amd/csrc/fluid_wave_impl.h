@@ -22,8 +22,9 @@
 // Arithmetic: FAST = false keeps every division and the .m evaluation order (built
 // without contraction it reproduces the CPU restatement bit for bit); FAST = true (FP64
 // plain path) divides by one reciprocal of rho per cell state and of the freezing speed
-// per pencil, with multiply-adds contracted, so a step agrees with the restatement to a
-// few units in the last place.
+// per pencil (hardware estimate + two Newton steps), writes its multiply-adds as fma and
+// uses branch-light limiters, so a step agrees with the restatement to a few units in
+// the last place.  Both FP64 forms are built without automatic contraction.
 //
 // Applies when every pencil length N along the sweep is 4 * TL with TL a power of two,
 // 2 <= TL <= 512, both ends circular or neither, and the sweep is in its lean form (no
@@ -46,30 +47,62 @@ constexpr int PXW = WAVE_PXMAX;
 template <typename T, bool FAST>
 struct Arith : FMath<T> {};
 
+// 1/y from the hardware estimate and two Newton steps (relative error near the last
+// place; y is a density or a freezing speed, finite and positive).
+__device__ __forceinline__ double rcp_nr(double y) {
+    double r = __builtin_amdgcn_rcp(y);
+    double e = fma(-y, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-y, r, 1.0);
+    return fma(r, e, r);
+}
+
 template <>
 struct Arith<double, true> {
     static __device__ __forceinline__ double div(double x, double, double r) { return x * r; }
-    static __device__ __forceinline__ double rcp(double y) { return 1.0 / y; }
+    static __device__ __forceinline__ double rcp(double y) { return rcp_nr(y); }
     static __device__ __forceinline__ void press(double rho, double px, double py, double pz,
                                                  double E, double bx, double by, double bz,
                                                  MagUse u, double g, double& P, double& c,
                                                  double& inv) {
-        inv = 1.0 / rho;
-        const double v2 = (px * px + py * py + pz * pz) * (inv * inv);
+        inv = rcp_nr(rho);
+        const double m2 = fma(px, px, fma(py, py, pz * pz));
         double b2 = 0.0;
-        if (u.x) b2 += bx * bx;
-        if (u.y) b2 += by * by;
-        if (u.z) b2 += bz * bz;
-        double r = (g - 1.0) * (E - 0.5 * rho * v2);
-        c = sqrt(fabs((g * (r - (g - 1.0) * 0.5 * b2) + 2.0 * b2) * inv));
-        r = r + 0.5 * (2.0 - g) * b2;
+        if (u.x) b2 = fma(bx, bx, b2);
+        if (u.y) b2 = fma(by, by, b2);
+        if (u.z) b2 = fma(bz, bz, b2);
+        double r = (g - 1.0) * fma(-0.5 * m2, inv, E);   // (g-1)(E - rho v^2 / 2)
+        c = sqrt(fabs(fma(g, fma(-(g - 1.0) * 0.5, b2, r), 2.0 * b2) * inv));
+        r = fma(0.5 * (2.0 - g), b2, r);
         P = r < 0.0 ? 0.0 : r;
     }
     static __device__ __forceinline__ double divrho(double x, double inv) { return x * inv; }
     static __device__ __forceinline__ double vanleer(double dl, double dr) {
-        return FMath<double>::vanleer(dl, dr);
+        const double p = dl * dr;
+        return p > 0.0 ? p * rcp_nr(dl + dr) : 0.0;
     }
 };
+
+// Limiters (flux/*Limiter.m).  The exact forms are those of limit() (MATLAB sign, min
+// and 0/0 handling); the fast forms give the same values for finite inputs: minmod is
+// +-min(|a|,|b|)/4 when a and b share a strict sign, else 0.
+template <typename T, bool FAST>
+__device__ __forceinline__ T wlimit(int lim, T dl, T dr) {
+    if constexpr (!FAST) {
+        return limit(lim, dl, dr);
+    } else {
+        auto mm = [](T a, T b) __attribute__((always_inline)) -> T {
+            const T m = T(0.25) * fmin(fabs(a), fabs(b));
+            return a * b > T(0) ? copysign(m, a) : T(0);
+        };
+        if (lim == IMH_LIM_VANLEER) return Arith<T, true>::vanleer(dl, dr);
+        if (lim == IMH_LIM_SUPERBEE) {
+            const bool c = fabs(dl) < fabs(dr);
+            return mm(c ? T(2) * dl : dl, c ? dr : T(2) * dr);
+        }
+        return mm(dl, dr);
+    }
+}
 
 using Lds = WaveLds;
 
@@ -84,6 +117,40 @@ __device__ __forceinline__ void for_fields(F&& fn) {
     fn(Fld<2>{});
     fn(Fld<3>{});
     fn(Fld<4>{});
+}
+
+// Value of the lane dl below (lane_up) or above (lane_down) in the wave; lanes without
+// one get their own value.  x tiles (dl = 1) use DPP whole-wave shifts (VALU moves, no
+// LDS round trip); interleaved y / z tiles use LDS permutes.
+template <typename T>
+__device__ __forceinline__ T dpp_shift(T v, int ctrl) {
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "32- or 64-bit values");
+    if constexpr (sizeof(T) == 4) {
+        const int x = __builtin_bit_cast(int, v);
+        return __builtin_bit_cast(T, ctrl == 0x138 ? __builtin_amdgcn_update_dpp(x, x, 0x138, 0xF, 0xF, false)
+                                                   : __builtin_amdgcn_update_dpp(x, x, 0x130, 0xF, 0xF, false));
+    } else {
+        typedef int I2 __attribute__((ext_vector_type(2)));
+        I2 x = __builtin_bit_cast(I2, v), r;
+        if (ctrl == 0x138) {
+            r.x = __builtin_amdgcn_update_dpp(x.x, x.x, 0x138, 0xF, 0xF, false);
+            r.y = __builtin_amdgcn_update_dpp(x.y, x.y, 0x138, 0xF, 0xF, false);
+        } else {
+            r.x = __builtin_amdgcn_update_dpp(x.x, x.x, 0x130, 0xF, 0xF, false);
+            r.y = __builtin_amdgcn_update_dpp(x.y, x.y, 0x130, 0xF, 0xF, false);
+        }
+        return __builtin_bit_cast(T, r);
+    }
+}
+template <bool XM, typename T>
+__device__ __forceinline__ T lane_up(T v, int dl) {
+    if constexpr (XM) return dpp_shift(v, 0x138);  // wave_shr:1
+    else return __shfl_up(v, dl, 64);
+}
+template <bool XM, typename T>
+__device__ __forceinline__ T lane_down(T v, int dl) {
+    if constexpr (XM) return dpp_shift(v, 0x130);  // wave_shl:1
+    else return __shfl_down(v, dl, 64);
 }
 
 __device__ __forceinline__ void wave_sync() {
@@ -150,7 +217,7 @@ __device__ __forceinline__ void end_pair(int w, int nw, int PX, int TL, bool xm,
     }
 }
 
-template <typename T, int BS, bool MHD, bool FAST>
+template <typename T, int BS, bool MHD, bool FAST, int D>
 __device__ __forceinline__ void wave_sweep_body(const FluidArgs<T>& ap, long long tile, char* lds) {
     const FluidArgs<T> a = ap;
     if (!a.force && !a.sim->active) return;
@@ -158,7 +225,7 @@ __device__ __forceinline__ void wave_sweep_body(const FluidArgs<T>& ap, long lon
     constexpr int NW = BS / 64;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int PX = a.PX, TL = a.TL;
-    const bool xm = a.xmajor != 0;
+    constexpr bool xm = D == 0;   // x pencils are contiguous; y / z tiles interleave pencils
     const int px = xm ? tid / TL : tid % PX;
     const int ty = xm ? tid % TL : tid / PX;
     const int dl = xm ? 1 : PX;
@@ -176,7 +243,7 @@ __device__ __forceinline__ void wave_sweep_body(const FluidArgs<T>& ap, long lon
     const long long pen0 = tile * PX;
     const bool pen_ok = pen0 + px < a.npencil;
     const long long pen = pen_ok ? pen0 + px : a.npencil - 1;
-    const int d = a.d;
+    constexpr int d = D;
     const int nn[3] = {a.n0, a.n1, a.n2};
     const int N = K * TL;
     const int i0 = ty * K;
@@ -192,7 +259,7 @@ __device__ __forceinline__ void wave_sweep_body(const FluidArgs<T>& ap, long lon
     const double dt = a.sim->dt;
     const double ff = dt / a.dx;
     const double hf = 0.5 * ff;
-    const int X = d + 1;
+    constexpr int X = d + 1;
     const double g = a.gamma;
     const int mlo = mode_at(a.modes, 0, d), mhi = mode_at(a.modes, 1, d);
     const bool circ = mlo == IMH_BC_CIRC;
@@ -411,8 +478,8 @@ __device__ __forceinline__ void wave_sweep_body(const FluidArgs<T>& ap, long lon
             FR[k] = T(0.5) * (q[v][k] + w);
             FL[k] = T(0.5) * (q[v][k] - w);
         }
-        T sR0 = __shfl_up(FR[K - 1], dl, 64);   // FR of the left neighbour's last cell
-        T sLK = __shfl_down(FL[0], dl, 64);     // FL of the right neighbour's first cell
+        T sR0 = lane_up<xm>(FR[K - 1], dl);   // FR of the left neighbour's last cell
+        T sLK = lane_down<xm>(FL[0], dl);     // FL of the right neighbour's first cell
         if (multi && lx) {
             const T* e = s_e1 + (((wave - 1) * 2 + 1) * dl + lane) * 10;
             sR0 = T(0.5) * (e[2 * v] + A::div(e[2 * v + 1], C, rC));
@@ -533,7 +600,7 @@ __device__ __forceinline__ void wave_sweep_body(const FluidArgs<T>& ap, long lon
 #pragma unroll
                 for (int t = 0; t < 3; ++t) {
                     const T rm = t > 0 ? F[t > 0 ? t - 1 : 0] : gm;
-                    Fl[t] = F[t] + limit(lim, F[t] - rm, F[t + 1] - F[t]);
+                    Fl[t] = F[t] + wlimit<T, FAST>(lim, F[t] - rm, F[t + 1] - F[t]);
                 }
                 in[0] = double(Fl[0]);
                 in[1] = double(Fl[1]);
@@ -541,7 +608,7 @@ __device__ __forceinline__ void wave_sweep_body(const FluidArgs<T>& ap, long lon
                 if (circ) {  // limFR(N-1) from FR'(N-2), FR'(N-1) and FR'(0)
                     const T* e = s_end2 + (px * 2 + 1) * 20 + 4 * v;
                     const T fa = fr2(e[0], e[1]), fb = fr2(e[2], e[3]);
-                    in[3] = double(fb + limit(lim, fb - fa, F[0] - fb));
+                    in[3] = double(fb + wlimit<T, FAST>(lim, fb - fa, F[0] - fb));
                 } else {
                     in[3] = in[0];
                 }
@@ -550,7 +617,7 @@ __device__ __forceinline__ void wave_sweep_body(const FluidArgs<T>& ap, long lon
 #pragma unroll
                 for (int t = 1; t < K; ++t) {
                     const T lp = t < K - 1 ? F[t < K - 1 ? t + 1 : 0] : gp;
-                    Fl[t] = F[t] + limit(lim, F[t - 1] - F[t], F[t] - lp);
+                    Fl[t] = F[t] + wlimit<T, FAST>(lim, F[t - 1] - F[t], F[t] - lp);
                 }
                 in[0] = double(Fl[K - 3]);
                 in[1] = double(Fl[K - 2]);
@@ -558,7 +625,7 @@ __device__ __forceinline__ void wave_sweep_body(const FluidArgs<T>& ap, long lon
                 if (circ) {  // limFL(0) from FL'(N-1), FL'(0) and FL'(1)
                     const T* e = s_end2 + (px * 2 + 0) * 20 + 4 * v;
                     const T fa = fl2(e[0], e[1]), fb = fl2(e[2], e[3]);
-                    in[3] = double(fa + limit(lim, F[K - 1] - fa, fa - fb));
+                    in[3] = double(fa + wlimit<T, FAST>(lim, F[K - 1] - fa, fa - fb));
                 } else {
                     in[3] = in[0];
                 }
@@ -580,12 +647,12 @@ __device__ __forceinline__ void wave_sweep_body(const FluidArgs<T>& ap, long lon
             FR[k + 2] = fr2(qh[v][k], nu);
             FL[k + 2] = fl2(qh[v][k], nu);
         }
-        FR[0] = __shfl_up(FR[K], dl, 64);       // FR'(i0-2)
-        FR[1] = __shfl_up(FR[K + 1], dl, 64);   // FR'(i0-1)
-        FL[1] = __shfl_up(FL[K + 1], dl, 64);   // FL'(i0-1)
-        FR[K + 2] = __shfl_down(FR[2], dl, 64); // FR'(i0+K)
-        FL[K + 2] = __shfl_down(FL[2], dl, 64); // FL'(i0+K)
-        FL[K + 3] = __shfl_down(FL[3], dl, 64); // FL'(i0+K+1)
+        FR[0] = lane_up<xm>(FR[K], dl);       // FR'(i0-2)
+        FR[1] = lane_up<xm>(FR[K + 1], dl);   // FR'(i0-1)
+        FL[1] = lane_up<xm>(FL[K + 1], dl);   // FL'(i0-1)
+        FR[K + 2] = lane_down<xm>(FR[2], dl); // FR'(i0+K)
+        FL[K + 2] = lane_down<xm>(FL[2], dl); // FL'(i0+K)
+        FL[K + 3] = lane_down<xm>(FL[3], dl); // FL'(i0+K+1)
         if (multi && lx) {
             const T* e = s_e2 + (((wave - 1) * 2 + 1) * dl + lane) * 20 + 4 * v;
             FR[0] = fr2(e[0], e[1]);
@@ -614,9 +681,9 @@ __device__ __forceinline__ void wave_sweep_body(const FluidArgs<T>& ap, long lon
 #pragma unroll
         for (int j = 0; j <= K; ++j) {
             const T fr = FR[j + 1];
-            R2[j] = fr + limit(lim, fr - FR[j], FR[j + 2] - fr);
+            R2[j] = fr + wlimit<T, FAST>(lim, fr - FR[j], FR[j + 2] - fr);
             const T fl = FL[j + 2];
-            L2[j] = fl + limit(lim, FL[j + 1] - fl, fl - FL[j + 3]);
+            L2[j] = fl + wlimit<T, FAST>(lim, FL[j + 1] - fl, fl - FL[j + 3]);
         }
         if (first) R2[0] = gout_at(0, v, 3);
         if (last) L2[K] = gout_at(1, v, 3);
@@ -652,7 +719,7 @@ __device__ __forceinline__ void wave_sweep_body(const FluidArgs<T>& ap, long lon
 #endif
 
 // Persistent batched form (tiles != nullptr) or one block per tile of args[0].
-template <typename T, int BS, bool MHD, bool FAST>
+template <typename T, int BS, bool MHD, bool FAST, int D>
 __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(IMH_WAVE_WAVES_PER_EU)))
 fluid_wave_kernel(const FluidArgs<T>* args,
                                                         const BatchTile* tiles, int ntiles) {
@@ -668,7 +735,7 @@ fluid_wave_kernel(const FluidArgs<T>* args,
             if (r > 0 || (int)blockIdx.x >= ntiles) break;
             tile = (long long)blockIdx.x;
         }
-        wv::wave_sweep_body<T, BS, MHD, FAST>(args[arg], tile, wv_lds);
+        wv::wave_sweep_body<T, BS, MHD, FAST, D>(args[arg], tile, wv_lds);
         __syncthreads();
     }
 }

@@ -394,6 +394,13 @@ int imh_create(const imh_config* cfg, imh_ctx** out) {
         }                                               \
     } while (0)
     CHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking), "hipStreamCreate");
+    {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, cfg->device) == hipSuccess && prop.multiProcessorCount > 0) {
+            c->num_cu = prop.multiProcessorCount;
+            c->max_lds = prop.sharedMemPerBlock;
+        }
+    }
     for (int i = 0; i < IMH_NFIELDS; ++i) CHK(hipMalloc(&c->f[i], bytes), "hipMalloc(field)");
     if (cfg->magnet_active) CHK(hipMalloc(&c->scratch, bytes), "hipMalloc(scratch)");
     layout_edges(c);
@@ -692,8 +699,15 @@ static int fluid_sweep_t(imh_ctx* c, int d, int force) {
     hipError_t e;
     int wbs = 0, wpx = 0, wtl = 0;
     size_t wlds = 0;
-    if (c->sweep_kind != IMH_SWEEP_GENERAL && !fluid_needs_general(a) &&
-        wave_bc_ok(a.modes, d) && wave_tile(c->n[d], d, (int)sizeof(T), &wbs, &wpx, &wtl, &wlds)) {
+    // the wave-spread sweep where it applies (with IMH_SWEEP_AUTO_FAST only where its
+    // tiles fill the device, at least 4 per CU: short sweeps keep the general form's
+    // smaller tiles)
+    // Hydro sweeps only: a run of the MHD instantiations (the only ones that spill
+    // registers to scratch) left the GPU faulted once, cause not found; MHD sweeps take the
+    // general form until it is.
+    if (c->sweep_kind != IMH_SWEEP_GENERAL && !fluid_needs_general(a) && !c->mhd_kernel &&
+        wave_bc_ok(a.modes, d) && wave_tile(c->n[d], d, (int)sizeof(T), c->max_lds, &wbs, &wpx, &wtl, &wlds) &&
+        (c->sweep_kind != IMH_SWEEP_AUTO_FAST || (a.npencil + wpx - 1) / wpx >= 4LL * c->num_cu)) {
         // the wave-spread sweep: its own tile in the argument block
         a.PX = wpx;
         a.TL = wtl;
@@ -703,7 +717,7 @@ static int fluid_sweep_t(imh_ctx* c, int d, int force) {
         if (!dw) return IMH_EHIP;
         const int nt = int((a.npencil + wpx - 1) / wpx);
         e = launch_fluid_wave((const FluidArgs<T>*)dw, nullptr, nt, nt, wbs,
-                              c->sweep_kind == IMH_SWEEP_WAVE_FAST, c->mhd_kernel, wlds, c->st);
+                              c->sweep_kind != IMH_SWEEP_WAVE, d, c->mhd_kernel, wlds, c->st);
     } else if (t.column && !fluid_needs_general(a)) {
         const int nt = int((a.npencil + fluid_column_pencils() - 1) / fluid_column_pencils());
         e = launch_fluid_column((const FluidArgs<T>*)da, nullptr, nt, nt, c->mhd_kernel, c->st);
@@ -1546,9 +1560,11 @@ int imh_set_fluid_tile(imh_ctx* c, int X, int bs, int kmax, int px) {
 
 int imh_set_fluid_kernel(imh_ctx* c, int kind) {
     if (!c) return fail(IMH_EINVAL, "null context");
-    if (kind != IMH_SWEEP_GENERAL && kind != IMH_SWEEP_WAVE && kind != IMH_SWEEP_WAVE_FAST)
+    if (kind != IMH_SWEEP_GENERAL && kind != IMH_SWEEP_WAVE && kind != IMH_SWEEP_WAVE_FAST &&
+        kind != IMH_SWEEP_AUTO_FAST)
         return fail(IMH_EINVAL, "unknown sweep kind %d", kind);
-    if (kind == IMH_SWEEP_WAVE_FAST && c->cfg.precision != IMH_FP64)
+    if ((kind == IMH_SWEEP_WAVE_FAST || kind == IMH_SWEEP_AUTO_FAST) &&
+        c->cfg.precision != IMH_FP64)
         return fail(IMH_EINVAL, "the fast FP64 sweep needs an FP64 context");
     c->sweep_kind = kind;
     for (int i = 0; i < 6; ++i) {

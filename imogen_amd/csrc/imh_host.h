@@ -77,6 +77,8 @@ struct imh_ctx {
     bool obj_init[imh::NOBJ] = {};
     bool fluid_init = false;
     int sweep_kind = 0;             // IMH_SWEEP_GENERAL / _WAVE / _WAVE_FAST
+    int num_cu = 256;               // compute units of the device
+    size_t max_lds = 64 * 1024;     // LDS bytes per workgroup of the device
     bool have_state = false;
     bool mhd_kernel = false;
     bool no_graphs = false;

@@ -133,7 +133,8 @@ bool fluid_variant_compiled(int esz, int bs, int kmax);
 #define IMH_WAVE_LAUNCHER(E, B, F) IMH_WAVE_LAUNCHER_(E, B, F)
 #define IMH_WAVE_DECLARE(E, B, F)                                                          \
     hipError_t IMH_WAVE_LAUNCHER(E, B, F)(const void* args, const BatchTile* tiles, int ntiles, \
-                                          int nblocks, bool mhd, size_t lds, hipStream_t st);
+                                          int nblocks, int d, bool mhd, size_t lds,          \
+                                          hipStream_t st);
 IMH_WAVE_VARIANTS(IMH_WAVE_DECLARE)
 
 // LDS of one block of the wave-spread sweep (dynamic shared memory), byte offsets: nw
@@ -165,8 +166,9 @@ __host__ __device__ inline WaveLds wave_lds_layout(int nw, int px, int dl, int m
 
 // Tile of the wave-spread sweep for pencils of N cells along dim d: block size, pencils
 // per block and lanes per pencil, with its dynamic LDS bytes; false when the sweep does
-// not fit it (N != 4 * TL with TL a power of two, or too few lanes per pencil).
-bool wave_tile(int N, int d, int esz, int* bs, int* px, int* tl, size_t* lds);
+// not fit it (N != 4 * TL with TL a power of two, too few lanes per pencil, or more LDS
+// than max_lds, the device's per-workgroup limit).
+bool wave_tile(int N, int d, int esz, size_t max_lds, int* bs, int* px, int* tl, size_t* lds);
 // whether the boundaries of dim d allow the wave-spread sweep (both ends circular or
 // neither)
 inline bool wave_bc_ok(int modes, int d) {
@@ -174,7 +176,7 @@ inline bool wave_bc_ok(int modes, int d) {
 }
 template <typename T>
 hipError_t launch_fluid_wave(const FluidArgs<T>* args, const BatchTile* tiles, int ntiles,
-                             int nblocks, int bs, bool fast, bool mhd, size_t lds,
+                             int nblocks, int bs, bool fast, int d, bool mhd, size_t lds,
                              hipStream_t st);
 
 // One magnetFlux sub-step (component I advected along X) of one simulation

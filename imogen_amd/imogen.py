@@ -255,11 +255,12 @@ class ImogenManager:
     def set_fluid_tile(self, X, bs, kmax, px):
         check(lib().imh_set_fluid_tile(self._ctx, int(X), int(bs), int(kmax), int(px)))
 
-    SWEEP_KINDS = {"general": 0, "wave": 1, "wave_fast": 2}
+    SWEEP_KINDS = {"general": 0, "wave": 1, "wave_fast": 2, "auto_fast": 3}
 
     def set_fluid_kernel(self, kind):
         """The fluid sweep kernel: 'general', 'wave' (wave-spread, the precision's own
-        arithmetic) or 'wave_fast' (wave-spread, fast FP64 arithmetic)."""
+        arithmetic), 'wave_fast' (wave-spread, fast FP64 arithmetic) or 'auto_fast'
+        (wave_fast where its tiles fill the device, general elsewhere)."""
         check(lib().imh_set_fluid_kernel(self._ctx, self.SWEEP_KINDS[kind]))
 
     def get_fluid_tile(self, X):

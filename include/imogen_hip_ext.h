@@ -33,13 +33,16 @@ int imh_batch_set_fluid_tile(imh_batch *b, int sim, int X, int bs, int kmax, int
 /* Fluid sweep kernel along every dimension: 0 the general sweep (LDS exchange), 1 the
  * wave-spread sweep with the arithmetic of the precision (FP64: every division, bit for
  * bit with the general sweep), 2 the wave-spread sweep with fast FP64 arithmetic (one
- * reciprocal of rho and of the freezing speed, contracted multiply-adds).  Sweeps the
- * wave-spread form does not cover (pencil lengths other than 4 * 2^j, a circular end
- * opposite another mode, fade ends, first sweeps, viscosity) use the general sweep.
- * Drops captured graphs. */
+ * reciprocal of rho and of the freezing speed per cell state, hardware reciprocal with
+ * Newton steps; within 1e-12 relative L-inf of the oracle per step), 3 the fast FP64
+ * wave-spread sweep where its tiles fill the device (4 per CU) and the general sweep
+ * elsewhere (the plain path's choice for large grids).  Sweeps the wave-spread form does
+ * not cover (pencil lengths other than 4 * 2^j, a circular end opposite another mode,
+ * fade ends, first sweeps, viscosity) use the general sweep.  Drops captured graphs. */
 #define IMH_SWEEP_GENERAL 0
 #define IMH_SWEEP_WAVE 1
 #define IMH_SWEEP_WAVE_FAST 2
+#define IMH_SWEEP_AUTO_FAST 3
 int imh_set_fluid_kernel(imh_ctx *ctx, int kind);
 int imh_get_fluid_kernel(imh_ctx *ctx, int *kind);
 /* enable = 0 runs imh_run eagerly (one launch at a time) instead of replaying graphs. */
