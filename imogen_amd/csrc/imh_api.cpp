@@ -1247,8 +1247,8 @@ int imh_synchronize(imh_ctx* c) {
 
 extern "C" int imh_init_problem(imh_ctx* c, int kind, const double* params, int nparams) {
     if (!c) return fail(IMH_EINVAL, "null context");
-    static const int need[6] = {0, 0, 0, 3, 15, 4};
-    if (kind < IMH_IC_SOD || kind > IMH_IC_KELVIN_HELMHOLTZ)
+    static const int need[9] = {0, 0, 0, 3, 15, 4, 0, 0, 4};
+    if (kind < IMH_IC_SOD || kind > IMH_IC_MAGNETIC_SHOCK_TUBE)
         return fail(IMH_EINVAL, "unknown initial-condition kind %d", kind);
     if (nparams < need[kind] || (need[kind] > 0 && !params))
         return fail(IMH_EINVAL, "initial condition %d takes %d parameters", kind, need[kind]);
@@ -1300,6 +1300,41 @@ extern "C" int imh_init_problem(imh_ctx* c, int kind, const double* params, int 
         P.ip[0] = (nparams > 15 && params[15] >= 0) ? int(params[15]) : c->n[0] / 2;
         for (int i : {5, 6, 13, 14, 16}) anyB = anyB || P.p[i] != 0.0;
         break;
+    case IMH_IC_ORSZAG_TANG_REF:
+        if (c->n[0] < 2 || c->n[1] < 2)
+            return fail(IMH_EINVAL, "the reference Orszag-Tang grid needs 2+ cells in x and y "
+                        "(its sines divide by grid - 1)");
+        P.ip[0] = c->n[0] - 1;
+        P.ip[1] = c->n[1] - 1;
+        anyB = true;
+        break;
+    case IMH_IC_SEDOV_REF: {
+        // ndgrid(-half:half) has 2 * half + 1 points: the reference needs odd counts
+        for (int d = 0; d < 3; ++d) {
+            if (c->n[d] > 1 && c->n[d] % 2 == 0)
+                return fail(IMH_EINVAL, "the reference Sedov-Taylor initializer needs an odd "
+                            "cell count along every dimension (dim %d has %d): its "
+                            "ndgrid(-half:half) has 2*floor(n/2)+1 points", d + 1, c->n[d]);
+            P.ip[d] = c->n[d] / 2;
+        }
+        const int nmin = std::min(c->n[0], std::min(c->n[1], c->n[2]));
+        const int radius = std::max(5, int(std::floor(0.06 * nmin + 0.5)));  // MATLAB round
+        const double dx = 1.0 / double(c->n[0]);                               // obj.dGrid(1)
+        P.ip[3] = radius;
+        P.p[0] = 3.0 * (g - 1.0) / (4.0 * M_PI * std::pow(dx * radius, 3.0));
+        break;
+    }
+    case IMH_IC_MAGNETIC_SHOCK_TUBE: {
+        const int dir = int(params[0]);
+        if (dir < 0 || dir > 2) return fail(IMH_EINVAL, "shock direction must be 0, 1 or 2");
+        P.ip[0] = dir;
+        P.ip[1] = int(std::floor(c->n[dir] / 2.0 + 0.5));  // MATLAB round(grid / 2)
+        for (int q = 0; q < 3; ++q) {
+            P.ip[2 + q] = params[1 + q] != 0.0;
+            anyB = anyB || P.ip[2 + q];
+        }
+        break;
+    }
     case IMH_IC_KELVIN_HELMHOLTZ: {
         // external/speedFromMach.m with rho = 1, E = 1 / (gamma - 1), B = 0
         const double mach = params[0];

@@ -94,6 +94,53 @@ __device__ void ic_cell(const IcParams& P, int i, int j, int k, double out[8]) {
         out[4] = 1.0 / (g - 1.0) + 0.5 * (out[1] * out[1] + out[2] * out[2]) / mass;
         break;
     }
+    case IMH_IC_ORSZAG_TANG_REF: {
+        // OrszagTangVortexInitializer.m:72-90: x, y are cell indices, ip[0..1] = grid - 1
+        const double mass = 25.0 / (36.0 * M_PI);
+        const double x = double(i), y = double(j);
+        const double sy = sin(2.0 * M_PI * y / double(P.ip[1]));
+        const double sx = sin(2.0 * M_PI * x / double(P.ip[0]));
+        const double mag0 = 1.0 / sqrt(4.0 * M_PI);
+        out[0] = mass;
+        out[1] = -mass * sy;
+        out[2] = mass * sx;
+        out[5] = -mag0 * sy;
+        out[6] = mag0 * sin(4.0 * M_PI * x / double(P.ip[0]));
+        const double S = (out[1] * out[1] + out[2] * out[2]) + out[3] * out[3];
+        const double B = (out[5] * out[5] + out[6] * out[6]) + out[7] * out[7];
+        out[4] = (5.0 / (12.0 * M_PI) / (g - 1.0) + 0.5 * S / mass) + 0.5 * B;
+        break;
+    }
+    case IMH_IC_SEDOV_REF: {
+        // SedovTaylorBlastWaveInitializer.m:77-86: integer offsets from the centre cell
+        // (ip[0..2] = floor(grid / 2)), ip[3] radius in cells, p[0] the inside pressure
+        const long long X = i - P.ip[0], Y = j - P.ip[1], Z = k - P.ip[2];
+        const long long r = P.ip[3];
+        const double pr = (X * X + Y * Y + Z * Z > r * r) ? 1e-5 : P.p[0];
+        out[0] = 1.0;
+        out[4] = pr / (g - 1.0);
+        break;
+    }
+    case IMH_IC_MAGNETIC_SHOCK_TUBE: {
+        // MagneticShockTubeInitializer.m:79-117: ip[0] direction, ip[1] MATLAB round(N/2)
+        // (1-based first cell of the right side), ip[2..4] field components on
+        const int c[3] = {i, j, k};
+        const int dir = P.ip[0];
+        const bool right = c[dir] >= P.ip[1] - 1;
+        const double b = 5.0 / sqrt(4.0 * M_PI);
+        out[0] = 1.0;
+        for (int q = 0; q < 3; ++q) out[5 + q] = P.ip[2 + q] ? b : 0.0;
+        // mom(dir) = 10; on the right mom(dir) = -1.0 * mom(1) (the x component's value,
+        // 0 unless dir is x)
+        const double m1 = dir == 0 ? 10.0 : 0.0;
+        out[1 + dir] = right ? -1.0 * m1 : 10.0;
+        const double S = (out[1] * out[1] + out[2] * out[2]) + out[3] * out[3];
+        const double B = (out[5] * out[5] + out[6] * out[6]) + out[7] * out[7];
+        double e = 0.5 * S / out[0] + 0.5 * B;
+        e = right ? e + 1.0 / (g - 1.0) : e + 20.0 / (g - 1.0);
+        out[4] = e;
+        break;
+    }
     default: break;
     }
 }

@@ -195,7 +195,8 @@ class ImogenManager:
         check(lib().imh_memory_bytes(self._ctx, ctypes.byref(b)))
         return b.value
 
-    IC_KINDS = {"sod": 1, "orszag_tang": 2, "sedov": 3, "riemann": 4, "kelvin_helmholtz": 5}
+    IC_KINDS = {"sod": 1, "orszag_tang": 2, "sedov": 3, "riemann": 4, "kelvin_helmholtz": 5,
+                "orszag_tang_reference": 6, "sedov_reference": 7, "magnetic_shock_tube": 8}
 
     def init_problem(self, kind, params=()):
         """Generate the initial state of a benchmark problem in device memory (with the

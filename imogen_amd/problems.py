@@ -183,6 +183,107 @@ def mhd_riemann(n=1024, seed=0, gamma=5.0 / 3.0, limiter="vanleer", iter_max=100
     return ini, state
 
 
+# ---- the reference initializers themselves ---------------------------------------------
+# Cell for cell what experiment/*Initializer.m compute (the benchmark generators above
+# differ where the reference's construction is a quirk: an (N - 1)-period grid, an
+# odd-N-only Sedov grid).  The device kinds ORSZAG_TANG_REF, SEDOV_REF and
+# MAGNETIC_SHOCK_TUBE of imh_init_problem evaluate the same expressions.
+
+def orszag_tang_reference(nx=256, ny=None, limiter="vanleer"):
+    """OrszagTangVortexInitializer.m:42-58,72-90: dGrid = 1 ./ grid; x, y are the cell
+    indices 0..N-1 and the sines have period N - 1 (so the first and last columns carry
+    the same values); gamma 5/3, CFL 0.35, ITERMAX 1500, TIMEMAX 0.48, circular
+    boundaries.  The initializer sets no flux limiter (the manager's default applies)."""
+    ny = nx if ny is None else ny
+    shape = (1, ny, nx)
+    gamma = 5.0 / 3.0
+    mass = 25.0 / (36.0 * math.pi) * np.ones(shape)
+    x = np.broadcast_to(np.arange(nx, dtype=np.float64)[None, None, :], shape)
+    y = np.broadcast_to(np.arange(ny, dtype=np.float64)[None, :, None], shape)
+    mom = [-mass * np.sin(2 * math.pi * y / (ny - 1)),
+           mass * np.sin(2 * math.pi * x / (nx - 1)), _zeros(shape)]
+    mag0 = 1.0 / math.sqrt(4.0 * math.pi)
+    mag = [-mag0 * np.sin(2 * math.pi * y / (ny - 1)),
+           mag0 * np.sin(4 * math.pi * x / (nx - 1)), _zeros(shape)]
+    S = (mom[0] * mom[0] + mom[1] * mom[1]) + mom[2] * mom[2]
+    B = (mag[0] * mag[0] + mag[1] * mag[1]) + mag[2] * mag[2]
+    ener = (5.0 / (12.0 * math.pi) / (gamma - 1.0) + 0.5 * S / mass) + 0.5 * B
+    state = {"mass": mass, "mom": [np.ascontiguousarray(m) for m in mom], "ener": ener,
+             "mag": [np.ascontiguousarray(b) for b in mag]}
+    ini = _base_ini((nx, ny, 1), (1.0 / nx, 1.0 / ny, 1.0), gamma, 0.35, "circ", limiter,
+                    True, 1500, 0.48)
+    return ini, state
+
+
+def sedov_reference(n=65, ny=None, nz=None, limiter="vanleer"):
+    """SedovTaylorBlastWaveInitializer.m:30-50,71-86: dGrid = 1 ./ grid, integer offsets
+    X, Y, Z from the centre cell by ndgrid(-half:half) (half = floor(grid / 2), so every
+    dimension with more than one cell needs an odd count), radius max(5, round(0.06 *
+    min(grid))) cells, pressure 3 (gamma - 1) / (4 pi (dGrid(1) radius)^3) where
+    distance <= radius and 1e-5 elsewhere, E = P / (gamma - 1); gamma 1.4, CFL 0.7,
+    ITERMAX 100, fade boundaries."""
+    ny = n if ny is None else ny
+    nz = n if nz is None else nz
+    grid = (n, ny, nz)
+    for g in grid:
+        if g > 1 and g % 2 == 0:
+            raise ValueError("the reference Sedov-Taylor initializer needs odd cell counts "
+                             "(ndgrid(-half:half) has 2*floor(n/2)+1 points), got %s" % (grid,))
+    gamma = 1.4
+    half = [g // 2 for g in grid]
+    X = np.arange(-half[0], half[0] + 1)[None, None, :]
+    Y = np.arange(-half[1], half[1] + 1)[None, :, None]
+    Z = np.arange(-half[2], half[2] + 1)[:, None, None]
+    d2 = X * X + Y * Y + Z * Z                     # distance^2, exact in integers
+    radius = max(5, int(math.floor(0.06 * min(grid) + 0.5)))   # MATLAB round
+    dx = 1.0 / n
+    inside = 3 * (gamma - 1) / (4 * math.pi * (dx * radius) ** 3)
+    pressure = np.where(d2 > radius * radius, 1e-5, inside).astype(np.float64)
+    shape = (nz, ny, n)
+    state = {"mass": np.ones(shape), "mom": [_zeros(shape) for _ in range(3)],
+             "ener": pressure / (gamma - 1), "mag": [_zeros(shape) for _ in range(3)]}
+    ini = _base_ini(grid, (1.0 / n, 1.0 / ny, 1.0 / nz), gamma, 0.7, "fade", limiter, False,
+                    100)
+    return ini, state
+
+
+def magnetic_shock_tube(nx=1024, ny=3, nz=3, direction="X", fields=(True, True, False),
+                        limiter="vanleer"):
+    """MagneticShockTubeInitializer.m:42-66,79-117: unit spacing (the initializer leaves
+    dGrid empty; initialize.m:166-168), rho 1, B = 5/sqrt(4 pi) in the components that are
+    on, momentum 10 along the shock direction with the right side (from MATLAB round(N/2),
+    1-based) set to -1.0 times the x momentum there (so a y or z shock gets -0.0 on the
+    right: the .m text reads mom(1, ...)), E = kinetic + magnetic, plus 20/(gamma-1) on
+    the left and 1/(gamma-1) on the right; gamma 1.4, CFL 0.35, ITERMAX 250, fade in x,
+    circular in y and z."""
+    grid = (nx, ny, nz)
+    shape = (nz, ny, nx)
+    gamma = 1.4
+    dirn = "XYZ".index(direction.upper())
+    half = int(math.floor(grid[dirn] / 2.0 + 0.5))
+    mass = np.ones(shape)
+    mom = [_zeros(shape) for _ in range(3)]
+    b = 5 / math.sqrt(4 * math.pi)
+    mag = [np.full(shape, b) if on else _zeros(shape) for on in fields]
+    axis = 2 - dirn
+    idx = [slice(None)] * 3
+    idx[axis] = slice(half - 1, None)              # indeces{i} = half:grid(i)
+    right = tuple(idx)
+    mom[dirn][...] = 10.0
+    mom[dirn][right] = -1.0 * mom[0][right]
+    S = (mom[0] * mom[0] + mom[1] * mom[1]) + mom[2] * mom[2]
+    B = (mag[0] * mag[0] + mag[1] * mag[1]) + mag[2] * mag[2]
+    ener = 0.5 * S / mass + 0.5 * B
+    ener[right] = ener[right] + 1 / (gamma - 1) * 1.0
+    idx[axis] = slice(0, half - 1)                 # indeces{i} = 1:(half - 1)
+    left = tuple(idx)
+    ener[left] = ener[left] + 20 / (gamma - 1) * 1.0
+    state = {"mass": mass, "mom": mom, "ener": ener, "mag": mag}
+    ini = _base_ini(grid, (1.0, 1.0, 1.0), gamma, 0.35,
+                    {"x": "fade", "y": "circ", "z": "circ"}, limiter, True, 250)
+    return ini, state
+
+
 def random_state(grid, seed=0, magnet=True, amplitude=0.2):
     """Smooth-ish random state for unit and parity tests (positive density/pressure)."""
     rng = np.random.default_rng(seed)

@@ -216,6 +216,20 @@ int imh_memory_estimate(const imh_config *cfg, size_t *bytes);
 #define IMH_IC_SEDOV 3
 #define IMH_IC_RIEMANN 4
 #define IMH_IC_KELVIN_HELMHOLTZ 5
+/* The reference initializers themselves (experiment/*Initializer.m), cell for cell:
+ *   ORSZAG_TANG_REF  none: OrszagTangVortexInitializer.m:72-90 (sines of 2 pi i / (N - 1)
+ *                    over cell indices; the run's spacing is 1 ./ grid)
+ *   SEDOV_REF        none: SedovTaylorBlastWaveInitializer.m:77-86 (needs an odd cell
+ *                    count in every dimension with more than one cell, as the
+ *                    reference's ndgrid does; radius max(5, round(0.06 min(grid))) cells,
+ *                    pressure 3 (gamma-1) / (4 pi (radius / nx)^3) inside, 1e-5 outside)
+ *   MAGNETIC_SHOCK_TUBE  direction (0 x, 1 y, 2 z), x / y / z field on (1) or off (0):
+ *                    MagneticShockTubeInitializer.m:79-117 (field 5 / sqrt(4 pi), momentum
+ *                    10 on the left, the x momentum negated on the right, pressure 20 left
+ *                    and 1 right, split at MATLAB round(N / 2)) */
+#define IMH_IC_ORSZAG_TANG_REF 6
+#define IMH_IC_SEDOV_REF 7
+#define IMH_IC_MAGNETIC_SHOCK_TUBE 8
 int imh_init_problem(imh_ctx *ctx, int kind, const double *params, int nparams);
 
 /* ---- checkpoint / resume --------------------------------------------------------------

@@ -329,3 +329,51 @@ def test_device_kelvin_helmholtz_structure():
     assert np.all(np.isfinite(run.download("ener")))
     run.close()
     again.close()
+
+
+@pytest.mark.parametrize("name", ["orszag_tang_reference", "sedov_reference",
+                                  "magnetic_shock_tube_x", "magnetic_shock_tube_y"])
+def test_device_reference_initializers(name):
+    """The device kinds that follow the reference initializers give the state of the
+    host restatements (problems.*_reference, magnetic_shock_tube): to the last bit where
+    only + - * / enter, to ~1 ulp where sin / pow do."""
+    if name == "orszag_tang_reference":
+        ini, st = problems.orszag_tang_reference(48)
+        params, tol, kind = (), 4e-16, name
+    elif name == "sedov_reference":
+        ini, st = problems.sedov_reference(33)
+        params, tol, kind = (), 4e-16, name
+    else:
+        d = name[-1].upper()
+        # the initializer's fade x boundary needs bcInfinity (20) cells along x
+        grid = (64, 3, 3) if d == "X" else (24, 40, 3)
+        ini, st = problems.magnetic_shock_tube(*grid, direction=d)
+        params, tol, kind = ("XYZ".index(d), 1, 1, 0), 0.0, "magnetic_shock_tube"
+    dev = _device_ic(ini, kind, params)
+    host = make_gpu(ini, st)
+    mag = ini["mode"]["magnet"]
+    for f in FLUID_FIELDS + (MAG_FIELDS if mag else []):
+        a, b = dev.download(f), host.download(f)
+        assert rel_linf(a, b) <= tol, (f, rel_linf(a, b))
+        if tol == 0.0:  # signed zeros too (the y shock's -0.0 momentum)
+            assert np.array_equal(np.signbit(a), np.signbit(b)), f
+    dev.close()
+    host.close()
+
+
+def test_magnetic_shock_tube_lockstep():
+    """MagneticShockTubeInitializer's problem from the device-generated state against
+    the oracle from the host restatement: fade ends along x, 3-cell y and z pencils."""
+    from imogen_amd import iteration
+    ini, st = problems.magnetic_shock_tube(128, 3, 3)
+    orc = make_oracle(ini, st)
+    run = _device_ic(ini, "magnetic_shock_tube", (0, 1, 1, 0))
+    worst = 0.0
+    for it in range(20):
+        orc.iteration()
+        iteration(run)
+        worst = max(worst, compare(run, orc, FLUID_FIELDS + MAG_FIELDS, TOL))
+    print("magnetic shock tube 128x3x3: max relative L-inf over 20 iterations %.3e" % worst,
+          flush=True)
+    run.close()
+    orc.close()
