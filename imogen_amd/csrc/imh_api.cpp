@@ -398,7 +398,9 @@ int imh_create(const imh_config* cfg, imh_ctx** out) {
         hipDeviceProp_t prop;
         if (hipGetDeviceProperties(&prop, cfg->device) == hipSuccess && prop.multiProcessorCount > 0) {
             c->num_cu = prop.multiProcessorCount;
-            c->max_lds = prop.sharedMemPerBlock;
+            // dynamic LDS above 64 KiB would need a per-kernel attribute: the wave-spread
+            // sweep stays within 64 KiB
+            c->max_lds = std::min<size_t>(prop.sharedMemPerBlock, 64 * 1024);
         }
     }
     for (int i = 0; i < IMH_NFIELDS; ++i) CHK(hipMalloc(&c->f[i], bytes), "hipMalloc(field)");

@@ -123,6 +123,39 @@ __device__ __noinline__ T transverse_fade(T r, const T* arr, long long row0, int
     return fade_blend(r, jI, NI, v1, vN, fade, f);
 }
 
+// The ghost of a one-cell transverse shift for the modes whose ghost is an element of
+// the array (circ, const, mirror, flip, zero): the element ghost_values() picks, read
+// directly (no call, one load; none when it is the cell itself), the same value.  Returns
+// false for the modes that need the edge tracker or a blend (trans, wall, fade).
+// row0 is the cell of the line at coordinate 0 along dI; self the line's cell at jI.
+template <typename T>
+__device__ __forceinline__ bool transverse_index_ghost(const T* arr, long long row0, int jI, int NI,
+                                                       long long sI, const GhostSpec& s, T self,
+                                                       T& out) {
+    int src;
+    bool neg = false;
+    switch (s.mode) {
+    case IMH_BC_CIRC: src = s.upper ? 0 : NI - 1; break;
+    case IMH_BC_CONST: src = s.upper ? NI - 1 : 0; break;
+    case IMH_BC_MIRROR:  // lower a[min(1, N-1)], upper a[max(N-2, 0)], DIRECT negated
+        src = s.upper ? (NI - 2 > 0 ? NI - 2 : 0) : (NI - 1 < 1 ? NI - 1 : 1);
+        neg = s.flip_sign != 0;
+        break;
+    case IMH_BC_FLIP: src = s.upper ? NI - 1 : 0; neg = s.flip_sign != 0; break;
+    case IMH_BC_ZERO:
+        if (s.zero_ghost) {
+            out = T(0);
+            return true;
+        }
+        src = s.upper ? NI - 1 : 0;
+        break;
+    default: return false;
+    }
+    const T v = src == jI ? self : arr[row0 + (long long)src * sI];
+    out = neg ? -v : v;
+    return true;
+}
+
 // Transverse shift along dimension dI of global array `arr` at the cell whose
 // coordinate along dI is jI: shift(I,-1) (upper=0) or shift(I,+1) (upper=1), one cell.
 // tracker is this cell's stored edge (used when the cell lies on that face).
@@ -134,9 +167,10 @@ __device__ inline T transverse_shift(const T* arr, long long cell, int jI, int N
     T r;
     bool ghost = s.upper ? (jI >= NI - 1) : (jI <= 0);
     if (!ghost) r = arr[cell + (s.upper ? sI : -sI)];
-    else r = T(transverse_ghost(arr, cell - (long long)jI * sI, NI, sI, s.mode, s.upper,
-                                s.flip_sign, s.zero_ghost, s.active_linear, s.bc_infinity,
-                                tracker));
+    else if (!transverse_index_ghost(arr, cell - (long long)jI * sI, jI, NI, sI, s, arr[cell], r))
+        r = T(transverse_ghost(arr, cell - (long long)jI * sI, NI, sI, s.mode, s.upper,
+                               s.flip_sign, s.zero_ghost, s.active_linear, s.bc_infinity,
+                               tracker));
     if (s.mode == IMH_BC_FADE)
         r = transverse_fade(r, arr, cell - (long long)jI * sI, jI, NI, sI, s.upper, fade, f);
     return r;
@@ -155,9 +189,10 @@ __device__ inline T transverse_shift_pre(const T* arr, long long cell, int jI, i
     T r;
     bool ghost = s.upper ? (jI >= NI - 1) : (jI <= 0);
     if (!ghost) r = nb;
-    else r = T(transverse_ghost(arr, cell - (long long)jI * sI, NI, sI, s.mode, s.upper,
-                                s.flip_sign, s.zero_ghost, s.active_linear, s.bc_infinity,
-                                tracker));
+    else if (!transverse_index_ghost(arr, cell - (long long)jI * sI, jI, NI, sI, s, self, r))
+        r = T(transverse_ghost(arr, cell - (long long)jI * sI, NI, sI, s.mode, s.upper,
+                               s.flip_sign, s.zero_ghost, s.active_linear, s.bc_infinity,
+                               tracker));
     if (s.mode == IMH_BC_FADE)
         r = transverse_fade(r, arr, cell - (long long)jI * sI, jI, NI, sI, s.upper, fade, f);
     return r;
@@ -217,8 +252,10 @@ __device__ __forceinline__ void magnet_advect_body(const MagnetArgs<T>& ap, long
     const long long pen = tile * a.PX + px;
     const bool pen_ok = pen < a.npencil && px < a.PX;
     const int K = (N + a.TL - 1) / a.TL;
-    const int i0 = ty * K;
-    int kc = N - i0;
+    // cells are dealt to the lanes cyclically (cell ty + k * TL): loads, stores and LDS
+    // rows are touched at consecutive addresses by consecutive lanes
+    const int TLc = a.TL;
+    int kc = ty < N ? (N - 1 - ty) / TLc + 1 : 0;
     kc = kc < 0 ? 0 : (kc > K ? K : kc);
     if (!pen_ok) kc = 0;
     const bool ghost_thread = pen_ok && ty == 0;
@@ -288,13 +325,13 @@ __device__ __forceinline__ void magnet_advect_body(const MagnetArgs<T>& ap, long
         for (int k = 0; k < KMAX; ++k)
             if (k < kc)
                 mag_init_face(a.edges, a.eoff, a.modes, a.ndims, dX, obj,
-                              base + (long long)(i0 + k) * stride, a.n0, a.n1, a.n2,
+                              base + (long long)((ty + k * TLc)) * stride, a.n0, a.n1, a.n2,
                               double(vals[k]));
     };
     auto put = [&](T* row, const T (&v)[KMAX]) __attribute__((always_inline)) {
 #pragma unroll
         for (int k = 0; k < KMAX; ++k)
-            if (k < kc) row[2 + i0 + k] = v[k];
+            if (k < kc) row[2 + (ty + k * TLc)] = v[k];
     };
 
     // ---- grid-aligned velocity: (shift_I(p_X,-1) + p_X) ./ (shift_I(rho,-1) + rho) ----
@@ -311,7 +348,7 @@ __device__ __forceinline__ void magnet_advect_body(const MagnetArgs<T>& ap, long
         T m0[KMAX], r0[KMAX], mn[KMAX], rn[KMAX];
 #pragma unroll
         for (int k = 0; k < KMAX; ++k) {
-            const int i = i0 + k < N - 1 ? i0 + k : N - 1;
+            const int i = (ty + k * TLc) < N - 1 ? (ty + k * TLc) : N - 1;
             const long long cell = base + (long long)i * stride;
             m0[k] = gmem(a.mom)[cell];
             r0[k] = gmem(a.rho)[cell];
@@ -327,7 +364,7 @@ __device__ __forceinline__ void magnet_advect_body(const MagnetArgs<T>& ap, long
             T sm_v = I_enabled ? mn[k] : m0[k];
             T sr_v = I_enabled ? rn[k] : r0[k];
             if (slow && k < kc) {
-                long long cell = base + (long long)(i0 + k) * stride;
+                long long cell = base + (long long)((ty + k * TLc)) * stride;
                 double* tm = nullptr;
                 double* tr = nullptr;
                 double em = 0, er = 0;
@@ -335,8 +372,8 @@ __device__ __forceinline__ void magnet_advect_body(const MagnetArgs<T>& ap, long
                 bool trkI = (mIlo == IMH_BC_TRANS || mIlo == IMH_BC_WALL) &&
                             jI == 0 && I_enabled;
                 if (trkI) {
-                    fI = face_index(nn, dI, dX == 0 ? i0 + k : pc[0], dX == 1 ? i0 + k : pc[1],
-                                    dX == 2 ? i0 + k : pc[2]);
+                    fI = face_index(nn, dI, dX == 0 ? (ty + k * TLc) : pc[0], dX == 1 ? (ty + k * TLc) : pc[1],
+                                    dX == 2 ? (ty + k * TLc) : pc[2]);
                     em = a.edges[a.eoff[((OBJ_PRIM + Xc) * 2 + 0) * 3 + dI] + fI];
                     er = a.edges[a.eoff[((OBJ_PRIM + 0) * 2 + 0) * 3 + dI] + fI];
                     tm = &em;
@@ -371,7 +408,7 @@ __device__ __forceinline__ void magnet_advect_body(const MagnetArgs<T>& ap, long
 #pragma unroll
     for (int k = 0; k < KMAX; ++k) {
         if (k < kc) {
-            int i = i0 + k;
+            int i = (ty + k * TLc);
             T s1 = lds_shift(row0, N, i, -1, &s_g[px][0], mlo, a.fade, f);
             T s2 = lds_shift(row0, N, i, 1, &s_g[px][1], mhi, a.fade, f);
             vg[k] = T(0.25) * (s1 + s2 + T(2) * vg[k]);
@@ -387,7 +424,7 @@ __device__ __forceinline__ void magnet_advect_body(const MagnetArgs<T>& ap, long
 #pragma unroll
     for (int k = 0; k < KMAX; ++k) {
         if (k < kc) {
-            T s = lds_shift(row1, N, i0 + k, 1, &s_g[px][2], mhi, a.fade, f);
+            T s = lds_shift(row1, N, (ty + k * TLc), 1, &s_g[px][2], mhi, a.fade, f);
             vf[k] = (vg[k] + s < T(0)) ? T(1) : T(0);
         } else {
             vf[k] = T(0);
@@ -409,7 +446,7 @@ __device__ __forceinline__ void magnet_advect_body(const MagnetArgs<T>& ap, long
 #pragma unroll
     for (int k = 0; k < KMAX; ++k) {
         if (k < kc) {
-            T s = lds_shift(row0, N, i0 + k, 1, &s_g[px][3], mhi, a.fade, f);
+            T s = lds_shift(row0, N, (ty + k * TLc), 1, &s_g[px][3], mhi, a.fade, f);
             w[k] = w[k] * (T(1) - vf[k]) + s * vf[k];
         }
     }
@@ -421,7 +458,7 @@ __device__ __forceinline__ void magnet_advect_body(const MagnetArgs<T>& ap, long
 #pragma unroll
     for (int k = 0; k < KMAX; ++k) {
         if (k < kc) {
-            T s = lds_shift(row1, N, i0 + k, -1, &s_g[px][4], mlo, a.fade, f);
+            T s = lds_shift(row1, N, (ty + k * TLc), -1, &s_g[px][4], mlo, a.fade, f);
             st[k] = bI[k] - T(ff) * (w[k] - s);
         } else {
             st[k] = T(0);
@@ -449,7 +486,7 @@ __device__ __forceinline__ void magnet_advect_body(const MagnetArgs<T>& ap, long
 #pragma unroll
     for (int k = 0; k < KMAX; ++k) {
         if (k < kc) {
-            int i = i0 + k;
+            int i = (ty + k * TLc);
             T a1 = lds_shift(row0, N, i, 1, &s_g[px][0], mhi, a.fade, f);
             F[k] = w[k] * (T(1) - vf[k]) + a1 * vf[k];
         } else {
@@ -464,7 +501,7 @@ __device__ __forceinline__ void magnet_advect_body(const MagnetArgs<T>& ap, long
 #pragma unroll
     for (int k = 0; k < KMAX; ++k) {
         if (k < kc) {
-            int i = i0 + k;
+            int i = (ty + k * TLc);
             T b1 = lds_shift(row0, N, i, 1, &s_g[px][1], mhi, a.fade, f);
             T b2 = lds_shift(row0, N, i, 2, &s_g[px][2], mhi, a.fade, f);
             T dR = b1 - (vf[k] * b2 + (T(1) - vf[k]) * w[k]);
@@ -483,8 +520,8 @@ __device__ __forceinline__ void magnet_advect_body(const MagnetArgs<T>& ap, long
 #pragma unroll
     for (int k = 0; k < KMAX; ++k) {
         if (k < kc) {
-            long long cell = base + (long long)(i0 + k) * stride;
-            T s = lds_shift(row1, N, i0 + k, -1, &s_g[px][6], mlo, a.fade, f);
+            long long cell = base + (long long)((ty + k * TLc)) * stride;
+            T s = lds_shift(row1, N, (ty + k * TLc), -1, &s_g[px][6], mlo, a.fade, f);
             gmem(a.bI)[cell] = bI[k] - T(ff2) * (F[k] - s);
             gmem(a.flux)[cell] = F[k];
         }
@@ -527,8 +564,10 @@ __device__ __forceinline__ void magnet_ct_body(const MagnetArgs<T>& ap, long lon
     const long long pen = tile * a.PX + px;
     const bool pen_ok = pen < a.npencil && px < a.PX;
     const int K = (N + a.TL - 1) / a.TL;
-    const int i0 = ty * K;
-    int kc = N - i0;
+    // cells are dealt to the lanes cyclically (cell ty + k * TL): loads, stores and LDS
+    // rows are touched at consecutive addresses by consecutive lanes
+    const int TLc = a.TL;
+    int kc = ty < N ? (N - 1 - ty) / TLc + 1 : 0;
     kc = kc < 0 ? 0 : (kc > K ? K : kc);
     if (!pen_ok) kc = 0;
     const bool ghost_thread = pen_ok && ty == 0;
@@ -559,7 +598,7 @@ __device__ __forceinline__ void magnet_ct_body(const MagnetArgs<T>& ap, long lon
     T f0[KMAX], fn[KMAX], bx0[KMAX];
 #pragma unroll
     for (int k = 0; k < KMAX; ++k) {
-        const int i = i0 + k < N - 1 ? i0 + k : N - 1;
+        const int i = (ty + k * TLc) < N - 1 ? (ty + k * TLc) : N - 1;
         const long long cell = base + (long long)i * stride;
         f0[k] = gmem(a.flux)[cell];
         fn[k] = gmem(a.flux)[cell + nbo];
@@ -570,14 +609,14 @@ __device__ __forceinline__ void magnet_ct_body(const MagnetArgs<T>& ap, long lon
     for (int k = 0; k < KMAX; ++k) {
         T s = I_enabled ? fn[k] : f0[k];
         if (slow && k < kc) {
-            long long cell = base + (long long)(i0 + k) * stride;
+            long long cell = base + (long long)((ty + k * TLc)) * stride;
             double et = 0;
             double* tp = nullptr;
             long long fI = 0;
             bool onface = trkI && jI == NI - 1;
             if (onface) {
-                fI = face_index(nn, dI, dX == 0 ? i0 + k : pc[0], dX == 1 ? i0 + k : pc[1],
-                                dX == 2 ? i0 + k : pc[2]);
+                fI = face_index(nn, dI, dX == 0 ? (ty + k * TLc) : pc[0], dX == 1 ? (ty + k * TLc) : pc[1],
+                                dX == 2 ? (ty + k * TLc) : pc[2]);
                 et = a.edges[a.eoff[(oFL * 2 + 1) * 3 + dI] + fI];
                 tp = &et;
             }
@@ -589,7 +628,7 @@ __device__ __forceinline__ void magnet_ct_body(const MagnetArgs<T>& ap, long lon
     }
 #pragma unroll
     for (int k = 0; k < KMAX; ++k)
-        if (k < kc) row[2 + i0 + k] = G[k];
+        if (k < kc) row[2 + (ty + k * TLc)] = G[k];
     __syncthreads();
     if (ghost_thread) {
         double tr = 0;
@@ -606,8 +645,8 @@ __device__ __forceinline__ void magnet_ct_body(const MagnetArgs<T>& ap, long lon
 #pragma unroll
     for (int k = 0; k < KMAX; ++k) {
         if (k < kc) {
-            long long cell = base + (long long)(i0 + k) * stride;
-            T h = lds_shift(row, N, i0 + k, -1, &s_g[px][0], mlo, a.fade, f);
+            long long cell = base + (long long)((ty + k * TLc)) * stride;
+            T h = lds_shift(row, N, (ty + k * TLc), -1, &s_g[px][0], mlo, a.fade, f);
             gmem(a.bX)[cell] = bx0[k] - T(ffI) * h;
         }
     }
