@@ -39,7 +39,7 @@ class FluxLimiterEnum:
 
 # Mode codes of the C ABI (include/imogen_hip.h).
 BC_CODES = {"circ": 0, "const": 1, "trans": 2, "fade": 3, "mirror": 4, "flip": 5,
-            "wall": 6, "zero": 7}
+            "wall": 6, "zero": 7, "wormhole": 8}
 LIMITER_CODES = {FluxLimiterEnum.VAN_LEER: 0, FluxLimiterEnum.SUPERBEE: 1,
                  FluxLimiterEnum.MINMOD: 2}
 

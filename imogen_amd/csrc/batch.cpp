@@ -363,7 +363,11 @@ static int build_plans(imh_batch* b) {
                             }
                             g->bytes += double(10 + (c->mhd_kernel ? 3 : 0)) * c->ncell * es;
                             const long long per = col ? fluid_column_pencils() : ft.PX;
-                            long long ntile = (c->nface[X - 1] + per - 1) / per;
+                            // pencil slots: pairs with a lower wormhole boundary
+                            const long long nslot = c->cfg.bc[0][X - 1] == IMH_BC_WORMHOLE
+                                                        ? worm_slots(c->n, X - 1)
+                                                        : c->nface[X - 1];
+                            long long ntile = (nslot + per - 1) / per;
                             for (long long tt = 0; tt < ntile; ++tt)
                                 g->tl.push_back({c->n[X - 1], {i * 3 + (X - 1), 0, tt}});
                         } else if (!fluid_op && c->cfg.magnet_active) {

@@ -306,10 +306,23 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
     const int ty = a.xmajor ? tid % TL : tid / PX;
     const int dstep = a.xmajor ? 1 : PX;
     const long long pen0 = tile * PX;
-    const bool pen_ok = pen0 + px < a.npencil;
-    const long long pen = pen_ok ? pen0 + px : a.npencil - 1;  // padding pencils read the last one
     const int d = a.d;
     const int nn[3] = {a.n0, a.n1, a.n2};
+    // face of slot p of the tile, -1 for padding (with a lower wormhole boundary the slots
+    // hold pencil pairs, worm_face)
+    auto slot_face = [&](int p) __attribute__((always_inline)) -> long long {
+        const long long sl = pen0 + p;
+        if (sl >= a.npencil) return -1;
+        return a.worm ? worm_face(nn, d, sl) : sl;
+    };
+    // the slot whose cell 0 gives slot p's lower wormhole ghosts
+    auto partner_slot = [&](int p) __attribute__((always_inline)) -> int {
+        return slot_face(p ^ 1) >= 0 ? (p ^ 1) : p;
+    };
+    const long long pface = slot_face(px);
+    const bool pen_ok = pface >= 0;
+    // padding pencils read a real one (the last, or face 0 with pairs)
+    const long long pen = pen_ok ? pface : (a.worm ? 0 : a.npencil - 1);
     const int N = sel3i(d, a.n0, a.n1, a.n2);
     const int i0 = ty * K;
     const bool has_left = ty > 0;
@@ -353,8 +366,8 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
         for (int e = tid; e < PX * 30; e += BS) {
             int p = e / 30, r = e % 30;
             int side = r / 15, role = (r % 15) / 5, v = r % 5;
-            long long pp = pen0 + p;
-            if (pp >= a.npencil || (side ? !trk_hi : !trk_lo)) continue;
+            const long long pp = slot_face(p);
+            if (pp < 0 || (side ? !trk_hi : !trk_lo)) continue;
             int obj = side ? (role == 0 ? OBJ_SFL : (role == 1 ? OBJ_FR : OBJ_FL))
                            : (role == 0 ? OBJ_SFR : (role == 1 ? OBJ_FR : OBJ_FL));
             s_trk[p][side * 16 + role * 5 + v] = a.edges[a.eoff[((obj + v) * 2 + side) * 3 + d] + pp];
@@ -425,7 +438,7 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
             const int v = v0 + gv;
             const int jj = r - gv * J;
             const int j = (J == 4) ? jj : (jj ? 3 : 0);  // 0 FR lo, 1 FL lo, 2 FR hi, 3 FL hi
-            if (pen0 + p >= a.npencil) continue;
+            if (slot_face(p) < 0) continue;
             const int which = j & 1;           // 0 fluxR, 1 fluxL
             const int upper = j >> 1;
             const int mode = upper ? mhi : mlo;
@@ -440,6 +453,13 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
             const int role = kind == 0 ? 0 : 1 + which;
             const int slot = upper * 16 + role * 5 + v;
             const double* e = s_ends[which][gv][p];
+            if (mode == IMH_BC_WORMHOLE && !upper) {
+                // wormhole_shift.m:21-57: cell 0 of the partner pencil, negated for the
+                // momentum along the sweep (isVecNorm)
+                const double w = s_ends[which][gv][partner_slot(p)][0];
+                s_gh[p][gv][j] = (v == X) ? -w : w;
+                continue;
+            }
             if (GEN && a.init_edges && kind < 2) s_trk[p][slot] = upper ? e[5] : e[0];
             double gval;
             if (mode == IMH_BC_CIRC) gval = upper ? e[0] : e[5];
@@ -649,7 +669,7 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
             }
         }
         __syncthreads();
-        if (trk_any && tid < PX && pen0 + tid < a.npencil) {
+        if (trk_any && tid < PX && slot_face(tid) >= 0) {
             const int p = tid;
             for (int up = 0; up < 2; ++up) {
                 const int mode = up ? mhi : mlo;
@@ -657,7 +677,7 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
                 double* tp[8];
                 for (int r = 0; r < 8; ++r) {
                     const int o = r < 5 ? obj + r : OBJ_CMAG + (r - 5);
-                    tp[r] = a.edges + a.eoff[(o * 2 + up) * 3 + d] + pen0 + p;
+                    tp[r] = a.edges + a.eoff[(o * 2 + up) * 3 + d] + slot_face(p);
                 }
                 csw_side_ghosts(mode, up, d, N, a.active_lin, f, g, tp, xe + p * XE_ENDS,
                                 xe + PXMAX * XE_ENDS + p * XE_OUT + up * 5);
@@ -742,10 +762,10 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
                 }
             }
             __syncthreads();
-            if (tid < PX && pen0 + tid < a.npencil) {
+            if (tid < PX && slot_face(tid) >= 0) {
                 const double* e = s_vend[tid];
                 const bool trk = (mhi == IMH_BC_TRANS || mhi == IMH_BC_WALL);
-                double* tp = trk ? a.edges + a.eoff[((obj + X) * 2 + 1) * 3 + d] + pen0 + tid : nullptr;
+                double* tp = trk ? a.edges + a.eoff[((obj + X) * 2 + 1) * 3 + d] + slot_face(tid) : nullptr;
                 double tv = trk ? (first ? e[5] : *tp) : 0.0;
                 GhostSpec sp;
                 sp.mode = mhi;
@@ -970,8 +990,8 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
         for (int e = tid; e < PX * 30; e += BS) {
             int p = e / 30, r = e % 30;
             int side = r / 15, role = (r % 15) / 5, v = r % 5;
-            long long pp = pen0 + p;
-            if (pp >= a.npencil || (side ? !trk_hi : !trk_lo)) continue;
+            const long long pp = slot_face(p);
+            if (pp < 0 || (side ? !trk_hi : !trk_lo)) continue;
             int obj = side ? (role == 0 ? OBJ_SFL : (role == 1 ? OBJ_FR : OBJ_FL))
                            : (role == 0 ? OBJ_SFR : (role == 1 ? OBJ_FR : OBJ_FL));
             a.edges[a.eoff[((obj + v) * 2 + side) * 3 + d] + pp] = s_trk[p][side * 16 + role * 5 + v];

@@ -75,7 +75,7 @@ int validate_config(const imh_config* c) {
     for (int s = 0; s < 2; ++s)
         for (int d = 0; d < 3; ++d) {
             int m = c->bc[s][d];
-            if (m < IMH_BC_CIRC || m > IMH_BC_ZERO)
+            if (m < IMH_BC_CIRC || m > IMH_BC_WORMHOLE)
                 return fail(IMH_EINVAL, "Imogen:UnknownType: Unknown BC mode %d", m);
             if (d >= ndims) continue;  // every dim <= ndims is shifted (MagnetArray.m:36-37,103)
             if (m == IMH_BC_FADE &&
@@ -87,6 +87,12 @@ int validate_config(const imh_config* c) {
                 return fail(IMH_EINVAL, "transparent BC needs bcInfinity > 3 (pchip nodes)");
             if (m == IMH_BC_MIRROR && c->grid[d] < 2)
                 return fail(IMH_ENOTSUP, "mirror BC needs >= 2 cells along dim %d", d + 1);
+            if (m == IMH_BC_WORMHOLE && ndims < 3)
+                return fail(IMH_ENOTSUP, "wormhole BC along dim %d needs a 3-D grid "
+                            "(wormhole_shift.m indexes a third dimension)", d + 1);
+            if (m == IMH_BC_WORMHOLE && c->magnet_active)
+                return fail(IMH_ENOTSUP, "wormhole BC with the magnetic field evolved: the "
+                            "magnet sweeps take no wormhole ghosts");
         }
     for (int d = 0; d < 3; ++d)
         if (c->limiter[d] < 0 || c->limiter[d] > 2)
@@ -448,6 +454,27 @@ int imh_create(const imh_config* cfg, imh_ctx** out) {
             return fail(IMH_ENOTSUP, "pencils of %d cells along dim %d exceed the longest "
                         "compiled sweep tile (%d cells at this precision)", cfg->grid[d], d + 1, maxn);
         }
+        // a lower wormhole boundary pairs pencils inside a tile: long pencils take two
+        // pencils of 256 threads per 512-thread block (more cells per thread)
+        if (c->n[d] >= 3 && d < c->ndims && cfg->bc[0][d] == IMH_BC_WORMHOLE &&
+            (c->ftile[d].PX & 1)) {
+            bool ok = false;
+            for (int k = (c->n[d] + 255) / 256; k <= 64 && !ok; ++k) {
+                if (!fluid_variant_exists((int)es, 512, k)) continue;
+                const TileOverride ov = {1, 512, k, 2};
+                const FluidTile t = choose_fluid_tile(c->n[d], d, (int)es, &ov);
+                if (t.K <= t.kmax) {
+                    c->ftile[d] = t;
+                    ok = true;
+                }
+            }
+            if (!ok) {
+                ctx_free(c);
+                return fail(IMH_ENOTSUP, "wormhole BC along dim %d: pencils of %d cells need "
+                            "a sweep tile of their own, and a wormhole pairs two pencils per "
+                            "tile", d + 1, cfg->grid[d]);
+            }
+        }
     }
     c->host_iteration = 1;
 #undef CHK
@@ -678,7 +705,8 @@ FluidArgs<T> make_fluid_args(const imh_ctx* c, int d, int force) {
     a.TL = t.TL;
     a.K = t.K;
     a.xmajor = t.xmajor;
-    a.npencil = c->nface[d];
+    a.worm = c->cfg.bc[0][d] == IMH_BC_WORMHOLE;
+    a.npencil = a.worm ? worm_slots(c->n, d) : c->nface[d];
     a.bc_inf = c->cfg.bc_infinity;
     a.xcd_remap = 1;
     a.force = force;
@@ -1030,7 +1058,8 @@ int imh_set_gravity(imh_ctx* c, const double* potential, size_t n, double mass_t
                 // 1-D grid's y) every shift rule of the reference indexes element 0 and
                 // MATLAB errors; on two cells only circ / const / flip / zero stay inside
                 if (c->n[d] == 1 || (c->n[d] == 2 && !(m == IMH_BC_CIRC || m == IMH_BC_CONST ||
-                                                       m == IMH_BC_FLIP || m == IMH_BC_ZERO)))
+                                                       m == IMH_BC_FLIP || m == IMH_BC_ZERO ||
+                                                       m == IMH_BC_WORMHOLE)))
                     return fail(IMH_ENOTSUP, "gravity along dimension %d of %d cells: the "
                                 "reference's shift by two cells indexes outside the array",
                                 d + 1, c->n[d]);
@@ -1116,10 +1145,14 @@ int imh_set_viscosity(imh_ctx* c, int type, double linear, double quadratic) {
     if (type < 0 || type > 2) return fail(IMH_EINVAL, "viscosity type %d", type);
     if (type == 2)  // the fade blend reaches into every one of its shifted arrays
         for (int d = 0; d < c->ndims; ++d)
-            for (int s = 0; s < 2; ++s)
+            for (int s = 0; s < 2; ++s) {
                 if (c->cfg.bc[s][d] == IMH_BC_FADE)
                     return fail(IMH_ENOTSUP, "the Caramana-Shashkov-Whalen viscosity with a "
                                 "fade boundary is not supported");
+                if (c->cfg.bc[s][d] == IMH_BC_WORMHOLE)
+                    return fail(IMH_ENOTSUP, "the Caramana-Shashkov-Whalen viscosity with a "
+                                "wormhole boundary is not supported");
+            }
     hipError_t e = hipStreamSynchronize(c->st);
     if (e != hipSuccess) return hip_fail(e, "sync");
     drop_graphs(c);
@@ -1584,7 +1617,8 @@ int imh_set_fluid_tile(imh_ctx* c, int X, int bs, int kmax, int px) {
     TileOverride ov = {1, bs, kmax, px};
     FluidTile t = choose_fluid_tile(c->n[X - 1], X - 1, (int)esize(c), &ov);
     if (t.K > t.kmax || !fluid_variant_exists((int)esize(c), t.bs, t.kmax) || (t.bs % t.PX) != 0 ||
-        t.PX > 32 || (t.PX & (t.PX - 1)) != 0)
+        t.PX > 32 || (t.PX & (t.PX - 1)) != 0 ||
+        (c->cfg.bc[0][X - 1] == IMH_BC_WORMHOLE && (t.PX & 1)))
         return fail(IMH_EINVAL, "tile (bs=%d kmax=%d px=%d) does not fit pencils of %d cells",
                     bs, kmax, px, c->n[X - 1]);
     c->ftile[X - 1] = t;

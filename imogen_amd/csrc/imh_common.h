@@ -403,6 +403,51 @@ __device__ __forceinline__ IMH_GLOBAL_AS T* gmem(T* p) {
 }
 
 // Face index of cell (i,j,k) for dim d (same conventions).
+// Pencil slots of a sweep along d whose lower boundary is a wormhole (wormhole_shift.m):
+// the lower ghosts of the pencil at next-dimension coordinate c (next = x->y, y->z, z->x)
+// come from the pencil at N_next-1-c with the same previous-dimension coordinate, so the
+// two share a tile.  Slots 2P and 2P+1 hold such a pair; the middle pencil of an odd
+// N_next pairs with itself and its odd slot is padding.  Pairs run along the x-fastest
+// coordinate of the face, so neighbouring pairs are neighbours in memory.
+__host__ __device__ inline long long worm_slots(const int n[3], int d) {
+    const int nq = n[(d + 1) % 3], np = n[(d + 2) % 3];
+    return 2LL * ((nq + 1) / 2) * np;
+}
+// Face index (face_index order) of slot s, or -1 for a padding slot.
+__host__ __device__ inline long long worm_face(const int n[3], int d, long long s) {
+    const int nq = n[(d + 1) % 3], np = n[(d + 2) % 3];
+    const unsigned h = unsigned(nq + 1) / 2;
+    const unsigned P = unsigned(s >> 1);
+    unsigned cq, cp;
+    if (d != 1) {  // the next dimension is the face's fast one (d = 0: y, d = 2: x)
+        cq = P % h;
+        cp = P / h;
+    } else {       // d = 1: the next dimension (z) is the slow one, x the fast one
+        cp = P % unsigned(np);
+        cq = P / unsigned(np);
+    }
+    if (cp >= unsigned(np) || cq >= h) return -1;
+    if (s & 1) {
+        const unsigned m = unsigned(nq) - 1 - cq;
+        if (m == cq) return -1;
+        cq = m;
+    }
+    return d != 1 ? (long long)cq + (long long)nq * cp : (long long)cp + (long long)np * cq;
+}
+
+// Face index of the pencil whose lower wormhole ghosts face f takes: the next-dimension
+// coordinate mirrored, the previous one kept.
+__host__ __device__ inline long long worm_partner_face(const int n[3], int d, long long f) {
+    const long long nq = n[(d + 1) % 3];
+    if (d != 1) {
+        const long long cq = f % nq, cp = f / nq;
+        return (nq - 1 - cq) + nq * cp;
+    }
+    const long long np = n[0];
+    const long long cp = f % np, cq = f / np;
+    return cp + np * (nq - 1 - cq);
+}
+
 __host__ __device__ inline long long face_index(const int n[3], int d, int i, int j, int k) {
     if (d == 0) return j + (long long)n[1] * k;
     if (d == 1) return i + (long long)n[0] * k;

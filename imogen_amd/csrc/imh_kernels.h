@@ -40,7 +40,8 @@ struct FluidArgs {
     double gamma, minmass, dx;
     int init_edges;          // first assignment of the flux objects (first fluid sweep)
     int PX, TL, K, xmajor;   // tile: PX pencils x TL threads, K cells per thread
-    long long npencil;
+    long long npencil;       // pencil slots (worm_slots with a lower wormhole boundary)
+    int worm;                // lower boundary along d is a wormhole: slots hold pencil pairs
     int bc_inf;
     int xcd_remap;
     int force;               // run even when the loop's running flag is off (direct calls)
@@ -172,6 +173,8 @@ bool wave_tile(int N, int d, int esz, size_t max_lds, int* bs, int* px, int* tl,
 // whether the boundaries of dim d allow the wave-spread sweep (both ends circular or
 // neither)
 inline bool wave_bc_ok(int modes, int d) {
+    if (mode_at(modes, 0, d) == IMH_BC_WORMHOLE || mode_at(modes, 1, d) == IMH_BC_WORMHOLE)
+        return false;  // pencil pairs: the general sweep
     return (mode_at(modes, 0, d) == IMH_BC_CIRC) == (mode_at(modes, 1, d) == IMH_BC_CIRC);
 }
 template <typename T>

@@ -56,6 +56,16 @@ __global__ void __launch_bounds__(256) source_ghost_kernel(SourceArgs a) {
     ghost_values(up, e, N, 1, tr[1], g[2], unused);
     ghost_values(lo, e, N, 1, tr[0], g[3], unused);
     ghost_values(lo, e, N, 2, tr[0], g[4], g[5]);
+    if (a.modes[0][d] == IMH_BC_WORMHOLE) {
+        // wormhole_shift.m:21-57 (the potential has no vector component): shift(-1) at 0
+        // takes cell 0 of the partner pencil, shift(-2) at 0 and 1 its cells 1 and 0
+        long long pb, ps;
+        pencil_base(a.n, d, worm_partner_face(a.n, d, f), pb, ps);
+        const double* q = a.phi + pb;
+        g[3] = q[0];
+        g[4] = q[(long long)(N > 1 ? 1 : 0) * ps];
+        g[5] = q[0];
+    }
     double* out = a.ghosts + 6 * t;
     for (int k = 0; k < 6; ++k) out[k] = g[k];
 }

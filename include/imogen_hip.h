@@ -45,6 +45,12 @@ extern "C" {
 #define IMH_BC_FLIP 5
 #define IMH_BC_WALL 6
 #define IMH_BC_ZERO 7
+/* shifting/wormhole_shift.m: the lower ghosts of a pencil are the first cells of the pencil
+   whose next-dimension coordinate is mirrored (x->y, y->z, z->x), in reverse order and
+   negated for the component along the shift; the upper side shifts as const.  3-D grids
+   only (the reference indexes a third dimension); fluid sweeps and the gravity potential
+   (a magnet or artificial viscosity with it is refused). */
+#define IMH_BC_WORMHOLE 8
 
 /* ---- flux limiters (enums/FluxLimiterEnum.m) ---------------------------------------- */
 #define IMH_LIM_VANLEER 0
@@ -198,7 +204,7 @@ int imh_memory_bytes(imh_ctx *ctx, size_t *bytes);
 int imh_memory_estimate(const imh_config *cfg, size_t *bytes);
 
 /* ---- initial conditions on the device ---------------------------------------------------
- * No counterpart in the reference (its experiment/*Initializer.m classes build the state
+ * No counterpart in the reference (its experiment/ initializer classes build the state
  * on the host): the state of one of these problems is generated in device memory, with
  * the constructor side effects of imh_upload_state (edge stores from the initial arrays,
  * isZero, cell-centred B, a new run's time state), so a simulation starts without a host
@@ -216,7 +222,7 @@ int imh_memory_estimate(const imh_config *cfg, size_t *bytes);
 #define IMH_IC_SEDOV 3
 #define IMH_IC_RIEMANN 4
 #define IMH_IC_KELVIN_HELMHOLTZ 5
-/* The reference initializers themselves (experiment/*Initializer.m), cell for cell:
+/* The reference initializers themselves (the experiment/ initializer classes), cell for cell:
  *   ORSZAG_TANG_REF  none: OrszagTangVortexInitializer.m:72-90 (sines of 2 pi i / (N - 1)
  *                    over cell indices; the run's spacing is 1 ./ grid)
  *   SEDOV_REF        none: SedovTaylorBlastWaveInitializer.m:77-86 (needs an odd cell

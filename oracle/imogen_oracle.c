@@ -292,6 +292,7 @@ static void shift_pencil(const shift_ctx *c, const obj_t *o, int d, int upper, i
     case ORC_BC_WALL:                  /* wall_shift.m:12-33 */
     case ORC_BC_FADE:                  /* fade_shift.m:21 then blend */
     case ORC_BC_ZERO:                  /* zero_shift.m */
+    case ORC_BC_WORMHOLE:              /* wormhole_shift.m:61: constant_shift first */
         for (long i = 0; i < N; ++i) {
             long src = upper ? (i + m < N ? i + m : N - 1) : (i - m >= 0 ? i - m : 0);
             R(i) = A(src);
@@ -385,6 +386,30 @@ static void shift_pencil(const shift_ctx *c, const obj_t *o, int d, int upper, i
 #undef R
 }
 
+/* wormhole_shift.m:21-57 (n < 0, after the constant shift): with P = previousDir and
+ * Q = nextDir of D ([3 1 2] and [2 3 1]),
+ *   result(amt:-1:1, all of P, N_Q:-1:1) = isVecNorm * array(1:amt, all of P, 1:N_Q)
+ * along D, i.e. cell t < m of the pencil at Q-coordinate q takes cell m-1-t of the
+ * unshifted pencil at N_Q-1-q (same P-coordinate), negated when the object's component
+ * is D.  Written with explicit coordinates (x fastest in memory). */
+static void wormhole_lower(const orc_sim *s, const obj_t *o, int d, int m, double *out) {
+    const int q = (d + 1) % 3; /* nextDir(D) - 1 */
+    const double sgn = (o->component == d + 1) ? -1.0 : 1.0;
+    const long sx = 1, sy = s->n[0], sz = (long)s->n[0] * s->n[1];
+    const long st[3] = {sx, sy, sz};
+    for (long k = 0; k < s->n[2]; ++k)
+        for (long j = 0; j < s->n[1]; ++j)
+            for (long i = 0; i < s->n[0]; ++i) {
+                long c[3] = {i, j, k};
+                if (c[d] >= m) continue;
+                long src[3] = {i, j, k};
+                src[d] = m - 1 - c[d];
+                src[q] = s->n[q] - 1 - c[q];
+                out[c[0] * st[0] + c[1] * st[1] + c[2] * st[2]] =
+                    sgn * o->a[src[0] * st[0] + src[1] * st[1] + src[2] * st[2]];
+            }
+}
+
 /* result = obj.shift(D, n)  (D 1-based).  Writes a full-size array into out. */
 static void do_shift(orc_sim *s, obj_t *o, int D, int n, double *out) {
     int d = D - 1;
@@ -407,6 +432,7 @@ static void do_shift(orc_sim *s, obj_t *o, int D, int n, double *out) {
         shift_pencil(&c, o, d, upper, m, o->a + base, out + base, N, stride,
                      edge ? &edge[f] : NULL, active_linear);
     }
+    if (!upper && s->cfg.bc[0][d] == ORC_BC_WORMHOLE) wormhole_lower(s, o, d, m, out);
 }
 
 /* ------------------------------------------------------------------------------------ */
@@ -1012,7 +1038,8 @@ int orc_set_gravity(orc_sim *s, const double *potential, double mass_threshold) 
              * maps of circ / const / flip / zero stay inside the array */
             const int m = s->cfg.bc[side][d];
             if (s->n[d] == 1 || (s->n[d] == 2 && !(m == ORC_BC_CIRC || m == ORC_BC_CONST ||
-                                                  m == ORC_BC_FLIP || m == ORC_BC_ZERO))) {
+                                                  m == ORC_BC_FLIP || m == ORC_BC_ZERO ||
+                                                  m == ORC_BC_WORMHOLE))) {
                 snprintf(s->err, sizeof(s->err),
                          "gravity along dimension %d of %d cells: the reference's shift by two "
                          "cells indexes outside the array", d + 1, s->n[d]);
@@ -1211,7 +1238,7 @@ orc_sim *orc_create(const orc_config *cfg, char *err, int errlen) {
     for (int sd = 0; sd < 2; ++sd)
         for (int d = 0; d < 3; ++d) {
             int m = cfg->bc[sd][d];
-            if (m < ORC_BC_CIRC || m > ORC_BC_ZERO) FAIL("unknown BC mode %d", m);
+            if (m < ORC_BC_CIRC || m > ORC_BC_WORMHOLE) FAIL("unknown BC mode %d", m);
             /* Every dim <= ndims is shifted at least once: MagnetArray's constructor
              * calls updateCellCentered, which shifts mag(k) along k (MagnetArray.m:36-37,103). */
             if (d >= ndims) continue;
@@ -1221,6 +1248,11 @@ orc_sim *orc_create(const orc_config *cfg, char *err, int errlen) {
                 FAIL("transparent BC needs bcInfinity > 3");
             if (m == ORC_BC_MIRROR && cfg->grid[d] < 2)
                 FAIL("mirror BC needs >= 2 cells along dim %d", d + 1);
+            /* wormhole_shift.m:16-27 indexes N(nextDir) and index{previousDir}: a shifted
+             * dimension with this mode needs a third dimension in the array */
+            if (m == ORC_BC_WORMHOLE && ndims < 3)
+                FAIL("wormhole BC along dim %d needs a 3-D grid (wormhole_shift.m indexes a "
+                     "third dimension)", d + 1);
         }
     for (int d = 0; d < 3; ++d)
         if (cfg->limiter[d] < 0 || cfg->limiter[d] > 2) FAIL("unknown limiter %d", cfg->limiter[d]);

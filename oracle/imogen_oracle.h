@@ -63,7 +63,8 @@ extern "C" {
 /* Boundary-condition modes (enums/ENUM.m:23-32). */
 enum {
     ORC_BC_CIRC = 0, ORC_BC_CONST = 1, ORC_BC_TRANS = 2, ORC_BC_FADE = 3,
-    ORC_BC_MIRROR = 4, ORC_BC_FLIP = 5, ORC_BC_WALL = 6, ORC_BC_ZERO = 7
+    ORC_BC_MIRROR = 4, ORC_BC_FLIP = 5, ORC_BC_WALL = 6, ORC_BC_ZERO = 7,
+    ORC_BC_WORMHOLE = 8
 };
 
 /* Flux limiters (enums/FluxLimiterEnum.m). */

@@ -15,7 +15,8 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
-BC = {"circ": 0, "const": 1, "trans": 2, "fade": 3, "mirror": 4, "flip": 5, "wall": 6, "zero": 7}
+BC = {"circ": 0, "const": 1, "trans": 2, "fade": 3, "mirror": 4, "flip": 5, "wall": 6, "zero": 7,
+      "wormhole": 8}
 LIMITER = {"vanleer": 0, "superbee": 1, "minmod": 2}
 FIELDS = ["mass", "momx", "momy", "momz", "ener", "magx", "magy", "magz",
           "cmagx", "cmagy", "cmagz"]

@@ -54,6 +54,14 @@ def test_create_rejects_bad_configs_before_touching_the_device():
                           bc=[["fade"] * 3, ["fade"] * 3])     # fade needs >= 20 cells
     assert L.imh_create(ctypes.byref(make_imh_config(rc)), ctypes.byref(h)) == -6
     assert b"fade" in L.imh_last_error()
+    rc = config.RunConfig(grid=(10, 10, 1), dgrid=(1, 1, 1),
+                          bc=[["wormhole", "circ", "circ"], ["const", "circ", "circ"]])
+    assert L.imh_create(ctypes.byref(make_imh_config(rc)), ctypes.byref(h)) == -6
+    assert b"3-D grid" in L.imh_last_error()   # wormhole_shift.m indexes a third dimension
+    rc = config.RunConfig(grid=(10, 10, 4), dgrid=(1, 1, 1), magnet_active=True,
+                          bc=[["circ", "circ", "wormhole"], ["circ", "circ", "wormhole"]])
+    assert L.imh_create(ctypes.byref(make_imh_config(rc)), ctypes.byref(h)) == -6
+    assert b"magnet" in L.imh_last_error()
     rc = config.RunConfig(grid=(10, 10, 1), dgrid=(1, 1, 1))
     c = make_imh_config(rc)
     c.limiter[1] = 7
@@ -71,4 +79,5 @@ def test_config_parsing_mirrors_initialize():
     assert rc.bc == [["trans", "circ", "circ"], ["const", "circ", "circ"]]
     assert rc.cfl == 0.35  # Initializer.m:143-150 default with the magnet on
     with pytest.raises(ValueError, match="Unknown BC mode"):
-        config.from_ini({"grid": (4, 4, 1), "bcMode": "wormhole"})
+        config.from_ini({"grid": (4, 4, 1), "bcMode": "linear"})
+    assert config.from_ini({"grid": (4, 4, 4), "bcMode": {"z": "wormhole"}}).bc[0][2] == "wormhole"

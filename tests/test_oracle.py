@@ -182,6 +182,102 @@ def test_shift_sign_flips_for_the_shift_component():
     np.testing.assert_array_equal(out, [1, 1, 2, 3, 4, 5])
 
 
+# wormhole_shift.m:21-57, restated with numpy flips on a MATLAB-ordered (x, y, z) array:
+# after the constant shift, result(amt:-1:1, all of P, N_Q:-1:1) = isVecNorm *
+# array(1:amt, all of P, 1:N_Q) along D, with Q = nextDir = [2 3 1](D); the upper side is
+# the constant shift alone.
+def _wormhole_expected(M, D, n, component):
+    ax, q = D - 1, [1, 2, 0][D - 1]
+    m = abs(n)
+    N = M.shape[ax]
+    idx = np.arange(N)
+    src = np.clip(idx + n, 0, N - 1)   # constant_shift: shifted(i) = a(i + n) clamped
+    R = np.take(M, src, axis=ax)
+    if n < 0:
+        T = np.take(M, np.arange(m), axis=ax)
+        T = np.flip(np.flip(T, axis=ax), axis=q)
+        sl = [slice(None)] * 3
+        sl[ax] = slice(0, m)
+        R[tuple(sl)] = (-1.0 if component == D else 1.0) * T
+    return R
+
+
+@pytest.mark.parametrize("D", [1, 2, 3])
+@pytest.mark.parametrize("n", [-2, -1, 1, 2])
+@pytest.mark.parametrize("component", [0, 1, 2, 3])
+def test_wormhole_shift_index_map(D, n, component):
+    nz, ny, nx = 5, 4, 3   # odd and even extents along every nextDir
+    a = np.arange(1.0, 1.0 + nx * ny * nz).reshape(nz, ny, nx)
+    bc = {"x": "circ", "y": "circ", "z": "circ"}
+    bc["xyz"[D - 1]] = "wormhole"
+    out, _, _ = O.shift_test(a, bc, D, n, component=component)
+    M = a.transpose(2, 1, 0)   # MATLAB's array(i, j, k)
+    expected = _wormhole_expected(M, D, n, component).transpose(2, 1, 0)
+    np.testing.assert_array_equal(out, expected)
+
+
+def test_wormhole_index_map_by_hand():
+    # x wormhole on a 2 x 3 x 2 grid: the lower ghost of pencil (y, z) is cell 0 of pencil
+    # (2 - y, z); two-cell shifts take cells 1 and 0 of that pencil in that order
+    a = np.arange(12.0).reshape(2, 3, 2)   # [z][y][x]
+    out, _, _ = O.shift_test(a, {"x": "wormhole", "y": "circ", "z": "circ"}, 1, -1)
+    for z in range(2):
+        for y in range(3):
+            assert out[z, y, 0] == a[z, 2 - y, 0]
+            assert out[z, y, 1] == a[z, y, 0]
+    out, _, _ = O.shift_test(a, {"x": "wormhole", "y": "circ", "z": "circ"}, 1, -2, component=1)
+    for z in range(2):
+        for y in range(3):
+            assert (out[z, y, 0], out[z, y, 1]) == (-a[z, 2 - y, 1], -a[z, 2 - y, 0])
+
+
+def test_wormhole_needs_three_dimensions():
+    with pytest.raises(ValueError, match="3-D grid"):
+        O.Oracle(O.make_config((8, 8, 1), (0.1, 0.1, 0.1), bc={"x": "wormhole"}))
+
+
+def _uniform_along(state, q):
+    """The state with every field constant along dimension q (0 x, 1 y, 2 z; arrays are
+    [z][y][x]) and no momentum along q."""
+    ax = 2 - q
+    out = {}
+    for k in ("mass", "ener"):
+        out[k] = np.broadcast_to(np.take(state[k], [0], axis=ax), state[k].shape).copy()
+    out["mom"] = [np.zeros_like(m) if i == q else
+                  np.broadcast_to(np.take(m, [0], axis=ax), m.shape).copy()
+                  for i, m in enumerate(state["mom"])]
+    out["mag"] = state["mag"]
+    return out
+
+
+@pytest.mark.parametrize("axis", ["x", "y", "z"])
+def test_wormhole_run_equals_flip_when_the_partner_is_the_pencil_itself(axis):
+    """wormhole_shift.m pairs the pencils at next-dimension coordinates q and N_Q-1-q.  On a
+    state that does not vary along Q, with no Q momentum, the partner's cells equal the
+    pencil's own, so every wormhole ghost equals the flip ghost (flip_shift.m: the
+    constant shift, negated for the shift component) and whole runs agree bit for bit."""
+    D = "xyz".index(axis)
+    q = (D + 1) % 3
+    grid = (10, 8, 6)
+    st = _uniform_along(problems.random_state(grid, seed=40 + D, magnet=False), q)
+    runs = []
+    for mode in ("wormhole", "flip", "const"):
+        bc = {"x": "circ", "y": "circ", "z": "circ"}
+        bc[axis] = [mode, "trans"]
+        ini = {"grid": grid, "dGrid": (0.1, 0.1, 0.1), "gamma": 1.4, "cfl": 0.6, "bcMode": bc,
+               "bcInfinity": 20, "fluxLimiter": "vanleer", "mode": {"fluid": True, "magnet": False},
+               "iterMax": 100, "timeMax": 1e9}
+        o = make_oracle(ini, st)
+        for _ in range(6):
+            o.iteration()
+        runs.append({f: o.get(f) for f in ("mass", "momx", "momy", "momz", "ener")})
+        o.close()
+    for f in runs[0]:
+        np.testing.assert_array_equal(runs[0][f], runs[1][f], err_msg=f)
+    # the sign of the shift component matters: a constant boundary gives another run
+    assert np.any(runs[0]["mom" + axis] != runs[2]["mom" + axis])
+
+
 def test_zero_shift_by_flux_type():
     # FluxArray: FLUXL keeps its lower ghost constant, FLUXR its upper one; others get 0
     np.testing.assert_array_equal(shift1d("zero", -1, is_flux=True, flux_type=2)[0], [0, 1, 2, 3, 4, 5])
