@@ -85,10 +85,12 @@ SIGNATURES = {
     "imh_init_problem": (_I, [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_int]),
     "imh_source": (_I, [_P]),
     "imh_set_gravity": (_I, [_P, _D, ctypes.c_size_t, ctypes.c_double]),
+    "imh_set_spacing": (_I, [_P, _I, _D, ctypes.c_size_t]),
     "imh_set_radiation": (_I, [_P, _I, ctypes.c_double, ctypes.c_double]),
     "imh_get_radiation_strength": (_I, [_P, _D]),
     "imh_set_viscosity": (_I, [_P, _I, ctypes.c_double, ctypes.c_double]),
     "imh_batch_set_gravity": (_I, [_P, _I, _D, ctypes.c_size_t, ctypes.c_double]),
+    "imh_batch_set_spacing": (_I, [_P, _I, _I, _D, ctypes.c_size_t]),
     "imh_batch_set_radiation": (_I, [_P, _I, _I, ctypes.c_double, ctypes.c_double]),
     "imh_batch_set_viscosity": (_I, [_P, _I, _I, ctypes.c_double, ctypes.c_double]),
     "imh_checkpoint_bytes": (_I, [_P, ctypes.POINTER(ctypes.c_size_t)]),

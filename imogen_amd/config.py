@@ -5,7 +5,9 @@ bcMode, bcInfinity, fluxLimiter, cfl, timeUpdateMode, minMass, iterMax, timeMax,
 dGrid, gamma and grid.  Errors are raised with the reference's message text.
 """
 from dataclasses import dataclass, field
-from typing import List, Tuple
+from typing import List, Optional, Tuple
+
+import numpy as np
 
 
 class ENUM:
@@ -67,6 +69,9 @@ class RunConfig:
     iter_max: int = 1000
     time_max: float = 5000.0
     perms_order: int = 0
+    # non-uniform DGRID{i}: a line of grid[i] spacings (dgrid[i] then holds its minimum,
+    # MINDGRID), or None for the uniform dgrid[i]
+    dgrid_lines: List[Optional[np.ndarray]] = field(default_factory=lambda: [None] * 3)
 
     @property
     def ndims(self):
@@ -110,21 +115,102 @@ def parse_bc_modes(bc):
     return out
 
 
+def pchip(x, y, xi):
+    """MATLAB's interp1(x, y, xi, 'pchip'): piecewise cubic Hermite with the
+    Fritsch-Carlson slopes (weighted harmonic mean of the neighbouring secants, zero at a
+    sign change) and the three-point, shape-preserving end slopes; two points give the
+    line; outside [x(1), x(end)] the end pieces extrapolate."""
+    x = np.asarray(x, dtype=np.float64)
+    y = np.asarray(y, dtype=np.float64)
+    xi = np.asarray(xi, dtype=np.float64)
+    n = x.size
+    if n < 2 or np.any(np.diff(x) <= 0):
+        raise ValueError("pchip needs at least two strictly increasing sample points")
+    h = np.diff(x)
+    delta = np.diff(y) / h
+    d = np.zeros(n)
+    if n == 2:
+        d[:] = delta[0]
+    else:
+        for k in range(1, n - 1):
+            if np.sign(delta[k - 1]) * np.sign(delta[k]) > 0:
+                w1 = 2.0 * h[k] + h[k - 1]
+                w2 = h[k] + 2.0 * h[k - 1]
+                d[k] = (w1 + w2) / (w1 / delta[k - 1] + w2 / delta[k])
+
+        def end(h1, h2, del1, del2):
+            s = ((2.0 * h1 + h2) * del1 - h1 * del2) / (h1 + h2)
+            if np.sign(s) != np.sign(del1):
+                return 0.0
+            if np.sign(del1) != np.sign(del2) and abs(s) > abs(3.0 * del1):
+                return 3.0 * del1
+            return s
+
+        d[0] = end(h[0], h[1], delta[0], delta[1])
+        d[-1] = end(h[-1], h[-2], delta[-1], delta[-2])
+    k = np.clip(np.searchsorted(x, xi, side="right") - 1, 0, n - 2)
+    t = xi - x[k]
+    c2 = (3.0 * delta[k] - 2.0 * d[k] - d[k + 1]) / h[k]
+    c3 = (d[k] - 2.0 * delta[k] + d[k + 1]) / (h[k] * h[k])
+    return y[k] + t * (d[k] + t * (c2 + t * c3))
+
+
+def grid_spacing(dg, grid):
+    """ImogenManager.setGridSpacing (classes/ImogenManager.m:190-254).  A number or a
+    3-vector is uniform.  A dict (the struct form) takes, per field x/y/z: a number (that
+    spacing), or a dict with 'value' (else the top-level 'value', else 1) and 'points'
+    (rows [percent, multiplier], default [[0, 1], [100, 1]]): the percentages map to grid
+    points floor((N-1)*p/100)+1 and DGRID is interp1(points, value*multipliers, 1:N,
+    'pchip'); a missing field takes the top-level 'value'.  Returns (dgrid, lines):
+    dgrid[i] is MINDGRID(i) (minFinderND) and lines[i] the line of a non-uniform i."""
+    if dg is None:
+        return (1.0, 1.0, 1.0), [None] * 3
+    if isinstance(dg, (int, float)):
+        return (float(dg),) * 3, [None] * 3
+    if not isinstance(dg, dict):
+        v = [float(a) for a in dg]
+        if len(v) == 2:
+            v = v + [max(v)]
+        return tuple(v), [None] * 3
+    dgrid, lines = [], []
+    for i, key in enumerate("xyz"):
+        n = int(grid[i])
+        if key in dg:
+            f = dg[key]
+            if isinstance(f, dict):
+                value = float(f.get("value", dg.get("value", 1.0)))
+                pts = np.asarray(f.get("points", [[0.0, 1.0], [100.0, 1.0]]), dtype=np.float64)
+                px = np.floor((n - 1) * pts[:, 0] / 100.0) + 1.0
+                line = pchip(px, value * pts[:, 1], np.arange(1, n + 1, dtype=np.float64))
+            else:
+                arr = np.asarray(f, dtype=np.float64)
+                if arr.size == 1:
+                    dgrid.append(float(arr.ravel()[0]))
+                    lines.append(None)
+                    continue
+                line = arr.ravel()
+                if line.size != n:
+                    raise ValueError("dGrid.%s holds %d spacings, the grid %d cells"
+                                     % (key, line.size, n))
+        else:
+            if "value" not in dg:
+                raise ValueError("dGrid has no field %s and no value" % key)
+            dgrid.append(float(dg["value"]))
+            lines.append(None)
+            continue
+        if np.any(line <= 0):
+            raise ValueError("dGrid.%s: spacings must be > 0" % key)
+        dgrid.append(float(np.min(line)))
+        lines.append(line)
+    return tuple(dgrid), lines
+
+
 def from_ini(ini, perms_order=0):
     """Parse an ini dict (keys as produced by ``Initializer.getRunSettings``)."""
     grid = tuple(int(g) for g in ini["grid"])
     if len(grid) != 3:
         raise ValueError("ini.grid must have 3 entries")
-    dg = ini.get("dGrid")
-    if dg is None:
-        dgrid = (1.0, 1.0, 1.0)
-    elif isinstance(dg, (int, float)):
-        dgrid = (float(dg),) * 3
-    else:
-        dg = [float(v) for v in dg]
-        if len(dg) == 2:
-            dg = dg + [max(dg)]
-        dgrid = tuple(dg)
+    dgrid, dlines = grid_spacing(ini.get("dGrid"), grid)
     lim = ini.get("fluxLimiter", FluxLimiterEnum.VAN_LEER)
     if isinstance(lim, str):
         lims = [lim] * 3
@@ -144,4 +230,5 @@ def from_ini(ini, perms_order=0):
         bc=parse_bc_modes(ini.get("bcMode", "circ")), limiter=lims,
         fluid_active=bool(mode.get("fluid", True)), magnet_active=bool(mode.get("magnet", False)),
         update_mode=int(ini.get("timeUpdateMode", 1)), iter_max=int(ini.get("iterMax", 1000)),
-        time_max=float(ini.get("timeMax", 5000.0)), perms_order=int(perms_order))
+        time_max=float(ini.get("timeMax", 5000.0)), perms_order=int(perms_order),
+        dgrid_lines=dlines)

@@ -349,7 +349,7 @@ static int build_plans(imh_batch* b) {
                             const bool gen = c->cfg.bc[0][dX] == IMH_BC_FADE ||
                                              c->cfg.bc[1][dX] == IMH_BC_FADE || c->visc_type;
                             // bs 0: the column march (one thread per pencil)
-                            const bool col = ft.column && !gen;
+                            const bool col = ft.column && !gen && !c->dline_dev[dX];
                             const int gbs = col ? 0 : ft.bs, gkm = col ? 0 : ft.kmax;
                             G* g = nullptr;
                             for (G& gg : groups)
@@ -752,6 +752,15 @@ int imh_batch_set_gravity(imh_batch* b, int sim, const double* potential, size_t
     int rc = batch_quiesce(b);
     if (rc) return rc;
     rc = imh_set_gravity(b->ctx[sim], potential, n, mass_threshold);
+    free_plans(b);
+    return rc;
+}
+
+int imh_batch_set_spacing(imh_batch* b, int sim, int dim, const double* line, size_t n) {
+    if (!b || sim < 0 || sim >= b->n) return fail(IMH_EINVAL, "bad simulation index");
+    int rc = batch_quiesce(b);
+    if (rc) return rc;
+    rc = imh_set_spacing(b->ctx[sim], dim, line, n);
     free_plans(b);
     return rc;
 }

@@ -834,7 +834,11 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
                     sR = fade_fix(true, 0, gg, i0 + k, sR);
                     sL = fade_fix(false, 1, gg, i0 + k, sL);
                 }
-                qh[v][k] = q[v][k] - hfC * (FR[gg][k] - sR + FL[gg][k] - sL);
+                // 0.5*fluxFactor .* tempFreeze; fluxFactor = dTime ./ DGRID{X} per cell
+                // on a non-uniform grid
+                const T hk = a.dxl ? T(0.5 * (dt / a.dxl[i0 + k < N ? i0 + k : N - 1])) * C
+                                   : hfC;
+                qh[v][k] = q[v][k] - hk * (FR[gg][k] - sR + FL[gg][k] - sL);
             }
         });
     });
@@ -966,7 +970,8 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
                     sR = fade_fix(true, 0, gg, i0 + k, sR);
                     sL = fade_fix(false, 1, gg, i0 + k, sL);
                 }
-                dq[k] = ffC * (FR[gg][k] - sR + FL[gg][k] - sL);
+                const T fk = a.dxl ? T(dt / a.dxl[i0 + k < N ? i0 + k : N - 1]) * C2 : ffC;
+                dq[k] = fk * (FR[gg][k] - sR + FL[gg][k] - sL);
             }
             T un[K];
 #pragma unroll

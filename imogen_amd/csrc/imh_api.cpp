@@ -60,13 +60,28 @@ void sweep_order(int iteration, int order, int perms_order, int dirs[3], int mi[
     }
 }
 
+// FluidArray.m:48 and MagnetArray.m:43 store squeeze(array): a 3-D grid with a singleton
+// x or y becomes a 2-D array whose dimensions are the non-singleton ones in order, and
+// everything per dimension (gridSize, ImogenArray.m:93-96; the shift switches,
+// :292-297; the BC modes, limiters and DGRID, indexed by dimension number) then applies
+// to the squeezed array's dimensions.  The cells keep their memory order, so the grid is
+// relabelled and nothing moves.  (MATLAB's squeeze leaves 2-D arrays alone.)
+imh_config squeeze_grid(const imh_config& in, bool* squeezed) {
+    imh_config c = in;
+    *squeezed = false;
+    if (in.grid[2] > 1 && (in.grid[0] == 1 || in.grid[1] == 1)) {
+        int k = 0;
+        for (int d = 0; d < 3; ++d)
+            if (in.grid[d] > 1) c.grid[k++] = in.grid[d];
+        while (k < 3) c.grid[k++] = 1;
+        *squeezed = true;
+    }
+    return c;
+}
+
 int validate_config(const imh_config* c) {
     for (int d = 0; d < 3; ++d)
         if (c->grid[d] < 1) return fail(IMH_EINVAL, "grid[%d] must be >= 1", d);
-    if (c->grid[2] > 1 && (c->grid[0] == 1 || c->grid[1] == 1))
-        return fail(IMH_ENOTSUP,
-                    "a grid with nz > 1 and a singleton x or y is re-ordered by squeeze() "
-                    "in the reference (FluidArray.m:48)");
     for (int d = 0; d < 3; ++d)
         if (c->dgrid[d] <= 0) return fail(IMH_EINVAL, "dgrid[%d] must be > 0", d);
     if (c->precision != IMH_FP64 && c->precision != IMH_FP32_BULK)
@@ -203,6 +218,8 @@ static void ctx_free(imh_ctx* c) {
     for (int i = 0; i < IMH_NFIELDS; ++i)
         if (c->f[i]) hipFree(c->f[i]);
     if (c->scratch) hipFree(c->scratch);
+    for (int d = 0; d < 3; ++d)
+        if (c->dline_dev[d]) hipFree(c->dline_dev[d]);
     if (c->edges) hipFree(c->edges);
     if (c->eoff_dev) hipFree(c->eoff_dev);
     if (c->sim) hipFree(c->sim);
@@ -359,15 +376,19 @@ const char* imh_last_error(void) { return imh::g_last_error.c_str(); }
 
 const char* imh_version(void) { return "imogen-hip 0.1 (gfx950)"; }
 
-int imh_create(const imh_config* cfg, imh_ctx** out) {
-    if (!cfg || !out) return fail(IMH_EINVAL, "null argument");
+int imh_create(const imh_config* cfg_in, imh_ctx** out) {
+    if (!cfg_in || !out) return fail(IMH_EINVAL, "null argument");
     *out = nullptr;
+    bool squeezed = false;
+    const imh_config sq = squeeze_grid(*cfg_in, &squeezed);
+    const imh_config* cfg = &sq;
     int rc = validate_config(cfg);
     if (rc) return rc;
     hipError_t e = hipSetDevice(cfg->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     imh_ctx* c = new imh_ctx();
     c->cfg = *cfg;
+    c->squeezed = squeezed;
     for (int d = 0; d < 3; ++d) c->n[d] = cfg->grid[d];
     c->ndims = cfg->grid[2] > 1 ? 3 : 2;
     c->ncell = (long long)c->n[0] * c->n[1] * c->n[2];
@@ -699,6 +720,7 @@ FluidArgs<T> make_fluid_args(const imh_ctx* c, int d, int force) {
     a.gamma = c->cfg.gamma;
     a.minmass = c->cfg.minmass;
     a.dx = c->cfg.dgrid[d];
+    a.dxl = c->dline_dev[d];
     a.init_edges = c->fluid_init ? 0 : 1;
     const FluidTile& t = c->ftile[d];
     a.PX = t.PX;
@@ -735,7 +757,7 @@ static int fluid_sweep_t(imh_ctx* c, int d, int force) {
     // Hydro sweeps only: a run of the MHD instantiations (the only ones that spill
     // registers to scratch) left the GPU faulted once, cause not found; MHD sweeps take the
     // general form until it is.
-    if (c->sweep_kind != IMH_SWEEP_GENERAL && !fluid_needs_general(a) && !c->mhd_kernel &&
+    if (c->sweep_kind != IMH_SWEEP_GENERAL && !fluid_needs_general(a) && !c->mhd_kernel && !a.dxl &&
         wave_bc_ok(a.modes, d) && wave_tile(c->n[d], d, (int)sizeof(T), c->max_lds, &wbs, &wpx, &wtl, &wlds) &&
         (c->sweep_kind != IMH_SWEEP_AUTO_FAST || (a.npencil + wpx - 1) / wpx >= 4LL * c->num_cu)) {
         // the wave-spread sweep: its own tile in the argument block
@@ -748,7 +770,7 @@ static int fluid_sweep_t(imh_ctx* c, int d, int force) {
         const int nt = int((a.npencil + wpx - 1) / wpx);
         e = launch_fluid_wave((const FluidArgs<T>*)dw, nullptr, nt, nt, wbs,
                               c->sweep_kind != IMH_SWEEP_WAVE, d, c->mhd_kernel, wlds, c->st);
-    } else if (t.column && !fluid_needs_general(a)) {
+    } else if (t.column && !fluid_needs_general(a) && !a.dxl) {
         const int nt = int((a.npencil + fluid_column_pencils() - 1) / fluid_column_pencils());
         e = launch_fluid_column((const FluidArgs<T>*)da, nullptr, nt, nt, c->mhd_kernel, c->st);
     } else {
@@ -794,6 +816,8 @@ MagnetArgs<T> make_magnet_args(const imh_ctx* c, int dX, int dI, int force) {
     a.limiter = c->cfg.limiter[dX];
     a.dxX = c->cfg.dgrid[dX];
     a.dxI = c->cfg.dgrid[dI];
+    a.dxlX = c->dline_dev[dX];
+    a.dxlI = c->dline_dev[dI];
     int pair = 3 * dI + dX;
     a.init_vg = !c->obj_init[OBJ_VG + pair];
     a.init_store = !c->obj_init[OBJ_MSFR + pair];
@@ -947,6 +971,7 @@ hipError_t imh_launch_source(imh_ctx* c, hipStream_t st) {
         a.n[d] = c->n[d];
         a.nface[d] = c->nface[d];
         a.dgrid[d] = c->cfg.dgrid[d];
+        a.dxl[d] = c->dline_dev[d];
         a.active_lin[d] = c->active_lin[d];
         a.goff[d] = g;
         if (d < c->ndims) g += c->nface[d];
@@ -1050,6 +1075,10 @@ int imh_set_gravity(imh_ctx* c, const double* potential, size_t n, double mass_t
     if (!c) return fail(IMH_EINVAL, "null context");
     if (potential && n != (size_t)c->ncell)
         return fail(IMH_EINVAL, "potential holds %zu values, the grid %lld", n, c->ncell);
+    if (potential && c->squeezed)
+        return fail(IMH_ENOTSUP, "gravity on a grid with a singleton x or y: the potential "
+                    "keeps run.gridSize (GravityArray.m:34) while the fluid arrays are "
+                    "squeezed, and the source terms mix the two shapes");
     if (potential) {
         for (int d = 0; d < c->ndims; ++d)
             for (int s = 0; s < 2; ++s) {
@@ -1092,6 +1121,42 @@ int imh_set_gravity(imh_ctx* c, const double* potential, size_t n, double mass_t
             e = hipMemset(c->edges + c->eoff[OBJ_GRAV][s][d], 0, sizeof(double) * c->nface[d]);
             if (e != hipSuccess) return hip_fail(e, "potential edges");
         }
+    return IMH_OK;
+}
+
+int imh_set_spacing(imh_ctx* c, int dim, const double* line, size_t n) {
+    if (!c) return fail(IMH_EINVAL, "null context");
+    if (dim < 1 || dim > 3) return fail(IMH_EINVAL, "dim must be 1, 2 or 3");
+    const int d = dim - 1;
+    if (line && c->squeezed)
+        return fail(IMH_ENOTSUP, "non-uniform spacing on a grid with a singleton x or y: "
+                    "DGRID keeps run.gridSize while the arrays are squeezed");
+    if (line) {
+        if (n != (size_t)c->n[d])
+            return fail(IMH_EINVAL, "spacing line of %zu values along a dimension of %d cells",
+                        n, c->n[d]);
+        for (size_t i = 0; i < n; ++i)
+            if (!(line[i] > 0)) return fail(IMH_EINVAL, "spacing must be > 0");
+    }
+    hipError_t e = hipStreamSynchronize(c->st);
+    if (e != hipSuccess) return hip_fail(e, "sync");
+    drop_graphs(c);
+    if (!line) {
+        if (c->dline_dev[d]) hipFree(c->dline_dev[d]);
+        c->dline_dev[d] = nullptr;
+        c->host_par.dgrid[d] = c->cfg.dgrid[d];
+    } else {
+        if (!c->dline_dev[d] &&
+            (e = hipMalloc(&c->dline_dev[d], sizeof(double) * n)) != hipSuccess)
+            return hip_fail(e, "hipMalloc(spacing)");
+        e = hipMemcpy(c->dline_dev[d], line, sizeof(double) * n, hipMemcpyHostToDevice);
+        if (e != hipSuccess) return hip_fail(e, "spacing upload");
+        double mn = line[0];
+        for (size_t i = 0; i < n; ++i) mn = std::min(mn, line[i]);
+        c->host_par.dgrid[d] = mn;  // MINDGRID(d) = minFinderND(DGRID{d})
+    }
+    e = hipMemcpy(c->par, &c->host_par, sizeof(SimParams), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, "par upload");
     return IMH_OK;
 }
 
@@ -1285,6 +1350,9 @@ extern "C" int imh_init_problem(imh_ctx* c, int kind, const double* params, int 
     static const int need[9] = {0, 0, 0, 3, 15, 4, 0, 0, 4};
     if (kind < IMH_IC_SOD || kind > IMH_IC_MAGNETIC_SHOCK_TUBE)
         return fail(IMH_EINVAL, "unknown initial-condition kind %d", kind);
+    if (c->squeezed)
+        return fail(IMH_ENOTSUP, "initial conditions on the device are built on the grid as "
+                    "given; this context's grid was squeezed (singleton x or y)");
     if (nparams < need[kind] || (need[kind] > 0 && !params))
         return fail(IMH_EINVAL, "initial condition %d takes %d parameters", kind, need[kind]);
     IcParams P;
@@ -1593,10 +1661,12 @@ int imh_memory_bytes(imh_ctx* c, size_t* bytes) {
 
 int imh_memory_estimate(const imh_config* cfg, size_t* bytes) {
     if (!cfg || !bytes) return fail(IMH_EINVAL, "null argument");
-    int rc = validate_config(cfg);
+    bool squeezed = false;
+    const imh_config sq = squeeze_grid(*cfg, &squeezed);
+    int rc = validate_config(&sq);
     if (rc) return rc;
     long long eoff[NOBJ][2][3];
-    *bytes = ctx_bytes(*cfg, (size_t)edges_layout(*cfg, eoff));
+    *bytes = ctx_bytes(sq, (size_t)edges_layout(sq, eoff));
     return IMH_OK;
 }
 

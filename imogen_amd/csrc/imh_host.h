@@ -76,6 +76,7 @@ struct imh_ctx {
     double* csum_out = nullptr;
     bool obj_init[imh::NOBJ] = {};
     bool fluid_init = false;
+    bool squeezed = false;   // the grid was relabelled by squeeze_grid (singleton x or y)
     int sweep_kind = 0;             // IMH_SWEEP_GENERAL / _WAVE / _WAVE_FAST
     int num_cu = 256;               // compute units of the device
     size_t max_lds = 64 * 1024;     // LDS bytes per workgroup of the device
@@ -104,6 +105,8 @@ struct imh_ctx {
     imh::Profiler* prof = nullptr;      // set while imh_profile_iterations runs
     // source terms (source.m): fixed gravitational potential and radiation
     double* grav = nullptr;             // grav.array (FP64), nullptr: gravity off
+    // non-uniform DGRID{d}: one line of n[d] spacings on the device, nullptr: uniform
+    double* dline_dev[3] = {nullptr, nullptr, nullptr};
     double* grav_ghost = nullptr;       // 6 ghost values per pencil end (source.hip)
     double mass_threshold = 0.0;        // run.fluid.MASS_THRESHOLD
     int rad_type = 0;                   // 0 'empty', 1 'optically_thin'
@@ -158,6 +161,7 @@ struct SourceArgs {
     const double* fade;       // fade_shift.m coefficients (device), bcInfinity long
     double mass_threshold;
     double dgrid[3];
+    const double* dxl[3];     // non-uniform spacing lines (device), nullptr: dgrid[d]
     double gamma;
     double rad_strength;
     double rad_exponent;

@@ -38,6 +38,7 @@ struct FluidArgs {
     int limiter;
     MagUse use;              // !MagnetArray.isZero per component
     double gamma, minmass, dx;
+    const double* dxl;       // non-uniform DGRID{X}: spacing of cell i along d, or nullptr
     int init_edges;          // first assignment of the flux objects (first fluid sweep)
     int PX, TL, K, xmajor;   // tile: PX pencils x TL threads, K cells per thread
     long long npencil;       // pencil slots (worm_slots with a lower wormhole boundary)
@@ -201,6 +202,8 @@ struct MagnetArgs {
     int active_linX, active_linI;  // Edges.ACTIVE(D) (linear index) for D = X, I
     int limiter;
     double dxX, dxI;
+    const double* dxlX;      // non-uniform DGRID{X} / DGRID{I} lines, or nullptr
+    const double* dxlI;
     int init_vg, init_store, init_wmag, init_flux;  // first assignments
     int PX, TL;              // tile for the pencil kernel
     long long npencil;

@@ -459,7 +459,9 @@ __device__ __forceinline__ void magnet_advect_body(const MagnetArgs<T>& ap, long
     for (int k = 0; k < KMAX; ++k) {
         if (k < kc) {
             T s = lds_shift(row1, N, (ty + k * TLc), -1, &s_g[px][4], mlo, a.fade, f);
-            st[k] = bI[k] - T(ff) * (w[k] - s);
+            // fluxFactor = 0.5*dTime ./ DGRID{X}: per cell on a non-uniform grid
+            const double fk = a.dxlX ? 0.5 * dt / a.dxlX[ty + k * TLc] : ff;
+            st[k] = bI[k] - T(fk) * (w[k] - s);
         } else {
             st[k] = T(0);
         }
@@ -522,7 +524,8 @@ __device__ __forceinline__ void magnet_advect_body(const MagnetArgs<T>& ap, long
         if (k < kc) {
             long long cell = base + (long long)((ty + k * TLc)) * stride;
             T s = lds_shift(row1, N, (ty + k * TLc), -1, &s_g[px][6], mlo, a.fade, f);
-            gmem(a.bI)[cell] = bI[k] - T(ff2) * (F[k] - s);
+            const double f2k = a.dxlX ? 2.0 * (0.5 * dt / a.dxlX[ty + k * TLc]) : ff2;
+            gmem(a.bI)[cell] = bI[k] - T(f2k) * (F[k] - s);
             gmem(a.flux)[cell] = F[k];
         }
     }
@@ -578,7 +581,8 @@ __device__ __forceinline__ void magnet_ct_body(const MagnetArgs<T>& ap, long lon
         pencil_base(nn, dX, pen, base, stride);
         pencil_coords(nn, dX, pen, pc);
     }
-    const double ffI = a.sim->dt / a.dxI;
+    // fluxFactor = dTime ./ DGRID{I}: the pencil's own I coordinate on a non-uniform grid
+    const double ffI = a.sim->dt / (a.dxlI ? a.dxlI[pc[dI]] : a.dxI);
     const int oFL = OBJ_MFLUX + 3 * dI + dX;
     const int mlo = mode_at(a.modes, 0, dX);
     const bool I_enabled = dI < a.ndims;

@@ -120,7 +120,8 @@ __global__ void __launch_bounds__(256) source_cell_kernel(SourceArgs a, T* mass,
                     b2 = faded(2, 0, b2);
                 }
             }
-            const double d5 = (((-s2 + 8.0 * s1) - 8.0 * b1) + b2) / (12.0 * a.dgrid[d]);
+            const double dg = a.dxl[d] ? a.dxl[d][co[d]] : a.dgrid[d];  // DGRID{d} here
+            const double d5 = (((-s2 + 8.0 * s1) - 8.0 * b1) + b2) / (12.0 * dg);
             const double ms = (dt * thr) * d5;
             es = es + ms * m[d] / rho;
             m[d] = m[d] - ms;

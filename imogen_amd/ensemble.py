@@ -28,6 +28,13 @@ class Ensemble:
         h = ctypes.c_void_p()
         check(lib().imh_batch_create(cfgs, len(sims), ctypes.byref(h)))
         self._b = h
+        self._dlines = []
+        for i, rc in enumerate(self.configs):
+            for d, line in enumerate(rc.dgrid_lines):
+                if line is not None:
+                    arr = np.ascontiguousarray(line, dtype=np.float64)
+                    self._dlines.append(arr)
+                    check(lib().imh_batch_set_spacing(self._b, i, d + 1, dptr(arr), arr.size))
         for i, (_, st) in enumerate(sims):
             self.upload_state(i, st)
 

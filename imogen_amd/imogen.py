@@ -144,6 +144,27 @@ class ImogenManager:
         self._ctx = h
         self._arrays = None
         self.gravity_potential = None
+        # ImogenManager.setGridSpacing: a non-uniform DGRID{i} is a line along i
+        self._dlines = []
+        for d, line in enumerate(rc.dgrid_lines):
+            if line is not None:
+                self.set_spacing(d + 1, line)
+
+    def set_spacing(self, dim, line):
+        """DGRID{dim} as a line of spacings along that dimension (None: the uniform
+        configured value); MINDGRID(dim) becomes its minimum."""
+        if line is None:
+            check(lib().imh_set_spacing(self._ctx, int(dim), None, 0))
+            return
+        arr = np.ascontiguousarray(line, dtype=np.float64).ravel()
+        self._dlines.append(arr)
+        check(lib().imh_set_spacing(self._ctx, int(dim), dptr(arr), arr.size))
+        dg = list(self.DGRID)
+        dg[dim - 1] = arr.copy()
+        self.DGRID = tuple(dg)
+        md = list(self.MINDGRID)
+        md[dim - 1] = float(np.min(arr))
+        self.MINDGRID = tuple(md)
 
     def close(self):
         if getattr(self, "_ctx", None):

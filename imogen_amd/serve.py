@@ -43,8 +43,15 @@ class Server:
         except Exception:
             pass
 
+    @staticmethod
+    def _uniform(rc):
+        if any(line is not None for line in rc.dgrid_lines):
+            raise ValueError("serving takes uniform grid spacing (a request's configuration "
+                             "carries no spacing lines)")
+        return rc
+
     def submit(self, ini, st):
-        rc = from_ini(ini)
+        rc = self._uniform(from_ini(ini))
         c = make_imh_config(rc, self.precision)
         n = rc.ncell
         arrs = [as_field(a, n) for a in (st["mass"], *st["mom"], st["ener"], *st["mag"])]
@@ -61,7 +68,7 @@ class Server:
         """A request whose initial state is generated on the device at admission (kind:
         sod, orszag_tang, sedov, riemann, kelvin_helmholtz; params as imh_init_problem)."""
         from .imogen import ImogenManager
-        rc = from_ini(ini)
+        rc = self._uniform(from_ini(ini))
         c = make_imh_config(rc, self.precision)
         arr = (ctypes.c_double * max(1, len(params)))(*[float(p) for p in params])
         rid = ctypes.c_longlong()

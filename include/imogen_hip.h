@@ -165,6 +165,16 @@ int imh_set_gravity(imh_ctx *ctx, const double *potential, size_t n, double mass
 int imh_set_radiation(imh_ctx *ctx, int type, double exponent, double initial_maximum);
 int imh_get_radiation_strength(imh_ctx *ctx, double *strength);
 
+/* Non-uniform grid spacing: ImogenManager.setGridSpacing (classes/ImogenManager.m:190-254)
+ * with a struct dgrid builds DGRID{dim} as one interpolated line of grid(dim) spacings
+ * replicated over the other dimensions, and MINDGRID(dim) as its minimum.  This sets that
+ * line (n == the context's extent along dim, every value > 0; line == NULL restores the
+ * configuration's uniform dgrid[dim-1]).  fluxFactor = dt ./ DGRID then acts per cell in
+ * the fluid sweep (relaxingFluid.m:16), the magnet advection and CT (relaxingMagnet.m:15,
+ * 56) and the potential derivative (source.m:20); the CFL step uses MINDGRID
+ * (TimeManager.m:90).  Sweeps along such a dimension take the general fluid kernel. */
+int imh_set_spacing(imh_ctx *ctx, int dim, const double *line, size_t n);
+
 /* run.fluid.viscosity -- fluid/ArtificialViscosity.m (FluidManager.m:42, initialize.m:
  * 395-400): type 0 'empty', 1 'neumann_richtmyer' (ArtificialViscosity.m:73-77), 2
  * 'caramana_shashkov_whalen' (ArtificialViscosity.m:80-148, 155-190) with
@@ -275,11 +285,12 @@ int imh_batch_step(imh_batch *b, int iterations);
 int imh_batch_get_time(imh_batch *b, int sim, double *time, double *dtime, int *iteration);
 int imh_batch_synchronize(imh_batch *b);
 int imh_batch_memory_bytes(imh_batch *b, size_t *bytes);
-/* imh_set_gravity / imh_set_radiation / imh_set_viscosity for simulation `sim` of the
- * batch: its sweeps then run in the general kernel form and its source step follows each
- * flux call inside the batched iteration. */
+/* imh_set_gravity / imh_set_spacing / imh_set_radiation / imh_set_viscosity for
+ * simulation `sim` of the batch: its sweeps then run in the general kernel form and its
+ * source step follows each flux call inside the batched iteration. */
 int imh_batch_set_gravity(imh_batch *b, int sim, const double *potential, size_t n,
                           double mass_threshold);
+int imh_batch_set_spacing(imh_batch *b, int sim, int dim, const double *line, size_t n);
 int imh_batch_set_radiation(imh_batch *b, int sim, int type, double exponent,
                             double initial_maximum);
 int imh_batch_set_viscosity(imh_batch *b, int sim, int type, double linear, double quadratic);

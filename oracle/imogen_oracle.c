@@ -48,6 +48,9 @@ struct orc_sim {
     orc_config cfg;
     int n[3];
     int ndims;
+    int squeezed;                  /* grid relabelled by squeeze (singleton x or y) */
+    double *dline[3];              /* non-uniform DGRID{i}: one line along i, or NULL */
+    double mindgrid[3];            /* MINDGRID(i) = minFinderND(DGRID{i}) */
     long ncell;
     long nface[3];
     obj_t prim[5];                 /* mass, mom(1..3), ener  (FluidArray) */
@@ -87,6 +90,17 @@ int orc_get_threads(void) { return g_threads; }
 /* ------------------------------------------------------------------------------------ */
 
 /* MATLAB sign(): NaN for NaN, 0 for +-0. */
+/* DGRID{d} at cell c (the cell's coordinate along d indexes the line; ImogenManager.m:
+ * 226-254 replicates one interpolated line over the other dimensions) */
+static inline double dx_cell(const orc_sim *s, int d, long c) {
+    if (!s->dline[d]) return s->cfg.dgrid[d];
+    long i;
+    if (d == 0) i = c % s->n[0];
+    else if (d == 1) i = (c / s->n[0]) % s->n[1];
+    else i = c / ((long)s->n[0] * s->n[1]);
+    return s->dline[d][i];
+}
+
 static inline double msign(double x) {
     if (x > 0) return 1.0;
     if (x < 0) return -1.0;
@@ -656,7 +670,7 @@ int orc_relaxing_fluid(orc_sim *s, int X) {
     if (!s->have_state) return -1;
     long n = s->ncell;
     int d = X - 1;
-    double ff = s->dtime / s->cfg.dgrid[d]; /* fluxFactor = dTime ./ DGRID{X} */
+    const double dtime = s->dtime; /* fluxFactor = dTime ./ DGRID{X}, per cell */
     double *w[5];
     for (int i = 0; i < 5; ++i) w[i] = alloc_arr(n);
     double *q[5];
@@ -679,8 +693,8 @@ int orc_relaxing_fluid(orc_sim *s, int X) {
         do_shift(s, &s->sfr[i], X, -1, s1);
         do_shift(s, &s->sfl[i], X, 1, s2);
         double *st = alloc_arr(n);
-        double hf = 0.5 * ff;
-        EW_LOOP(n) st[c] = v[c] - hf * C[c] * (fr[c] - s1[c] + fl[c] - s2[c]);
+        EW_LOOP(n) st[c] = v[c] - (0.5 * (dtime / dx_cell(s, d, c))) * C[c] *
+                                     (fr[c] - s1[c] + fl[c] - s2[c]);
         obj_assign(s, &s->store[i], st);
         obj_cleanup(&s->sfr[i]);
         obj_cleanup(&s->sfl[i]);
@@ -738,7 +752,8 @@ int orc_relaxing_fluid(orc_sim *s, int X) {
         {
             const double *v = s->prim[i].a, *a = s->fr[i].a, *b = s->fl[i].a;
             double *nv = alloc_arr(n);
-            EW_LOOP(n) nv[c] = v[c] - ff * C2[c] * (a[c] - s1[c] + b[c] - s3[c]);
+            EW_LOOP(n) nv[c] = v[c] - (dtime / dx_cell(s, d, c)) * C2[c] *
+                                          (a[c] - s1[c] + b[c] - s3[c]);
             obj_assign(s, &s->prim[i], nv);
         }
         obj_cleanup(&s->fr[i]);
@@ -777,7 +792,7 @@ static void update_cell_centered(orc_sim *s, int k) {
 static void relaxing_magnet(orc_sim *s, obj_t *vg, int X, int I) {
     long n = s->ncell;
     int dx = X - 1, di = I - 1;
-    double ff = 0.5 * s->dtime / s->cfg.dgrid[dx];
+    const double hdt = 0.5 * s->dtime; /* fluxFactor = 0.5*dTime ./ DGRID{X}, per cell */
     double *sh = alloc_arr(n), *sh2 = alloc_arr(n), *vf = alloc_arr(n);
     /* velocityFlow = ((velGrid.array + velGrid.shift(X,1)) < 0) */
     do_shift(s, vg, X, 1, sh);
@@ -806,11 +821,10 @@ static void relaxing_magnet(orc_sim *s, obj_t *vg, int X, int I) {
     {
         double *a = alloc_arr(n);
         const double *b = B->a, *f = sfr->a;
-        EW_LOOP(n) a[c] = b[c] - ff * (f[c] - sh[c]);
+        EW_LOOP(n) a[c] = b[c] - (hdt / dx_cell(s, dx, c)) * (f[c] - sh[c]);
         obj_assign(s, st, a);
     }
-    /* corrector */
-    ff = 2.0 * ff;
+    /* corrector: fluxFactor = 2*fluxFactor */
     {
         double *a = alloc_arr(n);
         const double *t = st->a, *v = vg->a;
@@ -844,11 +858,11 @@ static void relaxing_magnet(orc_sim *s, obj_t *vg, int X, int I) {
     {
         double *a = alloc_arr(n);
         const double *b = B->a, *f = fl->a;
-        EW_LOOP(n) a[c] = b[c] - ff * (f[c] - sh[c]);
+        EW_LOOP(n) a[c] = b[c] - (2.0 * (hdt / dx_cell(s, dx, c))) * (f[c] - sh[c]);
         obj_assign(s, B, a);
     }
     /* constrained transport */
-    double ffi = s->dtime / s->cfg.dgrid[di];
+    const double dtime = s->dtime; /* fluxFactor = dTime ./ DGRID{I}, per cell */
     do_shift(s, fl, I, 1, sh);
     {
         double *a = alloc_arr(n);
@@ -862,7 +876,7 @@ static void relaxing_magnet(orc_sim *s, obj_t *vg, int X, int I) {
         obj_t *BX = &s->mag[dx];
         double *a = alloc_arr(n);
         const double *b = BX->a, *f = fl->a;
-        EW_LOOP(n) a[c] = b[c] - ffi * f[c];
+        EW_LOOP(n) a[c] = b[c] - (dtime / dx_cell(s, di, c)) * f[c];
         obj_assign(s, BX, a);
     }
     free(sh);
@@ -999,7 +1013,9 @@ int orc_time_update(orc_sim *s, int i) {
     }
     s->grid_index = gi;
     double upd = (double)(s->cfg.update_mode + 1);
-    s->dtime = s->cfg.cfl * s->cfg.dgrid[gi - 1] / cmax;
+    /* MINDGRID(gridIndex): the spacing itself, or the minimum of its line */
+    const double md = s->dline[gi - 1] ? s->mindgrid[gi - 1] : s->cfg.dgrid[gi - 1];
+    s->dtime = s->cfg.cfl * md / cmax;
     double newtime = s->time + upd * s->dtime;
     if (newtime > s->cfg.time_max) {
         s->dtime = (s->cfg.time_max - s->time) / upd;
@@ -1031,6 +1047,12 @@ int orc_set_gravity(orc_sim *s, const double *potential, double mass_threshold) 
     obj_free(&s->grav);
     s->mass_threshold = mass_threshold;
     if (!potential) return 0;
+    if (s->squeezed) {
+        /* GravityArray.m:34 sizes the potential by run.gridSize, unsqueezed, while the
+         * fluid arrays are squeezed: the source terms would mix the two shapes */
+        snprintf(s->err, sizeof(s->err), "gravity on a grid with a singleton x or y");
+        return -1;
+    }
     for (int d = 0; d < s->ndims; ++d)
         for (int side = 0; side < 2; ++side) {
             /* the 5-point derivative shifts by two cells: on a one-cell dimension every
@@ -1048,6 +1070,34 @@ int orc_set_gravity(orc_sim *s, const double *potential, double mass_threshold) 
         }
     s->grav_potential = dup_arr(potential, s->ncell);
     grav_create(s);
+    return 0;
+}
+
+/* ImogenManager.setGridSpacing (ImogenManager.m:190-254) with a non-uniform dimension:
+ * DGRID{dim} is one line of n[dim-1] spacings along that dimension (replicated over the
+ * others) and MINDGRID(dim) its minimum (minFinderND).  line == NULL: uniform again. */
+int orc_set_spacing(orc_sim *s, int dim, const double *line, int n) {
+    if (dim < 1 || dim > 3) return -1;
+    const int d = dim - 1;
+    free(s->dline[d]);
+    s->dline[d] = NULL;
+    if (!line) return 0;
+    if (n != s->n[d]) {
+        snprintf(s->err, sizeof(s->err), "spacing line of %d values along a dimension of %d "
+                 "cells", n, s->n[d]);
+        return -1;
+    }
+    double mn = line[0];
+    for (int i = 0; i < n; ++i) {
+        if (!(line[i] > 0)) {
+            snprintf(s->err, sizeof(s->err), "spacing must be > 0");
+            return -1;
+        }
+        mn = fmin(mn, line[i]);
+    }
+    s->dline[d] = (double *)malloc(sizeof(double) * (size_t)n);
+    memcpy(s->dline[d], line, sizeof(double) * (size_t)n);
+    s->mindgrid[d] = mn;
     return 0;
 }
 
@@ -1158,10 +1208,10 @@ int orc_source(orc_sim *s) {
             do_shift(s, &s->grav, i, 1, a1);
             do_shift(s, &s->grav, i, -1, b1);
             do_shift(s, &s->grav, i, -2, b2);
-            double den = 12.0 * s->cfg.dgrid[i - 1];
             const double *mom = s->prim[i].a;
             double *nm = alloc_arr(n);
             EW_LOOP(n) {
+                const double den = 12.0 * dx_cell(s, i - 1, c); /* ./ (12*dGrid) */
                 double d5 = (((-a2[c] + 8.0 * a1[c]) - 8.0 * b1[c]) + b2[c]) / den;
                 double ms = (dt * thr[c]) * d5;
                 es[c] = es[c] + ms * mom[c] / rho[c];
@@ -1228,12 +1278,25 @@ const char *orc_last_error(const orc_sim *s) { return s->err; }
 /* Lifecycle and state                                                                   */
 /* ------------------------------------------------------------------------------------ */
 
-orc_sim *orc_create(const orc_config *cfg, char *err, int errlen) {
+orc_sim *orc_create(const orc_config *cfg_in, char *err, int errlen) {
 #define FAIL(...) do { if (err) snprintf(err, (size_t)errlen, __VA_ARGS__); return NULL; } while (0)
     for (int d = 0; d < 3; ++d)
-        if (cfg->grid[d] < 1) FAIL("grid[%d] must be >= 1", d);
-    if (cfg->grid[2] > 1 && (cfg->grid[0] == 1 || cfg->grid[1] == 1))
-        FAIL("grid with a singleton x or y and nz > 1 is re-ordered by squeeze()");
+        if (cfg_in->grid[d] < 1) FAIL("grid[%d] must be >= 1", d);
+    /* FluidArray.m:48 / MagnetArray.m:43 keep squeeze(array): with nz > 1 and a singleton
+     * x or y the array is 2-D over the non-singleton dimensions in order, and gridSize
+     * (ImogenArray.m:93-96), the shift switches (:292-297), the BC modes, limiters and
+     * DGRID, all indexed by dimension number, then act on the squeezed dimensions.  Same
+     * cells in the same memory order: the grid is relabelled. */
+    orc_config sq = *cfg_in;
+    int squeezed = 0;
+    if (cfg_in->grid[2] > 1 && (cfg_in->grid[0] == 1 || cfg_in->grid[1] == 1)) {
+        int k = 0;
+        for (int d = 0; d < 3; ++d)
+            if (cfg_in->grid[d] > 1) sq.grid[k++] = cfg_in->grid[d];
+        while (k < 3) sq.grid[k++] = 1;
+        squeezed = 1;
+    }
+    const orc_config *cfg = &sq;
     int ndims = cfg->grid[2] > 1 ? 3 : 2;
     for (int sd = 0; sd < 2; ++sd)
         for (int d = 0; d < 3; ++d) {
@@ -1259,6 +1322,7 @@ orc_sim *orc_create(const orc_config *cfg, char *err, int errlen) {
 #undef FAIL
     orc_sim *s = (orc_sim *)calloc(1, sizeof(orc_sim));
     s->cfg = *cfg;
+    s->squeezed = squeezed;
     for (int d = 0; d < 3; ++d) s->n[d] = cfg->grid[d];
     s->ndims = ndims;
     s->ncell = (long)s->n[0] * s->n[1] * s->n[2];
@@ -1290,6 +1354,7 @@ void orc_destroy(orc_sim *s) {
     obj_free(&s->grav);
     free(s->grav_potential);
     free(s->fade);
+    for (int d = 0; d < 3; ++d) free(s->dline[d]);
     free(s);
 }
 

@@ -100,6 +100,10 @@ typedef struct orc_sim orc_sim;
 /* Lifecycle.  orc_create validates the configuration the way the reference would fail
  * (unknown mode, fade/mirror on a too-short dimension, ...): returns NULL and fills err. */
 orc_sim *orc_create(const orc_config *cfg, char *err, int errlen);
+/* ImogenManager.setGridSpacing with a non-uniform dimension: DGRID{dim} is a line of
+ * grid[dim-1] spacings (replicated over the other dimensions), MINDGRID(dim) its minimum.
+ * line == NULL restores the uniform spacing of the configuration. */
+int orc_set_spacing(orc_sim *s, int dim, const double *line, int n);
 void orc_destroy(orc_sim *s);
 
 /* Initial state: arrays are column-major [nx ny nz] (x fastest), the layout of imogen.m:34-42.

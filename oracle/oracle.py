@@ -84,6 +84,7 @@ def lib():
                                      ctypes.c_int, P, P, P, P, ctypes.c_int]
         L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_set_gravity.argtypes = [ctypes.c_void_p, P, ctypes.c_double]
+        L.orc_set_spacing.argtypes = [ctypes.c_void_p, ctypes.c_int, P, ctypes.c_int]
         L.orc_set_radiation.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
                                         ctypes.c_double]
         L.orc_radiation_strength.argtypes = [ctypes.c_void_p]
@@ -213,6 +214,17 @@ class Oracle:
             return
         v = np.ascontiguousarray(potential, dtype=np.float64).reshape(self.shape)
         self._check(self.L.orc_set_gravity(self.h, _ptr(v), float(mass_threshold)))
+
+    def set_spacing(self, dim, line):
+        """Non-uniform DGRID{dim} (1..3): a line of spacings along that dimension, or None
+        for the configuration's uniform value."""
+        if line is None:
+            self._check(self.L.orc_set_spacing(self.h, int(dim), None, 0))
+            return
+        v = np.ascontiguousarray(line, dtype=np.float64).ravel()
+        self._keep_dx = getattr(self, "_keep_dx", {})
+        self._keep_dx[dim] = v
+        self._check(self.L.orc_set_spacing(self.h, int(dim), _ptr(v), int(v.size)))
 
     def set_radiation(self, kind, exponent=0.5, initial_maximum=1.0):
         """Radiation type ('empty' / 'optically_thin'), strength from the current state."""

@@ -20,6 +20,9 @@ def oracle_config(rc):
 def make_oracle(ini, state, threads=1):
     rc = cfgmod.from_ini(ini)
     o = O.Oracle(oracle_config(rc), threads)
+    for d, line in enumerate(rc.dgrid_lines):
+        if line is not None:
+            o.set_spacing(d + 1, line)
     o.set_state(state["mass"], state["mom"], state["ener"], state["mag"])
     return o
 

@@ -47,8 +47,10 @@ def test_version_and_error_string():
 def test_create_rejects_bad_configs_before_touching_the_device():
     L = _lib.lib()
     from imogen_amd.imogen import make_imh_config
-    rc = config.RunConfig(grid=(8, 1, 4), dgrid=(1, 1, 1))   # squeeze() re-orders this grid
     h = ctypes.c_void_p()
+    # (10, 1, 8) squeezes to (10, 8): x's fade needs 20 cells along the first dimension
+    rc = config.RunConfig(grid=(10, 1, 8), dgrid=(1, 1, 1),
+                          bc=[["fade", "circ", "circ"], ["fade", "circ", "circ"]])
     assert L.imh_create(ctypes.byref(make_imh_config(rc)), ctypes.byref(h)) == -6
     rc = config.RunConfig(grid=(10, 10, 1), dgrid=(1, 1, 1),
                           bc=[["fade"] * 3, ["fade"] * 3])     # fade needs >= 20 cells

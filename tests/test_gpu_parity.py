@@ -377,3 +377,26 @@ def test_magnetic_shock_tube_lockstep():
           flush=True)
     run.close()
     orc.close()
+
+
+@pytest.mark.parametrize("grid", [(1, 24, 16), (20, 1, 12), (1, 1, 64)])
+@pytest.mark.parametrize("magnet", [False, True])
+def test_squeezed_grid_lockstep(grid, magnet):
+    """A grid with nz > 1 and a singleton x or y: the reference squeezes the arrays
+    (FluidArray.m:48, MagnetArray.m:43) and runs the non-singleton extents as dimensions
+    1, 2 with the per-dimension settings by position; device and oracle both relabel."""
+    st = problems.random_state(grid, seed=sum(grid) + magnet, magnet=magnet)
+    ini = {"grid": grid, "dGrid": (0.05, 0.07, 0.06), "gamma": 5.0 / 3.0, "cfl": 0.5,
+           "bcMode": {"x": "trans", "y": ["const", "flip"], "z": "circ"}, "bcInfinity": 20,
+           "fluxLimiter": {"x": "superbee", "y": "vanleer", "z": "minmond"},
+           "mode": {"fluid": True, "magnet": magnet}, "iterMax": 100, "timeMax": 1e9}
+    from imogen_amd import iteration
+    orc = make_oracle(ini, st)
+    run = make_gpu(ini, st)
+    fields = FLUID_FIELDS + (MAG_FIELDS if magnet else [])
+    for it in range(8):
+        orc.iteration()
+        iteration(run)
+        compare(run, orc, fields, 1e-12)
+    run.close()
+    orc.close()
