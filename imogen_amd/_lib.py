@@ -123,8 +123,9 @@ SIGNATURES = {
 
 # Tuning and measurement entry points (include/imogen_hip_ext.h; not part of the drop-in
 # boundary).
-NKIND = 7
-KERNEL_KINDS = ["cfl", "fluid", "mag_adv", "mag_ct", "cellmag", "bookkeeping", "source"]
+NKIND = 8
+KERNEL_KINDS = ["cfl", "fluid", "mag_adv", "mag_ct", "cellmag", "bookkeeping", "source",
+                "mhd_1d"]
 EXTRA = {
     "imh_set_fluid_tile": (_I, [_P, _I, _I, _I, _I]),
     "imh_get_fluid_tile": (_I, [_P, _I, ctypes.POINTER(_I), ctypes.POINTER(_I),
@@ -133,6 +134,7 @@ EXTRA = {
     "imh_set_fluid_kernel": (_I, [_P, _I]),
     "imh_get_fluid_kernel": (_I, [_P, ctypes.POINTER(_I)]),
     "imh_batch_set_fluid_tile": (_I, [_P, _I, _I, _I, _I, _I]),
+    "imh_batch_set_fused_1d": (_I, [_P, _I]),
     "imh_profile_iterations": (_I, [_P, _I, _D, _D, ctypes.POINTER(_I)]),
     "imh_batch_profile_step": (_I, [_P, _I, _D, _D, ctypes.POINTER(_I)]),
     "imh_copy_probe": (_I, [ctypes.c_size_t, _I, _D]),

@@ -25,7 +25,9 @@ COMMON = [
 ]
 
 SOURCES = ["magnet_dispatch.hip", "cfl.hip", "fluid_dispatch.cpp", "fluid_column.hip",
-           "initial.hip", "source.hip", "imh_api.cpp", "batch.cpp", "serve.cpp", "noalloc.hip"]
+           "initial.hip", "source.hip", "imh_api.cpp", "batch.cpp", "serve.cpp", "noalloc.hip",
+           "lane1d.hip"]
+FP32_ONLY = ("lane1d.hip",)
 VARIANT_SRC = "fluid_sweep_variant.hip"
 WAVE_VARIANT_SRC = "fluid_wave_variant.hip"
 MAG_VARIANT_SRC = "magnet_sweep_variant.hip"
@@ -56,10 +58,11 @@ def wave_variants():
 
 def units():
     """(source, object name, extra flags) of every translation unit."""
-    out = [(s, os.path.splitext(s)[0], []) for s in SOURCES]
     # FP64 units keep -ffp-contract=off (the FP64 kernels reproduce the oracle's rounding);
-    # FP32-bulk units may fuse multiply-adds
+    # FP32-bulk units may fuse multiply-adds (the FP32-only 1-D lane too, so that its
+    # bodies round as the per-sweep FP32 kernels do)
     fp32 = ["-ffp-contract=fast"]
+    out = [(s, os.path.splitext(s)[0], fp32 if s in FP32_ONLY else []) for s in SOURCES]
     for e, b, k in fluid_variants():
         out.append((VARIANT_SRC, "fluid_sweep_v%d_%d_%d" % (e, b, k),
                     ["-DIMH_FLUID_ESZ=%d" % e, "-DIMH_FLUID_BS=%d" % b, "-DIMH_FLUID_KMAX=%d" % k]

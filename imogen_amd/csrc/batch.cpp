@@ -28,6 +28,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "imh_common.h"
@@ -39,7 +40,7 @@ using namespace imh;
 namespace {
 
 enum LaunchKind { L_BEGIN, L_CFL_RESET, L_CFL, L_FINALIZE, L_FLUID, L_MAG_ADV, L_MAG_CT, L_CELLMAG, L_END,
-                  L_FORK, L_JOIN, L_SOURCE };
+                  L_FORK, L_JOIN, L_SOURCE, L_LANE1D };
 
 constexpr int MAX_LANES = 2;
 
@@ -64,6 +65,7 @@ int launch_kind_class(LaunchKind k) {
     case L_MAG_CT: return IMH_K_MAG_CT;
     case L_CELLMAG: return IMH_K_CELLMAG;
     case L_SOURCE: return IMH_K_SOURCE;
+    case L_LANE1D: return IMH_K_LANE1D;
     default: return IMH_K_BOOK;
     }
 }
@@ -97,7 +99,26 @@ struct imh_batch {
     hipStream_t lane_st[MAX_LANES] = {};
     hipEvent_t fork_ev = nullptr;
     hipEvent_t join_ev[MAX_LANES] = {};
+    bool fuse_1d = true;               // one launch per iteration for the 1-D MHD lane
+    std::vector<char> fused;           // per simulation: runs in that launch
+    std::vector<int> fused_list;       // those simulations, longest first
+    void* l1args = nullptr;            // Lane1dArgs<float>[fused_list.size()]
+    size_t l1cap = 0;
 };
+
+// A simulation whose whole flux call runs in the 1-D lane launch (lane1d.hip): FP32
+// bulk, one pencil (N x 1 x 1, 3 <= N <= 4096), fluid and magnet both active, no source
+// terms, viscosity, fade ends or spacing lines (those take the general sweep), the time
+// step once per iteration in every simulation of the batch.
+static bool lane1d_ok(const imh_batch* b, const imh_ctx* c) {
+    if (!b->fuse_1d || b->prec != IMH_FP32_BULK || b->any_step_mode) return false;
+    if (c->n[1] != 1 || c->n[2] != 1 || c->n[0] < 3 || c->n[0] > LANE1D_NMAX) return false;
+    if (!c->cfg.fluid_active || !c->cfg.magnet_active || !c->mhd_kernel) return false;
+    if (c->grav || c->rad_type || c->visc_type || c->dline_dev[0]) return false;
+    for (int s = 0; s < 2; ++s)
+        if (c->cfg.bc[s][0] == IMH_BC_FADE) return false;
+    return true;
+}
 
 // Wait until nothing of the batch is in flight: the batch stream (graph replays join
 // their lane branches back into it) and the lane streams.  Every call that changes a
@@ -129,6 +150,7 @@ static void batch_free(imh_batch* b) {
     hipFree(b->cargs);
     hipFree(b->cflargs);
     hipFree(b->tiles);
+    hipFree(b->l1args);
     if (b->owns)
         for (imh_ctx* c : b->ctx) imh_ctx_free(c);
     else
@@ -177,6 +199,46 @@ static int upload_args_t(imh_batch* b) {
     if (e == hipSuccess)
         e = hipMemcpyAsync(b->cflargs, cf.data(), sizeof(CflArgs<T>) * cf.size(),
                            hipMemcpyHostToDevice, b->st);
+    // the 1-D lane: argument blocks with one pencil over the whole workgroup
+    b->fused.assign(n, 0);
+    b->fused_list.clear();
+    if constexpr (std::is_same<T, float>::value) {
+        for (int i = 0; i < n; ++i)
+            if (lane1d_ok(b, b->ctx[i])) {
+                b->fused[i] = 1;
+                b->fused_list.push_back(i);
+            }
+        std::stable_sort(b->fused_list.begin(), b->fused_list.end(), [&](int x, int y) {
+            return b->ctx[x]->n[0] > b->ctx[y]->n[0];
+        });
+        std::vector<Lane1dArgs<T>> la(b->fused_list.size());
+        for (size_t j = 0; j < la.size(); ++j) {
+            const int i = b->fused_list[j];
+            Lane1dArgs<T>& l = la[j];
+            memset(&l, 0, sizeof(l));
+            l.fa = fa[i * 3 + 0];
+            l.fa.PX = 1;
+            l.fa.TL = LANE1D_BS;
+            l.fa.K = LANE1D_KMAX;
+            l.fa.xmajor = 1;
+            for (int dI = 1; dI < 3; ++dI) {
+                l.ma[dI] = ma[i * 9 + 3 * dI + 0];
+                l.ma[dI].PX = 1;
+                l.ma[dI].TL = LANE1D_BS;
+            }
+            for (int k = 0; k < 3; ++k) l.ca[k] = ca[i * 3 + k];
+        }
+        if (e == hipSuccess && la.size() > b->l1cap) {
+            hipFree(b->l1args);
+            b->l1args = nullptr;
+            b->l1cap = 0;
+            e = hipMalloc(&b->l1args, sizeof(Lane1dArgs<T>) * la.size());
+            if (e == hipSuccess) b->l1cap = la.size();
+        }
+        if (e == hipSuccess && !la.empty())
+            e = hipMemcpyAsync(b->l1args, la.data(), sizeof(Lane1dArgs<T>) * la.size(),
+                               hipMemcpyHostToDevice, b->st);
+    }
     if (e == hipSuccess) e = hipStreamSynchronize(b->st);
     return e == hipSuccess ? IMH_OK : hip_fail(e, "batch args upload");
 }
@@ -312,6 +374,17 @@ static int build_plans(imh_batch* b) {
         };
         cfl_block(1);
         fork_join(L_FORK);
+        if (!b->fused_list.empty()) {
+            Launch f{};
+            f.kind = L_LANE1D;
+            f.lane = 0;
+            double cells = 0;
+            for (int i : b->fused_list) cells += double(b->ctx[i]->ncell);
+            // one read and one write of the state (mass, momentum, energy, B) and the
+            // cell-centred B written
+            f.bytes = 19.0 * cells * es;
+            add_launch(p, f, none);
+        }
         for (int oi = 0; oi < 2; ++oi) {
             const int order = oi == 0 ? 1 : -1;
             if (oi == 1 && b->any_step_mode) {
@@ -337,6 +410,7 @@ static int build_plans(imh_batch* b) {
                     size_t magsmem[2] = {0, 0};
                     for (int i = 0; i < n; ++i) {
                         imh_ctx* c = b->ctx[i];
+                        if (b->fused[i]) continue;  // the 1-D lane launch runs it
                         int it = c->host_iteration + p;
                         int dirs[3], mi[3][2];
                         sweep_order(it, order, c->cfg.perms_order, dirs, mi);
@@ -347,7 +421,8 @@ static int build_plans(imh_batch* b) {
                             const FluidTile& ft = c->ftile[X - 1];
                             const int dX = X - 1;
                             const bool gen = c->cfg.bc[0][dX] == IMH_BC_FADE ||
-                                             c->cfg.bc[1][dX] == IMH_BC_FADE || c->visc_type;
+                                             c->cfg.bc[1][dX] == IMH_BC_FADE || c->visc_type ||
+                                             c->dline_dev[dX];
                             // bs 0: the column march (one thread per pencil)
                             const bool col = ft.column && !gen && !c->dline_dev[dX];
                             const int gbs = col ? 0 : ft.bs, gkm = col ? 0 : ft.kmax;
@@ -503,6 +578,7 @@ static hipError_t run_launch(imh_batch* b, const Launch& L, hipStream_t st) {
         return launch_cell_centered_batch<T>((const CellMagArgs<T>*)b->cargs, tl, L.ntiles, nb, st);
     case L_END: return launch_end_iteration(b->sims_dev, n, st);
     case L_SOURCE: return imh_launch_source(b->ctx[L.step], st);
+    case L_LANE1D: return launch_lane1d(b->l1args, (int)b->fused_list.size(), st);
     case L_FORK:
     case L_JOIN: return hipSuccess;  // handled by run_plan
     }
@@ -754,6 +830,15 @@ int imh_batch_set_gravity(imh_batch* b, int sim, const double* potential, size_t
     rc = imh_set_gravity(b->ctx[sim], potential, n, mass_threshold);
     free_plans(b);
     return rc;
+}
+
+int imh_batch_set_fused_1d(imh_batch* b, int enable) {
+    if (!b) return fail(IMH_EINVAL, "null batch");
+    int rc = batch_quiesce(b);
+    if (rc) return rc;
+    b->fuse_1d = enable != 0;
+    free_plans(b);
+    return IMH_OK;
 }
 
 int imh_batch_set_spacing(imh_batch* b, int sim, int dim, const double* line, size_t n) {

@@ -835,9 +835,10 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
                     sL = fade_fix(false, 1, gg, i0 + k, sL);
                 }
                 // 0.5*fluxFactor .* tempFreeze; fluxFactor = dTime ./ DGRID{X} per cell
-                // on a non-uniform grid
-                const T hk = a.dxl ? T(0.5 * (dt / a.dxl[i0 + k < N ? i0 + k : N - 1])) * C
-                                   : hfC;
+                // on a non-uniform grid (general form only)
+                T hk = hfC;
+                if constexpr (GEN)
+                    if (a.dxl) hk = T(0.5 * (dt / a.dxl[i0 + k < N ? i0 + k : N - 1])) * C;
                 qh[v][k] = q[v][k] - hk * (FR[gg][k] - sR + FL[gg][k] - sL);
             }
         });
@@ -970,7 +971,9 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
                     sR = fade_fix(true, 0, gg, i0 + k, sR);
                     sL = fade_fix(false, 1, gg, i0 + k, sL);
                 }
-                const T fk = a.dxl ? T(dt / a.dxl[i0 + k < N ? i0 + k : N - 1]) * C2 : ffC;
+                T fk = ffC;
+                if constexpr (GEN)
+                    if (a.dxl) fk = T(dt / a.dxl[i0 + k < N ? i0 + k : N - 1]) * C2;
                 dq[k] = fk * (FR[gg][k] - sR + FL[gg][k] - sL);
             }
             T un[K];
@@ -1035,6 +1038,7 @@ __global__ void __launch_bounds__(BS) IMH_FLUID_OCCUPANCY fluid_sweep_batch_kern
 // IMH_FLUID_KMAX defined, so the variants build in parallel; launch_fluid_batch
 // (fluid_dispatch.cpp) picks one per sweep dimension.
 // ------------------------------------------------------------------------------------
+#ifndef IMH_FLUID_NO_LAUNCHER
 #if IMH_FLUID_ESZ == 8
 using FluidT = double;
 #else
@@ -1055,5 +1059,6 @@ hipError_t IMH_FLUID_LAUNCHER(IMH_FLUID_ESZ, IMH_FLUID_BS, IMH_FLUID_KMAX)(
 #undef IMH_FL
     return hipErrorInvalidValue;
 }
+#endif
 
 }  // namespace imh

@@ -108,7 +108,7 @@ int fluid_column_pencils();
 // whether a sweep of these arguments needs the general kernel form
 template <typename T>
 inline bool fluid_needs_general(const FluidArgs<T>& a) {
-    return a.init_edges || a.visc_type || mode_at(a.modes, 0, a.d) == IMH_BC_FADE ||
+    return a.init_edges || a.visc_type || a.dxl || mode_at(a.modes, 0, a.d) == IMH_BC_FADE ||
            mode_at(a.modes, 1, a.d) == IMH_BC_FADE;
 }
 
@@ -241,6 +241,25 @@ template <typename T>
 hipError_t launch_cell_centered_batch(const CellMagArgs<T>* args, const BatchTile* tiles,
                                       int ntiles, int nblocks, hipStream_t st);
 int magnet_kmax_for(int N, int TL);
+
+// One launch per batched iteration for the 1-D MHD simulations of an FP32 batch
+// (lane1d.hip): one workgroup per simulation runs its fluid sweeps, magnet sub-steps and
+// cell-centred updates back to back.  The argument blocks carry a tile of one pencil
+// over the whole workgroup.  IMH_LANE1D_BS (a build-time knob) is the workgroup size;
+// each thread holds up to 4096 / IMH_LANE1D_BS cells.
+#ifndef IMH_LANE1D_BS
+#define IMH_LANE1D_BS 512
+#endif
+constexpr int LANE1D_BS = IMH_LANE1D_BS;
+constexpr int LANE1D_NMAX = 4096;
+constexpr int LANE1D_KMAX = LANE1D_NMAX / LANE1D_BS;
+template <typename T>
+struct Lane1dArgs {
+    FluidArgs<T> fa;        // the sweep along x
+    MagnetArgs<T> ma[3];    // [dI] for X = x (dI = 1, 2)
+    CellMagArgs<T> ca[3];   // [k]
+};
+hipError_t launch_lane1d(const void* args, int nsims, hipStream_t st);
 
 // Compiled magnet variants X(element size, KMAX): each a translation unit with the
 // advection and CT kernels for pencils of up to KMAX * TL cells.

@@ -233,17 +233,17 @@ __device__ __noinline__ void mag_init_face(double* edges, const long long* eoff,
 // Advection sub-step for component I along X (one pencil per PX group of threads).
 // ------------------------------------------------------------------------------------
 template <typename T, int KMAX>
-__device__ __forceinline__ void magnet_advect_body(const MagnetArgs<T>& ap, long long tile) {
+__device__ __forceinline__ void magnet_advect_body(const MagnetArgs<T>& ap, long long tile,
+                                                   unsigned char* smem) {
     // private copy: one read of the argument block per tile; stores to the fields
     // cannot alias it, so its fields are not re-read after every store
     const MagnetArgs<T> a = ap;
     if (!a.force && !a.sim->active) return;
-    extern __shared__ unsigned char smem_raw[];
     const int dX = a.dX, dI = a.dI;
     const int nn[3] = {a.n0, a.n1, a.n2};
     const int N = sel3i(dX, a.n0, a.n1, a.n2);
     const int rowlen = N + 4;
-    T* lds = reinterpret_cast<T*>(smem_raw);            // [2][PX][rowlen]
+    T* lds = reinterpret_cast<T*>(smem);                // [2][PX][rowlen]
     __shared__ double s_g[MAG_PXMAX][8];                 // ghost values per call
 
     const int tid = threadIdx.x;
@@ -550,15 +550,15 @@ __device__ __forceinline__ void magnet_advect_body(const MagnetArgs<T>& ap, long
 // mag(X) = mag(X) - dt/DGRID{I} .* flux   (relaxingMagnet.m:56-60).
 // ------------------------------------------------------------------------------------
 template <typename T, int KMAX>
-__device__ __forceinline__ void magnet_ct_body(const MagnetArgs<T>& ap, long long tile) {
+__device__ __forceinline__ void magnet_ct_body(const MagnetArgs<T>& ap, long long tile,
+                                               unsigned char* smem) {
     const MagnetArgs<T> a = ap;  // private copy (see magnet_advect_body)
     if (!a.force && !a.sim->active) return;
-    extern __shared__ unsigned char smem_raw[];
     const int dX = a.dX, dI = a.dI;
     const int nn[3] = {a.n0, a.n1, a.n2};
     const int N = sel3i(dX, a.n0, a.n1, a.n2);
     const int rowlen = N + 4;
-    T* lds = reinterpret_cast<T*>(smem_raw);
+    T* lds = reinterpret_cast<T*>(smem);
     __shared__ double s_g[MAG_PXMAX][2];
 
     const int tid = threadIdx.x;
@@ -661,12 +661,7 @@ __device__ __forceinline__ void magnet_ct_body(const MagnetArgs<T>& ap, long lon
 // mag(k) edge tracker on the upper k face.
 // ------------------------------------------------------------------------------------
 template <typename T>
-__device__ __forceinline__ void cell_centered_body(const CellMagArgs<T>& ap, long long chunk) {
-    const CellMagArgs<T> a = ap;  // private copy (see magnet_advect_body)
-    if (!a.force && !a.sim->active) return;
-    const long long ncell = (long long)a.n0 * a.n1 * a.n2;
-    long long cell = chunk * 256 + threadIdx.x;
-    if (cell >= ncell) return;
+__device__ __forceinline__ void cell_centered_cell(const CellMagArgs<T>& a, long long cell) {
     const int d = a.k;
     const int nn[3] = {a.n0, a.n1, a.n2};
     const bool enabled = d < a.ndims;
@@ -686,6 +681,15 @@ __device__ __forceinline__ void cell_centered_body(const CellMagArgs<T>& ap, lon
                             a.bc_inf);
     if (trk) a.edges[a.eoff[((OBJ_MAG + d) * 2 + 1) * 3 + d] + fi] = et;
     gmem(a.cb)[cell] = T(0.5) * (gmem(a.b)[cell] + sh);
+}
+
+template <typename T>
+__device__ __forceinline__ void cell_centered_body(const CellMagArgs<T>& ap, long long chunk) {
+    const CellMagArgs<T> a = ap;  // private copy (see magnet_advect_body)
+    if (!a.force && !a.sim->active) return;
+    const long long ncell = (long long)a.n0 * a.n1 * a.n2;
+    const long long cell = chunk * 256 + threadIdx.x;
+    if (cell < ncell) cell_centered_cell(a, cell);
 }
 
 // Persistent batched forms: blocks walk a tile list spanning many simulations.
@@ -712,10 +716,11 @@ template <typename T, int KMAX>
 __global__ void __launch_bounds__(MAG_BS) magnet_advect_batch_kernel(const MagnetArgs<T>* args,
                                                                      const BatchTile* tiles,
                                                                      int ntiles) {
+    extern __shared__ unsigned char smem_raw[];
     int arg;
     long long tile;
     for (int r = 0; next_tile(tiles, ntiles, r, arg, tile); ++r) {
-        magnet_advect_body<T, KMAX>(args[arg], tile);
+        magnet_advect_body<T, KMAX>(args[arg], tile, smem_raw);
         __syncthreads();
     }
 }
@@ -724,10 +729,11 @@ template <typename T, int KMAX>
 __global__ void __launch_bounds__(MAG_BS) magnet_ct_batch_kernel(const MagnetArgs<T>* args,
                                                                  const BatchTile* tiles,
                                                                  int ntiles) {
+    extern __shared__ unsigned char smem_raw[];
     int arg;
     long long tile;
     for (int r = 0; next_tile(tiles, ntiles, r, arg, tile); ++r) {
-        magnet_ct_body<T, KMAX>(args[arg], tile);
+        magnet_ct_body<T, KMAX>(args[arg], tile, smem_raw);
         __syncthreads();
     }
 }

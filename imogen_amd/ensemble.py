@@ -87,6 +87,11 @@ class Ensemble:
         check(lib().imh_batch_set_fluid_tile(self._b, int(i), int(X), int(bs), int(kmax),
                                              int(px)))
 
+    def set_fused_1d(self, enable):
+        """One launch per iteration for the 1-D MHD simulations of an FP32 batch (the
+        default) or the per-sweep launches; the results are the same bit for bit."""
+        check(lib().imh_batch_set_fused_1d(self._b, 1 if enable else 0))
+
     def set_gravity(self, i, potential, mass_threshold=0.0):
         """Fixed gravitational potential of simulation i (None turns it off)."""
         if potential is None:

@@ -22,7 +22,8 @@ extern "C" {
 #define IMH_K_CELLMAG 4   /* MagnetArray.updateCellCentered */
 #define IMH_K_BOOK 5      /* per-iteration time bookkeeping */
 #define IMH_K_SOURCE 6    /* source step (gravity, radiation) */
-#define IMH_NKIND 7
+#define IMH_K_LANE1D 7    /* whole flux call of the 1-D MHD simulations of a batch */
+#define IMH_NKIND 8
 
 /* Pin the fluid sweep tile along dim X (1..3): block size, registers per thread (KMAX)
  * and pencils per block; px = 0 restores the automatic choice.  Drops captured graphs. */
@@ -30,6 +31,10 @@ int imh_set_fluid_tile(imh_ctx *ctx, int X, int bs, int kmax, int px);
 int imh_get_fluid_tile(imh_ctx *ctx, int X, int *bs, int *kmax, int *px);
 /* The same for simulation `sim` of a batch (its launch plans are rebuilt). */
 int imh_batch_set_fluid_tile(imh_batch *b, int sim, int X, int bs, int kmax, int px);
+/* 1 (the default): the 1-D MHD simulations of an FP32 batch run their whole flux call in
+ * one launch per iteration (one workgroup each); 0: they take the per-sweep launches.
+ * Both give the same results bit for bit. */
+int imh_batch_set_fused_1d(imh_batch *b, int enable);
 /* Fluid sweep kernel along every dimension: 0 the general sweep (LDS exchange), 1 the
  * wave-spread sweep with the arithmetic of the precision (FP64: every division, bit for
  * bit with the general sweep), 2 the wave-spread sweep with fast FP64 arithmetic (one

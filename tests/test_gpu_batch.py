@@ -73,6 +73,38 @@ def test_batch_fp32_error_vs_oracle():
     ens.close()
 
 
+def test_fused_1d_lane_equals_per_sweep_launches():
+    """The 1-D MHD simulations of an FP32 batch run their whole flux call in one launch
+    per iteration (lane1d.hip); with that launch switched off they take the per-sweep
+    launches.  Same bodies, same order: the fields agree bit for bit, at pencil lengths
+    from 256 to 4096 cells, next to 2-D hydro and 2-D MHD simulations (not fused), and a
+    simulation that ends early (TIMEMAX) stops in both."""
+    sims = [problems.mhd_riemann(n, seed=30 + n) for n in (256, 700, 1500, 3000, 4096)]
+    ini, st = problems.mhd_riemann(1024, seed=7)
+    ini = dict(ini, timeMax=0.02)
+    sims.append((ini, st))
+    sims += [problems.kelvin_helmholtz(64, seed=8), problems.orszag_tang(48)]
+    fused = Ensemble(sims, "fp32")
+    plain = Ensemble(sims, "fp32")
+    plain.set_fused_1d(False)
+    fused.step(3)
+    plain.step(3)
+    pf = fused.profile_step(2)
+    pp = plain.profile_step(2)
+    assert pf["mhd_1d"][2] > 0 and pp["mhd_1d"][2] == 0, (pf["mhd_1d"], pp["mhd_1d"])
+    fused.step(10)
+    plain.step(10)
+    for i, (ini_i, _) in enumerate(sims):
+        assert fused.time(i) == plain.time(i), i
+        for f in fields_of(ini_i):
+            np.testing.assert_array_equal(fused.download(i, f), plain.download(i, f),
+                                          err_msg="sim %d %s" % (i, f))
+    print("1-D lane: profile fused %s, per-sweep fluid %s" % (pf["mhd_1d"], pp["fluid"]),
+          flush=True)
+    fused.close()
+    plain.close()
+
+
 def test_batch_handles_finished_simulations():
     """A simulation whose TIMEMAX is reached stops advancing while the others go on."""
     a = problems.sod(256, time_max=0.01)
