@@ -412,8 +412,31 @@ def serve_requests(n, seed):
     return reqs, kinds
 
 
+def serve_plain_fp64(reqs):
+    """The requests of a serving stream through the plain FP64 path, one after another
+    (imh_run to each request's end time, initial state generated on the device as at
+    admission): the equal-work reference for the serving rate.  Cell updates are counted
+    as the server counts them (2 x cells per completed iteration)."""
+    import imogen_amd
+    cu, t = 0.0, 0.0
+    for r in reqs:
+        run = imogen_amd.initialize(r["ini"], "fp64")
+        run.set_fluid_kernel(PLAIN_SWEEP)
+        run.synchronize()
+        t0 = time.perf_counter()
+        run.init_problem(r["kind"], r.get("params", ()))
+        while run.run_iterations(1000) == 1000:
+            pass
+        run.synchronize()
+        t += time.perf_counter() - t0
+        cu += 2.0 * ncells(r["ini"]) * (run.time.iteration - 1)
+        run.close()
+    return cu / t / 1e6, t
+
+
 def serving_leg(args):
-    """The serving stream at each arrival rate: latency p50 / p99 and Mcell-updates/s."""
+    """The serving stream at each arrival rate: latency p50 / p99 and Mcell-updates/s;
+    the last rate's requests also through the plain FP64 path one by one (equal work)."""
     from imogen_amd.serve import Server, run_stream
     out = []
     for k, rate in enumerate(float(r) for r in args.serve_rates.split(",")):
@@ -429,6 +452,14 @@ def serving_leg(args):
                     "mcell_updates_per_s": round(rep["mcell_updates_per_s"], 1)})
         log("serve rate %.0f/s: p50 %.3f s p99 %.3f s, %.0f Mcell-updates/s" % (
             rate, rep["p50_s"], rep["p99_s"], rep["mcell_updates_per_s"]))
+    last = reqs
+    plain, secs = serve_plain_fp64(last)
+    out[-1]["plain_fp64_sequential"] = {
+        "mcell_updates_per_s": round(plain, 1), "seconds": round(secs, 3),
+        "ratio_serving_over_plain": round(out[-1]["mcell_updates_per_s"] / plain, 3)}
+    log("serve rate %.0f/s requests through plain FP64 one by one: %.0f Mcell-updates/s "
+        "(%.2f s); serving / plain %.2f" % (out[-1]["rate_per_s"], plain, secs,
+                                           out[-1]["mcell_updates_per_s"] / plain))
     return out
 
 

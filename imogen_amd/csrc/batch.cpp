@@ -90,6 +90,7 @@ struct imh_batch {
     size_t tiles_cap = 0;
     std::vector<Launch> plan[6];
     hipGraphExec_t graph[6] = {};
+    int uses[6] = {};                  // eager runs of each plan since it was built
     bool planned = false;
     long long step = 0;
     bool any_step_mode = false;
@@ -135,6 +136,7 @@ static void free_plans(imh_batch* b) {
         if (b->graph[p]) hipGraphExecDestroy(b->graph[p]);
         b->graph[p] = nullptr;
         b->plan[p].clear();
+        b->uses[p] = 0;
     }
     b->planned = false;
 }
@@ -773,6 +775,16 @@ int imh_batch_step(imh_batch* b, int iterations) {
     if (rc) return rc;
     for (int it = 0; it < iterations; ++it) {
         const int p = (int)(b->step % 6);
+        // a plan's first run is eager: short-lived batches (the serving loop's views,
+        // re-formed at every admission or retirement) would capture graphs they replay
+        // once; from the second run on the plan is captured and replayed
+        if (!b->graph[p] && b->uses[p] == 0) {
+            b->uses[p] = 1;
+            if ((rc = run_plan(b, p))) return rc;
+            b->step += 1;
+            for (imh_ctx* c : b->ctx) c->host_iteration += 1;
+            continue;
+        }
         if (!b->graph[p]) {
             hipGraph_t g;
             hipError_t e = hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal);

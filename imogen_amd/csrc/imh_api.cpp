@@ -376,28 +376,18 @@ const char* imh_last_error(void) { return imh::g_last_error.c_str(); }
 
 const char* imh_version(void) { return "imogen-hip 0.1 (gfx950)"; }
 
-int imh_create(const imh_config* cfg_in, imh_ctx** out) {
-    if (!cfg_in || !out) return fail(IMH_EINVAL, "null argument");
-    *out = nullptr;
-    bool squeezed = false;
-    const imh_config sq = squeeze_grid(*cfg_in, &squeezed);
-    const imh_config* cfg = &sq;
-    int rc = validate_config(cfg);
-    if (rc) return rc;
-    hipError_t e = hipSetDevice(cfg->device);
-    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
-    imh_ctx* c = new imh_ctx();
+}  // extern "C"
+
+// Host-side fields of a context from its (squeezed, validated) configuration.
+static int ctx_init_host(imh_ctx* c, const imh_config* cfg, bool squeezed) {
     c->cfg = *cfg;
     c->squeezed = squeezed;
     for (int d = 0; d < 3; ++d) c->n[d] = cfg->grid[d];
     c->ndims = cfg->grid[2] > 1 ? 3 : 2;
     c->ncell = (long long)c->n[0] * c->n[1] * c->n[2];
-    if (c->ncell >= (1LL << 31)) {
-        const long long nc = c->ncell;
-        delete c;
+    if (c->ncell >= (1LL << 31))
         return fail(IMH_ENOTSUP, "a simulation of %lld cells (the kernels index cells with 32 "
-                    "bits; split it into grids below 2^31 cells)", nc);
-    }
+                    "bits; split it into grids below 2^31 cells)", c->ncell);
     for (int d = 0; d < 3; ++d) c->nface[d] = c->ncell / c->n[d];
     for (int d = 0; d < 3; ++d) {
         int L = d;  // Edges.ACTIVE(D) with linear index D = d+1 into the 2x3 array
@@ -408,18 +398,23 @@ int imh_create(const imh_config* cfg_in, imh_ctx** out) {
     }
     if (cfg->bc_infinity >= 4 && cfg->bc_infinity <= MAX_FADE)
         fade_coefficients(cfg->bc_infinity, c->fade);
-    const size_t es = esize(c);
-    size_t bytes = (size_t)c->ncell * es;
-#define CHK(x, what)                                    \
-    do {                                                \
-        hipError_t _e = (x);                            \
-        if (_e != hipSuccess) {                         \
-            int _r = (_e == hipErrorOutOfMemory) ? fail(IMH_ENOMEM, "%s: out of device memory", what) \
-                                                 : hip_fail(_e, what); \
-            ctx_free(c);                                \
-            return _r;                                  \
-        }                                               \
+    layout_edges(c);
+    return IMH_OK;
+}
+
+#define CHK(x, what)                                                                       \
+    do {                                                                                   \
+        hipError_t _e = (x);                                                               \
+        if (_e != hipSuccess)                                                              \
+            return (_e == hipErrorOutOfMemory) ? fail(IMH_ENOMEM, "%s: out of device memory", \
+                                                      what)                                \
+                                               : hip_fail(_e, what);                       \
     } while (0)
+
+// Device memory of a context: its stream, the fields, edge stores and bookkeeping.
+static int ctx_alloc(imh_ctx* c) {
+    const imh_config* cfg = &c->cfg;
+    const size_t bytes = (size_t)c->ncell * esize(c);
     CHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking), "hipStreamCreate");
     {
         hipDeviceProp_t prop;
@@ -432,13 +427,10 @@ int imh_create(const imh_config* cfg_in, imh_ctx** out) {
     }
     for (int i = 0; i < IMH_NFIELDS; ++i) CHK(hipMalloc(&c->f[i], bytes), "hipMalloc(field)");
     if (cfg->magnet_active) CHK(hipMalloc(&c->scratch, bytes), "hipMalloc(scratch)");
-    layout_edges(c);
-    CHK(hipMalloc(&c->edges, c->edges_len * sizeof(double)), "hipMalloc(edges)");
-    CHK(hipMemset(c->edges, 0, c->edges_len * sizeof(double)), "hipMemset(edges)");
+    c->edges_cap = c->edges_len;
+    CHK(hipMalloc(&c->edges, c->edges_cap * sizeof(double)), "hipMalloc(edges)");
     CHK(hipMalloc(&c->eoff_dev, sizeof(c->eoff)), "hipMalloc(eoff)");
-    CHK(hipMemcpy(c->eoff_dev, c->eoff, sizeof(c->eoff), hipMemcpyHostToDevice), "eoff upload");
     CHK(hipMalloc(&c->fade_dev, sizeof(c->fade)), "hipMalloc(fade)");
-    CHK(hipMemcpy(c->fade_dev, c->fade, sizeof(c->fade), hipMemcpyHostToDevice), "fade upload");
     CHK(hipMalloc(&c->sim, sizeof(SimDev)), "hipMalloc(sim)");
     CHK(hipMalloc(&c->par, sizeof(SimParams)), "hipMalloc(par)");
     CHK(hipMalloc(&c->sim_ptr, sizeof(SimDev*)), "hipMalloc(sim_ptr)");
@@ -451,6 +443,18 @@ int imh_create(const imh_config* cfg_in, imh_ctx** out) {
         "hipHostMalloc(args)");
     CHK(hipMalloc(&c->arg_dev, ARG_SLOT * ARG_RING), "hipMalloc(args)");
     c->cap_host = new char[ARG_SLOT * CAP_SLOTS];
+    return IMH_OK;
+}
+
+// Configuration-dependent device data and tiles of a context whose memory is in place
+// (fresh or recycled): zeroed edge stores, offsets, fade coefficients, time state and
+// parameters, sweep tiles.
+static int ctx_configure(imh_ctx* c) {
+    const imh_config* cfg = &c->cfg;
+    const size_t es = esize(c);
+    CHK(hipMemset(c->edges, 0, c->edges_cap * sizeof(double)), "hipMemset(edges)");
+    CHK(hipMemcpy(c->eoff_dev, c->eoff, sizeof(c->eoff), hipMemcpyHostToDevice), "eoff upload");
+    CHK(hipMemcpy(c->fade_dev, c->fade, sizeof(c->fade), hipMemcpyHostToDevice), "fade upload");
     SimDev sd;
     memset(&sd, 0, sizeof(sd));
     sd.iteration = 1;  // imogen.m:50
@@ -471,7 +475,6 @@ int imh_create(const imh_config* cfg_in, imh_ctx** out) {
         c->mtile[d] = choose_mag_tile(c->n[d]);
         if (c->n[d] >= 3 && c->ftile[d].K > c->ftile[d].kmax) {
             int maxn = c->ftile[d].kmax * c->ftile[d].bs;
-            ctx_free(c);
             return fail(IMH_ENOTSUP, "pencils of %d cells along dim %d exceed the longest "
                         "compiled sweep tile (%d cells at this precision)", cfg->grid[d], d + 1, maxn);
         }
@@ -489,19 +492,107 @@ int imh_create(const imh_config* cfg_in, imh_ctx** out) {
                     ok = true;
                 }
             }
-            if (!ok) {
-                ctx_free(c);
+            if (!ok)
                 return fail(IMH_ENOTSUP, "wormhole BC along dim %d: pencils of %d cells need "
                             "a sweep tile of their own, and a wormhole pairs two pencils per "
                             "tile", d + 1, cfg->grid[d]);
-            }
         }
     }
     c->host_iteration = 1;
+    return IMH_OK;
+}
 #undef CHK
+
+extern "C" {
+
+int imh_create(const imh_config* cfg_in, imh_ctx** out) {
+    if (!cfg_in || !out) return fail(IMH_EINVAL, "null argument");
+    *out = nullptr;
+    bool squeezed = false;
+    const imh_config sq = squeeze_grid(*cfg_in, &squeezed);
+    const imh_config* cfg = &sq;
+    int rc = validate_config(cfg);
+    if (rc) return rc;
+    hipError_t e = hipSetDevice(cfg->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    imh_ctx* c = new imh_ctx();
+    if ((rc = ctx_init_host(c, cfg, squeezed)) || (rc = ctx_alloc(c)) || (rc = ctx_configure(c))) {
+        ctx_free(c);
+        return rc;
+    }
     *out = c;
     return IMH_OK;
 }
+
+}  // extern "C"
+
+// A new context for cfg on the device memory of `old` (a finished context that is no
+// longer used): same cell count, precision, field set (the magnet's scratch) and room
+// for the edge stores.  Everything else starts fresh, as from imh_create; `old` is
+// consumed whether or not the call succeeds.  IMH_ENOTSUP (old untouched and not
+// consumed) when the memory does not fit.
+int imh_ctx_recycle(imh_ctx* old, const imh_config* cfg_in, imh_ctx** out) {
+    if (!old || !cfg_in || !out) return fail(IMH_EINVAL, "null argument");
+    *out = nullptr;
+    bool squeezed = false;
+    const imh_config sq = squeeze_grid(*cfg_in, &squeezed);
+    int rc = validate_config(&sq);
+    if (rc) return rc;
+    imh_ctx* c = new imh_ctx();
+    if ((rc = ctx_init_host(c, &sq, squeezed))) {
+        delete c;
+        return rc;
+    }
+    if (c->ncell != old->ncell || sq.precision != old->cfg.precision ||
+        sq.device != old->cfg.device || (sq.magnet_active && !old->scratch) ||
+        c->edges_len > old->edges_cap || old->external_stream) {
+        delete c;
+        return fail(IMH_ENOTSUP, "the context's memory does not fit this configuration");
+    }
+    hipError_t e = hipStreamSynchronize(old->st);
+    if (e != hipSuccess) {
+        delete c;
+        return hip_fail(e, "sync");
+    }
+    // move the device memory over; the old context keeps (and frees) everything else
+    c->st = old->st;
+    old->st = nullptr;
+    c->num_cu = old->num_cu;
+    c->max_lds = old->max_lds;
+    for (int i = 0; i < IMH_NFIELDS; ++i) {
+        c->f[i] = old->f[i];
+        old->f[i] = nullptr;
+    }
+    c->scratch = old->scratch;
+    old->scratch = nullptr;
+    c->edges = old->edges;
+    c->edges_cap = old->edges_cap;
+    old->edges = nullptr;
+#define IMH_MOVE(field) \
+    c->field = old->field; \
+    old->field = nullptr;
+    IMH_MOVE(eoff_dev)
+    IMH_MOVE(fade_dev)
+    IMH_MOVE(sim)
+    IMH_MOVE(par)
+    IMH_MOVE(sim_ptr)
+    IMH_MOVE(par_ptr)
+    IMH_MOVE(csum_part)
+    IMH_MOVE(csum_out)
+    IMH_MOVE(arg_host)
+    IMH_MOVE(arg_dev)
+    IMH_MOVE(cap_host)
+#undef IMH_MOVE
+    ctx_free(old);
+    if ((rc = ctx_configure(c))) {
+        ctx_free(c);
+        return rc;
+    }
+    *out = c;
+    return IMH_OK;
+}
+
+extern "C" {
 
 int imh_destroy(imh_ctx* c) {
     if (!c) return IMH_OK;

@@ -65,6 +65,7 @@ struct imh_ctx {
     void* scratch = nullptr;        // mag(I).flux(X) for the CT step
     double* edges = nullptr;
     size_t edges_len = 0;
+    size_t edges_cap = 0;               // doubles allocated at edges (>= edges_len)
     long long eoff[imh::NOBJ][2][3];
     long long* eoff_dev = nullptr;
     imh::SimDev* sim = nullptr;
@@ -196,3 +197,6 @@ int imh_source_internal(imh_ctx* c);
 hipError_t imh_launch_source(imh_ctx* c, hipStream_t st);
 double source_bytes(const imh_ctx* c);
 void imh_ctx_free(imh_ctx* c);
+// A new context for cfg on the device memory of a finished one (same cell count,
+// precision, field set, room for the edge stores); see imh_api.cpp.
+int imh_ctx_recycle(imh_ctx* old, const imh_config* cfg, imh_ctx** out);

@@ -56,7 +56,18 @@ struct imh_server {
     SimDev* host_sd = nullptr;  // pinned, max_active entries
     std::chrono::steady_clock::time_point t0;
     long long cell_updates = 0;  // 2 * cells per completed iteration, finished requests
+    // contexts of released requests, kept for the next admission of the same cell count
+    // (their device memory is reused instead of allocated), oldest first
+    struct Pooled {
+        imh_ctx* ctx;
+        size_t bytes;
+    };
+    std::deque<Pooled> pool;
+    size_t pooled = 0;           // device bytes held by the pool
 };
+
+// Released contexts kept for reuse at most (each holds its grid's fields).
+constexpr size_t POOL_MAX = 64;
 
 extern "C" imh_ctx* imh_batch_context(imh_batch* b, int sim);
 hipStream_t imh_batch_stream(imh_batch* b);
@@ -67,10 +78,41 @@ static double now_s(const imh_server* s) {
 
 static void release_request(imh_server* s, Request& r) {
     if (r.ctx) {
-        imh_destroy(r.ctx);
+        hipStreamSynchronize(r.ctx->st);
+        if (s->pool.size() < POOL_MAX && !r.ctx->external_stream) {
+            s->pool.push_back({r.ctx, r.bytes});
+            s->pooled += r.bytes;
+        } else {
+            imh_destroy(r.ctx);
+        }
         r.ctx = nullptr;
         s->used -= std::min(s->used, r.bytes);
     }
+}
+
+// A context for request r: a pooled one of the same cell count, precision and field set
+// when there is one (its memory reused), else a new one.  Pooled contexts are freed,
+// oldest first, while the budget needs their memory.
+static int context_for(imh_server* s, Request& r) {
+    const long long ncell = (long long)r.n;
+    for (size_t i = 0; i < s->pool.size(); ++i) {
+        imh_ctx* p = s->pool[i].ctx;
+        if (p->ncell != ncell || p->cfg.precision != r.cfg.precision ||
+            (r.cfg.magnet_active && !p->cfg.magnet_active))
+            continue;
+        const size_t pb = s->pool[i].bytes;
+        const int rc = imh_ctx_recycle(p, &r.cfg, &r.ctx);
+        if (rc == IMH_ENOTSUP) continue;  // edge stores do not fit: untouched, kept
+        s->pool.erase(s->pool.begin() + (long)i);
+        s->pooled -= std::min(s->pooled, pb);
+        return rc;
+    }
+    while (!s->pool.empty() && s->used + s->pooled + r.bytes > s->budget) {
+        imh_destroy(s->pool.front().ctx);
+        s->pooled -= std::min(s->pooled, s->pool.front().bytes);
+        s->pool.pop_front();
+    }
+    return imh_create(&r.cfg, &r.ctx);
 }
 
 // Admit queued requests in order while a slot is free and the budget allows.
@@ -89,7 +131,7 @@ static int admit(imh_server* s) {
         }
         const long long id = s->queue.front();
         s->queue.pop_front();
-        int rc = imh_create(&r.cfg, &r.ctx);
+        int rc = context_for(s, r);
         if (rc == IMH_OK && r.ic_kind > 0) {
             rc = imh_init_problem(r.ctx, r.ic_kind, r.ic_params.data(), (int)r.ic_params.size());
         } else if (rc == IMH_OK) {
@@ -195,7 +237,12 @@ int imh_serve_create(int device, int precision, size_t hbm_budget, int max_activ
 int imh_serve_destroy(imh_server* s) {
     if (!s) return IMH_OK;
     if (s->view) imh_batch_destroy(s->view);
-    for (auto& kv : s->reqs) release_request(s, kv.second);
+    for (auto& kv : s->reqs) {
+        if (kv.second.ctx) imh_destroy(kv.second.ctx);
+        kv.second.ctx = nullptr;
+    }
+    for (auto& p : s->pool) imh_destroy(p.ctx);
+    s->pool.clear();
     if (s->host_sd) (void)hipHostFree(s->host_sd);
     delete s;
     return IMH_OK;
