@@ -26,7 +26,7 @@ COMMON = [
 
 SOURCES = ["magnet_dispatch.hip", "cfl.hip", "fluid_dispatch.cpp", "fluid_column.hip",
            "initial.hip", "source.hip", "imh_api.cpp", "batch.cpp", "serve.cpp", "noalloc.hip",
-           "lane1d.hip"]
+           "lane1d.hip", "fluid_long.hip"]
 FP32_ONLY = ("lane1d.hip",)
 VARIANT_SRC = "fluid_sweep_variant.hip"
 WAVE_VARIANT_SRC = "fluid_wave_variant.hip"

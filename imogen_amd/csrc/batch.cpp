@@ -631,6 +631,12 @@ static int run_plan(imh_batch* b, int p) {
 extern "C" {
 
 static int batch_create_common(imh_batch* b) {
+    for (const imh_ctx* c : b->ctx)
+        for (int d = 0; d < 3; ++d)
+            if (c->long_dim[d])
+                return fail(IMH_ENOTSUP, "pencils of %d cells along dim %d: simulations with "
+                            "pencils longer than a workgroup holds run on their own (imh_run)",
+                            c->n[d], d + 1);
     int rc = batch_setup(b);
     if (rc) return rc;
     return IMH_OK;

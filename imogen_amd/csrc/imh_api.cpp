@@ -234,6 +234,7 @@ static void ctx_free(imh_ctx* c) {
     if (c->fade_dev) hipFree(c->fade_dev);
     if (c->csum_part) hipFree(c->csum_part);
     if (c->csum_out) hipFree(c->csum_out);
+    if (c->long_scratch) hipFree(c->long_scratch);
     if (c->grav) hipFree(c->grav);
     if (c->grav_ghost) hipFree(c->grav_ghost);
     if (c->st && !c->external_stream) hipStreamDestroy(c->st);
@@ -473,10 +474,27 @@ static int ctx_configure(imh_ctx* c) {
         // pencils (3-D grids); the autotuner can switch it on for a dimension
         c->ftile[d].column = column_possible(cfg, d) && c->nface[d] >= (1 << 16);
         c->mtile[d] = choose_mag_tile(c->n[d]);
+        c->long_dim[d] = false;
         if (c->n[d] >= 3 && c->ftile[d].K > c->ftile[d].kmax) {
-            int maxn = c->ftile[d].kmax * c->ftile[d].bs;
-            return fail(IMH_ENOTSUP, "pencils of %d cells along dim %d exceed the longest "
-                        "compiled sweep tile (%d cells at this precision)", cfg->grid[d], d + 1, maxn);
+            // longer than a workgroup holds: the multi-launch sweep (fluid_long.hip)
+            if (cfg->bc[0][d] == IMH_BC_FADE || cfg->bc[1][d] == IMH_BC_FADE ||
+                cfg->bc[0][d] == IMH_BC_WORMHOLE)
+                return fail(IMH_ENOTSUP, "pencils of %d cells along dim %d with a fade or "
+                            "wormhole boundary (the sweep of pencils longer than %d cells takes "
+                            "neither)", cfg->grid[d], d + 1,
+                            c->ftile[d].kmax * c->ftile[d].bs);
+            if (cfg->magnet_active && c->n[d] > MAG_PENCIL_MAX)
+                return fail(IMH_ENOTSUP, "magnet pencils of %d cells along dim %d (at most %d)",
+                            cfg->grid[d], d + 1, MAG_PENCIL_MAX);
+            c->long_dim[d] = true;
+            const size_t lb = fluid_long_bytes(es, c->ncell, c->nface[d]);
+            if (lb > c->long_bytes) {
+                if (c->long_scratch) hipFree(c->long_scratch);
+                c->long_scratch = nullptr;
+                c->long_bytes = 0;
+                CHK(hipMalloc(&c->long_scratch, lb), "hipMalloc(long sweep)");
+                c->long_bytes = lb;
+            }
         }
         // a lower wormhole boundary pairs pencils inside a tile: long pencils take two
         // pencils of 256 threads per 512-thread block (more cells per thread)
@@ -848,7 +866,9 @@ static int fluid_sweep_t(imh_ctx* c, int d, int force) {
     // Hydro sweeps only: a run of the MHD instantiations (the only ones that spill
     // registers to scratch) left the GPU faulted once, cause not found; MHD sweeps take the
     // general form until it is.
-    if (c->sweep_kind != IMH_SWEEP_GENERAL && !fluid_needs_general(a) && !c->mhd_kernel && !a.dxl &&
+    if (c->long_dim[d]) {
+        e = launch_fluid_long<T>(a, c->long_scratch, c->ncell, c->mhd_kernel, c->st);
+    } else if (c->sweep_kind != IMH_SWEEP_GENERAL && !fluid_needs_general(a) && !c->mhd_kernel && !a.dxl &&
         wave_bc_ok(a.modes, d) && wave_tile(c->n[d], d, (int)sizeof(T), c->max_lds, &wbs, &wpx, &wtl, &wlds) &&
         (c->sweep_kind != IMH_SWEEP_AUTO_FAST || (a.npencil + wpx - 1) / wpx >= 4LL * c->num_cu)) {
         // the wave-spread sweep: its own tile in the argument block
@@ -1299,6 +1319,10 @@ int imh_set_radiation(imh_ctx* c, int type, double exponent, double initial_maxi
 int imh_set_viscosity(imh_ctx* c, int type, double linear, double quadratic) {
     if (!c) return fail(IMH_EINVAL, "null context");
     if (type < 0 || type > 2) return fail(IMH_EINVAL, "viscosity type %d", type);
+    for (int d = 0; d < 3; ++d)
+        if (type != 0 && c->long_dim[d])
+            return fail(IMH_ENOTSUP, "artificial viscosity with pencils of %d cells along dim "
+                        "%d (the sweep of long pencils takes no viscosity)", c->n[d], d + 1);
     if (type == 2)  // the fade blend reaches into every one of its shifted arrays
         for (int d = 0; d < c->ndims; ++d)
             for (int s = 0; s < 2; ++s) {

@@ -66,6 +66,11 @@ struct imh_ctx {
     double* edges = nullptr;
     size_t edges_len = 0;
     size_t edges_cap = 0;               // doubles allocated at edges (>= edges_len)
+    // dimensions whose pencils exceed the fused sweep's tile: swept by fluid_long.hip,
+    // with its scratch area (fluid_long_bytes of the longest such sweep)
+    bool long_dim[3] = {false, false, false};
+    void* long_scratch = nullptr;
+    size_t long_bytes = 0;
     long long eoff[imh::NOBJ][2][3];
     long long* eoff_dev = nullptr;
     imh::SimDev* sim = nullptr;

@@ -241,6 +241,21 @@ template <typename T>
 hipError_t launch_cell_centered_batch(const CellMagArgs<T>* args, const BatchTile* tiles,
                                       int ntiles, int nblocks, hipStream_t st);
 int magnet_kmax_for(int N, int TL);
+constexpr int MAG_PENCIL_MAX = 4096;  // magnet pencils: 256 threads x 16 cells
+
+// The fluid sweep along pencils longer than the fused sweep's tile (fluid_long.hip):
+// five launches over the cells, the pencil-wide quantities in a scratch area of
+// fluid_long_bytes (U^{n+1/2}, two pencil maxima, the end ghosts).
+constexpr int FLUID_LONG_GH = 8;  // ghost values per pencil and field
+inline size_t fluid_long_align(size_t b) { return (b + 255) & ~size_t(255); }
+inline size_t fluid_long_bytes(size_t esz, long long ncell, long long npencil) {
+    return 5 * fluid_long_align(esz * (size_t)ncell) +
+           fluid_long_align(sizeof(unsigned long long) * 2 * (size_t)npencil) +
+           sizeof(double) * 5 * FLUID_LONG_GH * (size_t)npencil;
+}
+template <typename T>
+hipError_t launch_fluid_long(const FluidArgs<T>& a, void* scratch, long long ncell, bool mhd,
+                             hipStream_t st);
 
 // One launch per batched iteration for the 1-D MHD simulations of an FP32 batch
 // (lane1d.hip): one workgroup per simulation runs its fluid sweeps, magnet sub-steps and
