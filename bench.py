@@ -111,6 +111,7 @@ def trace_summary(trace_csv, bench_log):
     rows = sorted(csv.DictReader(open(trace_csv)), key=lambda r: int(r["Start_Timestamp"]))
 
     def short(n):
+        n = n.replace("(anonymous namespace)::", "")
         return n.split("(")[0].replace("void ", "").replace("imh::", "")
 
     def table(sel, title, ref):
