@@ -633,10 +633,10 @@ extern "C" {
 static int batch_create_common(imh_batch* b) {
     for (const imh_ctx* c : b->ctx)
         for (int d = 0; d < 3; ++d)
-            if (c->long_dim[d])
+            if (long_sweep(c, d))
                 return fail(IMH_ENOTSUP, "pencils of %d cells along dim %d: simulations with "
-                            "pencils longer than a workgroup holds run on their own (imh_run)",
-                            c->n[d], d + 1);
+                            "pencils longer than a workgroup holds, or Caramana-Shashkov-Whalen "
+                            "viscosity at a fade end, run on their own (imh_run)", c->n[d], d + 1);
     int rc = batch_setup(b);
     if (rc) return rc;
     return IMH_OK;
@@ -886,6 +886,12 @@ int imh_batch_set_viscosity(imh_batch* b, int sim, int type, double linear, doub
     // free_plans destroys graphs that may still run
     int rc = batch_quiesce(b);
     if (rc) return rc;
+    const imh_ctx* c = b->ctx[sim];
+    if (type == 2)
+        for (int d = 0; d < c->ndims; ++d)
+            if (c->cfg.bc[0][d] == IMH_BC_FADE || c->cfg.bc[1][d] == IMH_BC_FADE)
+                return fail(IMH_ENOTSUP, "Caramana-Shashkov-Whalen viscosity at a fade end runs "
+                            "on single contexts (imh_run), not in a batch");
     rc = imh_set_viscosity(b->ctx[sim], type, linear, quadratic);
     free_plans(b);
     return rc;

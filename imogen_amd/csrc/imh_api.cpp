@@ -447,6 +447,26 @@ static int ctx_alloc(imh_ctx* c) {
     return IMH_OK;
 }
 
+// 3-D grids do not take the multi-launch sweep (fluid_long.hip): its one 3-D run (MHD,
+// Caramana-Shashkov-Whalen viscosity at fade ends, gravity and radiation) faulted the
+// device while a serial host run of the same kernels on the same configuration kept every
+// access in bounds; until that is explained, such sweeps are refused.
+static const char* const LONG_3D_REFUSAL =
+    "the multi-launch sweep (pencils longer than a workgroup holds, or this viscosity at a "
+    "fade end) runs on 1-D and 2-D grids only";
+
+// The scratch area of the multi-launch sweep (fluid_long.hip) along d
+static hipError_t ensure_long_scratch(imh_ctx* c, int d) {
+    const size_t lb = fluid_long_bytes(esize(c), c->ncell, c->nface[d]);
+    if (lb <= c->long_bytes) return hipSuccess;
+    if (c->long_scratch) hipFree(c->long_scratch);
+    c->long_scratch = nullptr;
+    c->long_bytes = 0;
+    hipError_t e = hipMalloc(&c->long_scratch, lb);
+    if (e == hipSuccess) c->long_bytes = lb;
+    return e;
+}
+
 // Configuration-dependent device data and tiles of a context whose memory is in place
 // (fresh or recycled): zeroed edge stores, offsets, fade coefficients, time state and
 // parameters, sweep tiles.
@@ -477,24 +497,16 @@ static int ctx_configure(imh_ctx* c) {
         c->long_dim[d] = false;
         if (c->n[d] >= 3 && c->ftile[d].K > c->ftile[d].kmax) {
             // longer than a workgroup holds: the multi-launch sweep (fluid_long.hip)
-            if (cfg->bc[0][d] == IMH_BC_FADE || cfg->bc[1][d] == IMH_BC_FADE ||
-                cfg->bc[0][d] == IMH_BC_WORMHOLE)
-                return fail(IMH_ENOTSUP, "pencils of %d cells along dim %d with a fade or "
-                            "wormhole boundary (the sweep of pencils longer than %d cells takes "
-                            "neither)", cfg->grid[d], d + 1,
-                            c->ftile[d].kmax * c->ftile[d].bs);
+            if (cfg->bc[0][d] == IMH_BC_WORMHOLE)
+                return fail(IMH_ENOTSUP, "pencils of %d cells along dim %d with a wormhole "
+                            "boundary (the sweep of pencils longer than %d cells does not pair "
+                            "pencils)", cfg->grid[d], d + 1, c->ftile[d].kmax * c->ftile[d].bs);
             if (cfg->magnet_active && c->n[d] > MAG_PENCIL_MAX)
                 return fail(IMH_ENOTSUP, "magnet pencils of %d cells along dim %d (at most %d)",
                             cfg->grid[d], d + 1, MAG_PENCIL_MAX);
+            if (c->ndims == 3) return fail(IMH_ENOTSUP, "%s", LONG_3D_REFUSAL);
             c->long_dim[d] = true;
-            const size_t lb = fluid_long_bytes(es, c->ncell, c->nface[d]);
-            if (lb > c->long_bytes) {
-                if (c->long_scratch) hipFree(c->long_scratch);
-                c->long_scratch = nullptr;
-                c->long_bytes = 0;
-                CHK(hipMalloc(&c->long_scratch, lb), "hipMalloc(long sweep)");
-                c->long_bytes = lb;
-            }
+            CHK(ensure_long_scratch(c, d), "hipMalloc(long sweep)");
         }
         // a lower wormhole boundary pairs pencils inside a tile: long pencils take two
         // pencils of 256 threads per 512-thread block (more cells per thread)
@@ -866,7 +878,7 @@ static int fluid_sweep_t(imh_ctx* c, int d, int force) {
     // Hydro sweeps only: a run of the MHD instantiations (the only ones that spill
     // registers to scratch) left the GPU faulted once, cause not found; MHD sweeps take the
     // general form until it is.
-    if (c->long_dim[d]) {
+    if (long_sweep(c, d)) {
         e = launch_fluid_long<T>(a, c->long_scratch, c->ncell, c->mhd_kernel, c->st);
     } else if (c->sweep_kind != IMH_SWEEP_GENERAL && !fluid_needs_general(a) && !c->mhd_kernel && !a.dxl &&
         wave_bc_ok(a.modes, d) && wave_tile(c->n[d], d, (int)sizeof(T), c->max_lds, &wbs, &wpx, &wtl, &wlds) &&
@@ -1319,23 +1331,26 @@ int imh_set_radiation(imh_ctx* c, int type, double exponent, double initial_maxi
 int imh_set_viscosity(imh_ctx* c, int type, double linear, double quadratic) {
     if (!c) return fail(IMH_EINVAL, "null context");
     if (type < 0 || type > 2) return fail(IMH_EINVAL, "viscosity type %d", type);
-    for (int d = 0; d < 3; ++d)
-        if (type != 0 && c->long_dim[d])
-            return fail(IMH_ENOTSUP, "artificial viscosity with pencils of %d cells along dim "
-                        "%d (the sweep of long pencils takes no viscosity)", c->n[d], d + 1);
-    if (type == 2)  // the fade blend reaches into every one of its shifted arrays
+    if (type == 2)
         for (int d = 0; d < c->ndims; ++d)
             for (int s = 0; s < 2; ++s) {
-                if (c->cfg.bc[s][d] == IMH_BC_FADE)
-                    return fail(IMH_ENOTSUP, "the Caramana-Shashkov-Whalen viscosity with a "
-                                "fade boundary is not supported");
                 if (c->cfg.bc[s][d] == IMH_BC_WORMHOLE)
                     return fail(IMH_ENOTSUP, "the Caramana-Shashkov-Whalen viscosity with a "
                                 "wormhole boundary is not supported");
+                if (c->cfg.bc[s][d] == IMH_BC_FADE && c->ndims == 3)
+                    return fail(IMH_ENOTSUP, "the Caramana-Shashkov-Whalen viscosity at a fade "
+                                "end: %s", LONG_3D_REFUSAL);
             }
     hipError_t e = hipStreamSynchronize(c->st);
     if (e != hipSuccess) return hip_fail(e, "sync");
     drop_graphs(c);
+    // Caramana-Shashkov-Whalen with fade ends: every shifted array of the viscosity is
+    // blended near both ends, which the multi-launch sweep evaluates cell by cell
+    if (type == 2)
+        for (int d = 0; d < c->ndims; ++d)
+            if ((c->cfg.bc[0][d] == IMH_BC_FADE || c->cfg.bc[1][d] == IMH_BC_FADE) &&
+                c->n[d] >= 3 && (e = ensure_long_scratch(c, d)) != hipSuccess)
+                return hip_fail(e, "hipMalloc(long sweep)");
     c->visc_type = type;
     c->visc_lin = linear;
     c->visc_quad = quadratic;

@@ -122,6 +122,13 @@ struct imh_ctx {
     int visc_type = 0;
     double visc_lin = 0.0, visc_quad = 0.0;
 };
+// The fluid sweep along d takes the multi-launch form (fluid_long.hip): pencils longer
+// than a workgroup holds, or the Caramana-Shashkov-Whalen viscosity with a fade end.
+inline bool long_sweep(const imh_ctx* c, int d) {
+    return c->long_dim[d] ||
+           (c->visc_type == 2 && (c->cfg.bc[0][d] == IMH_BC_FADE || c->cfg.bc[1][d] == IMH_BC_FADE));
+}
+
 
 namespace imh {
 constexpr size_t ARG_SLOT = 512;       // bytes per staged argument block

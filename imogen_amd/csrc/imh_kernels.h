@@ -246,12 +246,16 @@ constexpr int MAG_PENCIL_MAX = 4096;  // magnet pencils: 256 threads x 16 cells
 // The fluid sweep along pencils longer than the fused sweep's tile (fluid_long.hip):
 // five launches over the cells, the pencil-wide quantities in a scratch area of
 // fluid_long_bytes (U^{n+1/2}, two pencil maxima, the end ghosts).
-constexpr int FLUID_LONG_GH = 8;  // ghost values per pencil and field
+constexpr int FLUID_LONG_GH = 8;   // ghost values per pencil and field
+constexpr int FLUID_LONG_EN = 36;  // row ends per pencil and field: 3 stages x 2 rows x 6
+constexpr int FLUID_LONG_VG = 11;  // viscosity ghosts per pencil
 inline size_t fluid_long_align(size_t b) { return (b + 255) & ~size_t(255); }
 inline size_t fluid_long_bytes(size_t esz, long long ncell, long long npencil) {
-    return 5 * fluid_long_align(esz * (size_t)ncell) +
+    return 6 * fluid_long_align(esz * (size_t)ncell) +
            fluid_long_align(sizeof(unsigned long long) * 2 * (size_t)npencil) +
-           sizeof(double) * 5 * FLUID_LONG_GH * (size_t)npencil;
+           fluid_long_align(sizeof(double) * 5 * FLUID_LONG_GH * (size_t)npencil) +
+           fluid_long_align(sizeof(double) * 5 * FLUID_LONG_EN * (size_t)npencil) +
+           sizeof(double) * FLUID_LONG_VG * (size_t)npencil;
 }
 template <typename T>
 hipError_t launch_fluid_long(const FluidArgs<T>& a, void* scratch, long long ncell, bool mhd,

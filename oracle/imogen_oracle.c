@@ -1174,13 +1174,6 @@ int orc_set_viscosity(orc_sim *s, int type, double linear, double quadratic) {
         snprintf(s->err, sizeof(s->err), "viscosity type %d is not supported", type);
         return -1;
     }
-    if (type == 2)
-        for (int d = 0; d < s->ndims; ++d)
-            for (int side = 0; side < 2; ++side)
-                if (s->cfg.bc[side][d] == ORC_BC_FADE) {
-                    snprintf(s->err, sizeof(s->err), "viscosity with a fade BC is not supported");
-                    return -1;
-                }
     s->visc_type = type;
     s->visc_lin = linear;
     s->visc_quad = quadratic;

@@ -4,8 +4,11 @@ The fused sweep keeps a pencil in one workgroup (4096 cells); longer ones take t
 multi-launch sweep with device-wide pencil maxima of the freezing speed.  Lockstep with
 the oracle at <= 1e-12 relative L-inf per iteration (observed 0): Sod on 8192 cells, every
 boundary mode on 5000-cell pencils, 2-D grids whose x pencils are long (the first sweep's
-edge records on the y faces included), the fluid's B terms (magnet off, field on); a
-graph replay equals the eager iterations; FP32 bulk within 1e-5 of FP64; the refusals.
+edge records on the y faces included), fade ends (the shifted fluxes blended over
+bcInfinity cells), the fluid's B terms (magnet off, field on), both artificial
+viscosities (fade ends included); a
+graph replay equals the eager iterations; FP32 bulk within 1e-5 of FP64; the refusals
+(a lower wormhole end, magnet pencils past 4096 cells, 3-D grids, batches).
 """
 import numpy as np
 import pytest
@@ -53,7 +56,8 @@ def test_sod_8192():
 
 
 @pytest.mark.parametrize("bc", ["trans", "const", "mirror", "flip", "wall", "zero", "circ",
-                                ["trans", "wall"], ["zero", "mirror"], ["circ", "const"]])
+                                ["trans", "wall"], ["zero", "mirror"], ["circ", "const"],
+                                "fade", ["fade", "trans"], ["mirror", "fade"]])
 def test_boundary_modes_5000(bc):
     ini, st = box((5000, 1, 1), {"x": bc, "y": "circ", "z": "circ"}, seed=3,
                   limiter="superbee")
@@ -70,6 +74,26 @@ def test_fluid_b_terms_on_long_pencils():
     ini, st = box((4500, 1, 1), {"x": "trans", "y": "circ", "z": "circ"}, seed=7,
                   magnet_field=True)
     lockstep(ini, st, 6)
+
+
+@pytest.mark.parametrize("kind,bc", [
+    ("neumann_richtmyer", "trans"), ("neumann_richtmyer", ["const", "fade"]),
+    ("caramana_shashkov_whalen", "const"), ("caramana_shashkov_whalen", ["trans", "wall"]),
+    ("caramana_shashkov_whalen", ["fade", "mirror"]),
+])
+def test_viscosity_on_long_pencils(kind, bc):
+    ini, st = box((5000, 1, 1), {"x": bc, "y": "circ", "z": "circ"}, seed=9,
+                  magnet_field=True)
+    orc = make_oracle(ini, st)
+    run = make_gpu(ini, st)
+    orc.set_viscosity(kind, 0.2, 0.5)
+    run.set_viscosity(kind, 0.2, 0.5)
+    for it in range(6):
+        orc.iteration()
+        iteration(run)
+        compare(run, orc, FLUID_FIELDS, TOL)
+    run.close()
+    orc.close()
 
 
 def test_graph_replay_equals_eager():
@@ -99,8 +123,11 @@ def test_fp32_close_to_fp64():
 
 
 def test_refusals():
-    ini, st = box((5000, 1, 1), {"x": "fade", "y": "circ", "z": "circ"}, seed=1)
-    with pytest.raises(Exception, match="fade"):
+    ini, st = box((5000, 8, 4), {"x": "wormhole", "y": "circ", "z": "circ"}, seed=1)
+    with pytest.raises(Exception, match="wormhole"):
+        make_gpu(ini, st)
+    ini, st = box((5000, 4, 3), {"x": "trans", "y": "circ", "z": "circ"}, seed=1)
+    with pytest.raises(Exception, match="1-D and 2-D grids"):
         make_gpu(ini, st)
     ini, st = problems.mhd_riemann(256, seed=1)
     ini = dict(ini, grid=(5000, 1, 1))

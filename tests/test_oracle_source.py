@@ -265,12 +265,7 @@ def test_viscosity_acts_in_compression_and_sod_still_converges():
     assert errs[(0.1, 0.25)] < 0.05 and errs[(0.1, 0.25)] < 1.5 * errs[None], errs
 
 
-def test_viscosity_rejects_fade_and_unknown_types():
-    o = O.Oracle(O.make_config((32, 24, 1), (0.1, 0.1, 0.1), bc="fade", bc_infinity=8))
-    o.set_state(*uniform((32, 24, 1)))
-    o.set_viscosity("neumann_richtmyer", 0.1, 0.2)
-    with pytest.raises(RuntimeError):
-        o.set_viscosity("caramana_shashkov_whalen", 0.1, 0.2)
+def test_viscosity_rejects_unknown_types():
     o = O.Oracle(O.make_config((32, 24, 1), (0.1, 0.1, 0.1)))
     o.set_state(*uniform((32, 24, 1)))
     with pytest.raises(RuntimeError):
@@ -340,6 +335,38 @@ def test_csw_term_closed_form_on_linear_compression():
     vs = np.where(x < 0.5, 0.5, -0.5)
     qs = _term("caramana_shashkov_whalen", vs, c1, c2, rho=rho, P=P)
     assert qs.max() > 0 and np.argmax(qs) in (n // 2 - 2, n // 2 - 1, n // 2, n // 2 + 1)
+
+
+@pytest.mark.parametrize("kind", ["caramana_shashkov_whalen", "neumann_richtmyer"])
+def test_viscosity_term_with_fade_ends(kind):
+    """fade_shift.m = constant_shift, then the first and last bcInfinity cells blended
+    toward the shifted array's end cell with coeff (1 on the first ceil(f/4) cells).  The
+    term at a cell reads only the arrays at that cell, so: away from the fade cells it is
+    the constant-BC term; for CSW (every input shifted) on the coeff = 1 cells next to an
+    end every shifted array holds its end value, so the term there is the constant-BC
+    term of the end cell."""
+    n, f, rho, P = 48, 8, 1.1, 0.9
+    x = (np.arange(n) + 0.5) / n
+    v = 0.3 - 0.8 * x + 0.05 * np.sin(13.0 * x)
+    c1, c2 = 0.4, 1.2
+    qc = _term(kind, v, c1, c2, rho=rho, P=P)
+    o = O.Oracle(O.make_config((n, 1, 1), (1.0 / n,) * 3, gamma=GAMMA,
+                               bc={"x": "fade", "y": "circ", "z": "circ"}, bc_infinity=f))
+    o.set_state(*_line_state(v, rho, P))
+    o.set_viscosity(kind, c1, c2)
+    qf = o.viscosity_term(1).ravel()
+    assert np.all(qc[f:n - f] != 0)
+    np.testing.assert_array_equal(qf[f:n - f], qc[f:n - f])
+    flat = np.flatnonzero(O.fade_coeffs(f) == 1.0)
+    assert flat.tolist() == [0, 1]
+    if kind == "caramana_shashkov_whalen":
+        np.testing.assert_array_equal(qf[flat], np.full(flat.size, qc[0]))
+        np.testing.assert_array_equal(qf[n - 1 - flat], np.full(flat.size, qc[n - 1]))
+    else:
+        # vm = p(i+1) - p(i) with only p(i+1) shifted: on cell 1 the shifted p holds
+        # the shifted array's end value p(1), so vm = 0 there
+        assert qf[0] == qc[0] and qf[1] == 0.0 and qf[n - 1] == qc[n - 1]
+    assert np.all(np.isfinite(qf)) and np.any(qf[:f] != qc[:f]) and np.any(qf[-f:] != qc[-f:])
 
 
 def test_csw_zero_strengths_is_the_inviscid_run():
