@@ -59,9 +59,11 @@ def wave_variants():
 def units():
     """(source, object name, extra flags) of every translation unit."""
     # FP64 units keep -ffp-contract=off (the FP64 kernels reproduce the oracle's rounding);
-    # FP32-bulk units may fuse multiply-adds (the FP32-only 1-D lane too, so that its
-    # bodies round as the per-sweep FP32 kernels do)
-    fp32 = ["-ffp-contract=fast"]
+    # FP32-bulk units fuse the multiply-adds written within one expression (the FP32-only
+    # 1-D lane too).  Not "fast": that also fuses across statements wherever the
+    # surrounding code allows, so one body could round differently in two kernels that
+    # inline it (the batched and single-simulation results must agree bit for bit).
+    fp32 = ["-ffp-contract=on"]
     out = [(s, os.path.splitext(s)[0], fp32 if s in FP32_ONLY else []) for s in SOURCES]
     for e, b, k in fluid_variants():
         out.append((VARIANT_SRC, "fluid_sweep_v%d_%d_%d" % (e, b, k),

@@ -105,6 +105,51 @@ def test_fused_1d_lane_equals_per_sweep_launches():
     plain.close()
 
 
+def _as_fp32(st):
+    """The state rounded to FP32; B on a grid of 2^-12, so that the cell-centred averages
+    of neighbouring face values are exact in FP32 as in FP64 (the CFL reads them)."""
+    f = lambda a: np.asarray(a, dtype=np.float32).astype(np.float64)
+    q = lambda b: np.round(np.asarray(b, dtype=np.float64) * 4096.0) / 4096.0
+    return {"mass": f(st["mass"]), "mom": [f(m) for m in st["mom"]], "ener": f(st["ener"]),
+            "mag": [q(b) for b in st["mag"]]}
+
+
+def _cold_gas(grid, seed):
+    """Pressure a millionth of the kinetic and magnetic energy: E - rho v^2 / 2 - B^2 / 2
+    cancels to six digits."""
+    st = problems.random_state(grid, seed=seed, magnet=True)
+    v2 = sum(m * m for m in st["mom"]) / st["mass"] ** 2
+    b2 = sum(b * b for b in st["mag"])
+    st["ener"] = (0.5 * st["mass"] * v2 + 0.5 * b2) * (1.0 + 1e-6)
+    return st
+
+
+@pytest.mark.parametrize("case", ["random_mhd", "cold_gas", "kh", "ot", "sedov"])
+def test_fp32_cfl_equals_the_fp64_evaluation(case):
+    """The FP32-bulk CFL reduction evaluates the speeds in FP64 from the FP32 state: the
+    maxima, hence dt, equal the oracle's time update on the same FP32-rounded state (whose
+    cell-centred B is then the same in both) bit for bit, a cold, nearly pressureless gas
+    included."""
+    if case in ("random_mhd", "cold_gas"):
+        grid = (40, 33, 5)
+        st = problems.random_state(grid, seed=3) if case == "random_mhd" else _cold_gas(grid, 4)
+        ini = {"grid": grid, "dGrid": (0.05, 0.06, 0.07), "gamma": 5.0 / 3.0, "cfl": 0.5,
+               "bcMode": "circ", "bcInfinity": 20, "fluxLimiter": "vanleer", "minMass": 1e-5,
+               "mode": {"fluid": True, "magnet": True}, "iterMax": 100, "timeMax": 1e9}
+    else:
+        ini, st = {"kh": lambda: problems.kelvin_helmholtz(96, seed=2),
+                   "ot": lambda: problems.orszag_tang(64),
+                   "sedov": lambda: problems.sedov(24, time_max=1.0)}[case]()
+    st = _as_fp32(st)
+    run = make_gpu(ini, st, "fp32")
+    orc = make_oracle(ini, st)
+    run.time.update(None, None, None, None, 1)
+    orc.time_update(1)
+    assert run.time.dTime == orc.dtime, (run.time.dTime, orc.dtime)
+    run.close()
+    orc.close()
+
+
 def test_batch_handles_finished_simulations():
     """A simulation whose TIMEMAX is reached stops advancing while the others go on."""
     a = problems.sod(256, time_max=0.01)
