@@ -221,7 +221,10 @@ static int upload_args_t(imh_batch* b) {
             l.fa = fa[i * 3 + 0];
             l.fa.PX = 1;
             l.fa.TL = LANE1D_BS;
-            l.fa.K = LANE1D_KMAX;
+            // cells per thread: half the longest pencil's where that holds the pencil (the
+            // lane kernel has a body for each)
+            l.fa.K = b->ctx[i]->n[0] <= LANE1D_BS * (LANE1D_KMAX / 2) ? LANE1D_KMAX / 2
+                                                                      : LANE1D_KMAX;
             l.fa.xmajor = 1;
             for (int dI = 1; dI < 3; ++dI) {
                 l.ma[dI] = ma[i * 9 + 3 * dI + 0];

@@ -14,8 +14,10 @@
 //
 // The bodies are those of the per-launch kernels (fluid_sweep_body, magnet_advect_body,
 // magnet_ct_body, cell_centered_cell), on argument blocks whose tile is one pencil over
-// the workgroup (LANE1D_BS threads, up to LANE1D_KMAX cells each), so the results are
-// those of the per-launch plan bit for bit.  The time step comes from the batch-wide CFL launch before, the iteration
+// the workgroup (LANE1D_BS threads of LANE1D_KMAX or LANE1D_KMAX / 2 cells each: the
+// bodies are instantiated for both, and a pencil that the smaller holds runs it), so the
+// results are those of the per-launch plan bit for bit.  (One body per further halving
+// would not fit: each instantiation keeps LDS arrays of its own.)  The time step comes from the batch-wide CFL launch before, the iteration
 // counters from the batch-wide launch after.  The state stays in the workgroup's L2
 // between the steps.
 #define IMH_FLUID_ESZ 4
@@ -27,14 +29,10 @@ namespace imh {
 
 namespace {
 
-template <typename T>
-__global__ void __launch_bounds__(LANE1D_BS) lane1d_kernel(const Lane1dArgs<T>* args,
-                                                           int nsims) {
-    // the magnet rows [2][rowlen] of the longest pencil
-    __shared__ __attribute__((aligned(16))) unsigned char s_mag[2 * (LANE1D_NMAX + 4) * sizeof(T)];
-    const int s = blockIdx.x;
-    if (s >= nsims) return;
-    const Lane1dArgs<T>& a = args[s];
+// The flux call of one simulation with KC cells per thread (the fluid and magnet bodies
+// instantiated for that many): a short pencil does not run the longest pencil's body.
+template <typename T, int KC>
+__device__ __forceinline__ void lane1d_run(const Lane1dArgs<T>& a, unsigned char* s_mag) {
     auto cellmag = [&](int k) {
         const CellMagArgs<T> ca = a.ca[k];
         if (!ca.force && !ca.sim->active) return;
@@ -45,15 +43,15 @@ __global__ void __launch_bounds__(LANE1D_BS) lane1d_kernel(const Lane1dArgs<T>* 
     // magnetFlux(X=1, I=[3 2]) (order -1).  One call site per body (they are large).
     for (int step = 0; step < 4; ++step) {
         if (step == 0 || step == 3) {
-            fluid_sweep_body<T, LANE1D_BS, LANE1D_KMAX, true, false>(a.fa, 0);
+            fluid_sweep_body<T, LANE1D_BS, KC, true, false>(a.fa, 0);
         } else {
             const int d1 = step == 1 ? 1 : 2, d2 = 3 - d1;
             for (int j = 0; j < 4; ++j) {
                 const int dI = j < 2 ? d1 : d2;
                 if ((j & 1) == 0)
-                    magnet_advect_body<T, LANE1D_KMAX>(a.ma[dI], 0, s_mag);
+                    magnet_advect_body<T, KC>(a.ma[dI], 0, s_mag);
                 else
-                    magnet_ct_body<T, LANE1D_KMAX>(a.ma[dI], 0, s_mag);
+                    magnet_ct_body<T, KC>(a.ma[dI], 0, s_mag);
                 __syncthreads();
             }
             // updateCellCentered of the last I and of mag(X)
@@ -62,6 +60,22 @@ __global__ void __launch_bounds__(LANE1D_BS) lane1d_kernel(const Lane1dArgs<T>* 
         }
         __syncthreads();
     }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(LANE1D_BS) lane1d_kernel(const Lane1dArgs<T>* args,
+                                                           int nsims) {
+    // the magnet rows [2][rowlen] of the longest pencil
+    __shared__ __attribute__((aligned(16))) unsigned char s_mag[2 * (LANE1D_NMAX + 4) * sizeof(T)];
+    const int s = blockIdx.x;
+    if (s >= nsims) return;
+    const Lane1dArgs<T>& a = args[s];
+    // cells per thread of this simulation (its fluid tile's K), uniform over the block
+    const int kc = __builtin_amdgcn_readfirstlane(a.fa.K);
+    if (kc <= LANE1D_KMAX / 2)
+        lane1d_run<T, LANE1D_KMAX / 2>(a, s_mag);
+    else
+        lane1d_run<T, LANE1D_KMAX>(a, s_mag);
 }
 
 }  // namespace
