@@ -941,13 +941,19 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
             put_ends(0, gg, FR[gg]);
             put_ends(1, gg, FL[gg]);
         });
-        // U^n of the group's fields, re-read (cache-resident) rather than kept live, all
-        // loads issued before the exchange: the addresses are clamped into the pencil and
-        // the pencil range, so the loads need no guard (only the stores do)
+        // U^n of the group's fields.  One group of all five fields (the common form with
+        // few cells per thread): the registers loaded at the start, kept live (measured:
+        // the re-read, 16-byte row segments in y sweeps and past the XCD's L2 by now,
+        // cost more than the spills of keeping them: 2-D sweep 202 -> 194 us).  Else
+        // re-read, all loads issued before the exchange: the addresses are clamped into
+        // the pencil and the pencil range, so the loads need no guard (only the stores do)
         T un_g[G][K];
         for_group(V0C, [&](auto VC, int gv) __attribute__((always_inline)) {
             constexpr int v = decltype(VC)::value;
-            if (vec) {
+            if constexpr (G == 5 && !GEN) {
+#pragma unroll
+                for (int k = 0; k < K; ++k) un_g[v - V0][k] = q[v][k];
+            } else if (vec) {
                 chunk_load<T, K>(a.q[v] + vbase, un_g[v - V0]);
             } else {
 #pragma unroll
