@@ -55,6 +55,7 @@ struct Launch {
     int ntiles;
     double bytes;   // algorithmic HBM bytes of the launch
     int lane;       // -1: the batch stream; else the lane's stream
+    bool reset;     // begin: also clear the CFL slots (the reduction follows)
 };
 
 int launch_kind_class(LaunchKind k) {
@@ -343,6 +344,7 @@ static int build_plans(imh_batch* b) {
         L.lane = -1;
         std::vector<BatchTile> none;
         L.kind = L_BEGIN;
+        L.reset = true;  // the first CFL reduction follows
         add_launch(p, L, none);
         auto cfl_block = [&](int step) {
             std::vector<BatchTile> tl;
@@ -355,10 +357,12 @@ static int build_plans(imh_batch* b) {
                 for (int k = 0; k < 3; ++k) nf += c->mag_is_zero[k] ? 0 : 1;
                 by += double(nf) * c->ncell * es;
             }
-            Launch r{};
-            r.kind = L_CFL_RESET;
-            r.lane = -1;
-            add_launch(p, r, none);
+            if (step > 1) {  // the begin launch cleared the slots for step 1
+                Launch r{};
+                r.kind = L_CFL_RESET;
+                r.lane = -1;
+                add_launch(p, r, none);
+            }
             Launch c{};
             c.kind = L_CFL;
             c.lane = -1;
@@ -563,7 +567,7 @@ static hipError_t run_launch(imh_batch* b, const Launch& L, hipStream_t st) {
     // a multiple of 8 blocks: the XCD-range schedule (xcd_round_tile)
     const int nb = ((std::max(1, std::min(L.ntiles, b->ncu * 4)) + 7) / 8) * 8;
     switch (L.kind) {
-    case L_BEGIN: return launch_begin_iteration(b->sims_dev, b->pars_dev, n, 0, st);
+    case L_BEGIN: return launch_begin_iteration(b->sims_dev, b->pars_dev, n, 0, st, L.reset ? 1 : 0);
     case L_CFL_RESET: return launch_cfl_reset(b->sims_dev, n, st);
     case L_CFL: return launch_cfl_batch<T>((const CflArgs<T>*)b->cflargs, tl, L.ntiles, nb, st);
     case L_FINALIZE: return launch_time_finalize(b->sims_dev, b->pars_dev, n, L.step, st);

@@ -75,6 +75,36 @@ __device__ __forceinline__ void cfl_body(const CflArgs<T>& a, long long c0, long
 
 constexpr long long CFL_CHUNK = 256 * 32;  // cells per tile of a batched launch
 
+// dt of one simulation from its reduced speeds cmax_bits[3] (TimeManager.m:65-98).
+__device__ inline void finalize_sim(SimDev& d, const SimParams& q, int step,
+                                    const unsigned long long (&cmax_bits)[3]) {
+    if (!d.active) return;
+    if (step > 1 && q.update_mode != 0) return;
+    double cmax = 1e-2;
+    int gi = 0;
+    for (int k = 0; k < 3; ++k) {
+        double c = __longlong_as_double((long long)cmax_bits[k]);
+        if (c > cmax) {
+            cmax = c;
+            gi = k + 1;
+        }
+    }
+    if (gi < 1) {
+        d.error = IMH_ECFL;
+        d.active = 0;
+        return;
+    }
+    d.grid_index = gi;
+    double upd = double(q.update_mode + 1);
+    d.dt = q.cfl * q.dgrid[gi - 1] / cmax;
+    double newtime = d.time + upd * d.dt;
+    if (newtime > q.time_max) {
+        d.dt = (q.time_max - d.time) / upd;
+        newtime = q.time_max;
+    }
+    d.time = newtime;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) cfl_batch_kernel(const CflArgs<T>* args,
                                                         const BatchTile* tiles, int ntiles) {
@@ -110,14 +140,18 @@ long long cfl_chunk() { return CFL_CHUNK; }
 // pointers (a batch is a view over independently allocated simulations).
 
 // TimeManager.running (TimeManager.m:36-39), evaluated at the top of the loop body.
+// reset: also clear the CFL slots of the active simulations (the reset before the CFL
+// reduction, done here when that reduction follows).
 __global__ void begin_iteration_kernel(SimDev* const* sim, const SimParams* const* p, int nsims,
-                                       int unconditional) {
+                                       int unconditional, int reset) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nsims) return;
     SimDev& d = *sim[s];
     const SimParams& q = *p[s];
     int running = (d.iteration <= q.iter_max) && (d.time < q.time_max) && d.error == 0;
     d.active = (unconditional || running) ? 1 : 0;
+    if (reset && d.active)
+        for (int k = 0; k < 3; ++k) d.cmax_bits[k] = 0ull;
 }
 
 __global__ void cfl_reset_kernel(SimDev* const* sim, int nsims) {
@@ -134,32 +168,8 @@ __global__ void time_finalize_kernel(SimDev* const* sim, const SimParams* const*
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nsims) return;
     SimDev& d = *sim[s];
-    const SimParams& q = *p[s];
-    if (!d.active) return;
-    if (step > 1 && q.update_mode != 0) return;
-    double cmax = 1e-2;
-    int gi = 0;
-    for (int k = 0; k < 3; ++k) {
-        double c = __longlong_as_double((long long)d.cmax_bits[k]);
-        if (c > cmax) {
-            cmax = c;
-            gi = k + 1;
-        }
-    }
-    if (gi < 1) {
-        d.error = IMH_ECFL;
-        d.active = 0;
-        return;
-    }
-    d.grid_index = gi;
-    double upd = double(q.update_mode + 1);
-    d.dt = q.cfl * q.dgrid[gi - 1] / cmax;
-    double newtime = d.time + upd * d.dt;
-    if (newtime > q.time_max) {
-        d.dt = (q.time_max - d.time) / upd;
-        newtime = q.time_max;
-    }
-    d.time = newtime;
+    const unsigned long long cm[3] = {d.cmax_bits[0], d.cmax_bits[1], d.cmax_bits[2]};
+    finalize_sim(d, *p[s], step, cm);
 }
 
 // TimeManager.step: iteration = iteration + 1.
@@ -172,9 +182,9 @@ __global__ void end_iteration_kernel(SimDev* const* sim, int nsims) {
 static inline int nblk(int n) { return (n + 255) / 256; }
 
 hipError_t launch_begin_iteration(SimDev* const* sim, const SimParams* const* p, int nsims,
-                                  int unconditional, hipStream_t st) {
+                                  int unconditional, hipStream_t st, int reset) {
     hipLaunchKernelGGL(begin_iteration_kernel, dim3(nblk(nsims)), dim3(256), 0, st, sim, p, nsims,
-                       unconditional);
+                       unconditional, reset);
     return hipGetLastError();
 }
 hipError_t launch_cfl_reset(SimDev* const* sim, int nsims, hipStream_t st) {

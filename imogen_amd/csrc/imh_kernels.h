@@ -311,7 +311,7 @@ long long cfl_chunk();
 // Device-side bookkeeping kernels (cfl.hip), one thread per simulation; `sim` and `p`
 // are device arrays of per-simulation pointers.
 hipError_t launch_begin_iteration(SimDev* const* sim, const SimParams* const* p, int nsims,
-                                  int unconditional, hipStream_t st);
+                                  int unconditional, hipStream_t st, int reset = 0);
 hipError_t launch_cfl_reset(SimDev* const* sim, int nsims, hipStream_t st);
 hipError_t launch_time_finalize(SimDev* const* sim, const SimParams* const* p, int nsims,
                                 int step, hipStream_t st);
