@@ -222,10 +222,12 @@ static int upload_args_t(imh_batch* b) {
             l.fa = fa[i * 3 + 0];
             l.fa.PX = 1;
             l.fa.TL = LANE1D_BS;
-            // cells per thread: half the longest pencil's where that holds the pencil (the
-            // lane kernel has a body for each)
-            l.fa.K = b->ctx[i]->n[0] <= LANE1D_BS * (LANE1D_KMAX / 2) ? LANE1D_KMAX / 2
-                                                                      : LANE1D_KMAX;
+            // cells per thread: the smallest of the longest pencil's, its half and its
+            // quarter that holds the pencil (the lane kernel has a body for each)
+            const int N = b->ctx[i]->n[0];
+            l.fa.K = N <= LANE1D_BS * (LANE1D_KMAX / 4)   ? LANE1D_KMAX / 4
+                     : N <= LANE1D_BS * (LANE1D_KMAX / 2) ? LANE1D_KMAX / 2
+                                                           : LANE1D_KMAX;
             l.fa.xmajor = 1;
             for (int dI = 1; dI < 3; ++dI) {
                 l.ma[dI] = ma[i * 9 + 3 * dI + 0];

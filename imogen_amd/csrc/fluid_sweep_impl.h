@@ -164,7 +164,7 @@ __device__ __noinline__ void csw_side_ghosts(int mode, int up, int d, int N, int
 // Max over each pencil of the block (NaN-ignoring): wave shuffles, then one LDS pass.
 // Takes the tile geometry by value: a reference to the argument block here would keep
 // the block's private copy in scratch memory.
-template <typename T>
+template <typename T, int PXM>
 __device__ __forceinline__ T pencil_reduce_max(T v, int xmajor, int TL, int PX, int tid, int px,
                                                T* s_red) {
     const int wave = tid >> 6;
@@ -174,15 +174,15 @@ __device__ __forceinline__ T pencil_reduce_max(T v, int xmajor, int TL, int PX, 
     } else {
         for (int off = PX; off < 64; off <<= 1) v = fmax(v, __shfl_xor(v, off, 64));
     }
-    s_red[wave * PXMAX + px] = v;
+    s_red[wave * PXM + px] = v;
     __syncthreads();
     T m = v;
     const int nw = blockDim.x >> 6;
     if (xmajor) {
         int w0 = (px * TL) >> 6, w1 = (px * TL + TL - 1) >> 6;
-        for (int w = w0; w <= w1; ++w) m = fmax(m, s_red[w * PXMAX + px]);
+        for (int w = w0; w <= w1; ++w) m = fmax(m, s_red[w * PXM + px]);
     } else {
-        for (int w = 0; w < nw; ++w) m = fmax(m, s_red[w * PXMAX + px]);
+        for (int w = 0; w < nw; ++w) m = fmax(m, s_red[w * PXM + px]);
     }
     return m;
 }
@@ -274,7 +274,10 @@ constexpr int fluid_group() {
 // GEN = false is the common case and keeps the kernel free of calls (calls cost
 // registers): no fade boundary and no first assignment of the flux objects.  GEN = true
 // handles those too, with their code out of line.
-template <typename T, int BS, int KMAX, bool MHD, bool GEN>
+//
+// PXM: the most pencils a tile holds (the size of the per-pencil LDS arrays; the 1-D
+// lane's tiles hold one).
+template <typename T, int BS, int KMAX, bool MHD, bool GEN, int PXM = PXMAX>
 __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long long tile) {
     const FluidArgs<T> a = ap;  // private copy: stores to the fields cannot alias it
     if (!a.force && !a.sim->active) return;
@@ -284,21 +287,21 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
     // chunk boundary values of the group: [field*4 + q], q = 0 first FR, 1 last FR,
     // 2 first FL, 3 last FL
     __shared__ __attribute__((aligned(16))) T s_x[4 * G][BS];
-    __shared__ double s_ends[2][G][PXMAX][6];   // FR / FL pencil ends (a[0..2], a[N-3..N-1])
-    __shared__ double s_gh[PXMAX][G][4];        // ghosts: FR lower, FL lower, FR upper, FL upper
-    __shared__ T s_red[2][(BS / 64) * PXMAX];
+    __shared__ double s_ends[2][G][PXM][6];     // FR / FL pencil ends (a[0..2], a[N-3..N-1])
+    __shared__ double s_gh[PXM][G][4];          // ghosts: FR lower, FL lower, FR upper, FL upper
+    __shared__ T s_red[2][(BS / 64) * PXM];
     // edge trackers of the block's pencils: [px][side*16 + role*5 + field], roles lower:
     // store.fluxR, fluxR, fluxL; upper: store.fluxL, fluxR, fluxL
-    __shared__ double s_trk[PXMAX][32];
+    __shared__ double s_trk[PXM][32];
     // artificial viscosity (general form only): first momentum value of each chunk, the
     // pencil ends of the momentum row and its upper ghost
     __shared__ T s_mv[GEN ? BS : 1];
-    __shared__ double s_vend[GEN ? PXMAX : 1][6];
-    __shared__ double s_vgh[GEN ? PXMAX : 1];
+    __shared__ double s_vend[GEN ? PXM : 1][6];
+    __shared__ double s_vgh[GEN ? PXM : 1];
     // Caramana-Shashkov-Whalen rows at the chunk edges (S first/last, m first/last, V at
     // k = 0, 1, K-2, K-1) and at the pencil ends (cells 0..2, N-3..N-1)
     __shared__ T s_cw[GEN ? 8 : 1][GEN ? BS : 1];
-    __shared__ double s_cend[GEN ? PXMAX : 1][3][6];
+    __shared__ double s_cend[GEN ? PXM : 1][3][6];
 
     const int tid = threadIdx.x;
     const int PX = a.PX, TL = a.TL;
@@ -645,7 +648,7 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
         const bool trk_any = trk_lo || trk_hi;
         double* const xe = reinterpret_cast<double*>(&s_x[0][0]);
         constexpr int XE_ENDS = 48, XE_OUT = 10;
-        static_assert(sizeof(s_x) >= sizeof(double) * PXMAX * (XE_ENDS + XE_OUT),
+        static_assert(sizeof(s_x) >= sizeof(double) * PXM * (XE_ENDS + XE_OUT),
                       "exchange rows too small for the CSW pencil ends");
         if (pen_ok) {
 #pragma unroll
@@ -680,7 +683,7 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
                     tp[r] = a.edges + a.eoff[(o * 2 + up) * 3 + d] + slot_face(p);
                 }
                 csw_side_ghosts(mode, up, d, N, a.active_lin, f, g, tp, xe + p * XE_ENDS,
-                                xe + PXMAX * XE_ENDS + p * XE_OUT + up * 5);
+                                xe + PXM * XE_ENDS + p * XE_OUT + up * 5);
             }
         }
         if (trk_any) __syncthreads();
@@ -691,7 +694,7 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
             if (j < 0 || j >= N) {
                 const int mode = j < 0 ? mlo : mhi;
                 if (mode == IMH_BC_TRANS || mode == IMH_BC_WALL) {
-                    const double* ov = xe + PXMAX * XE_ENDS + px * XE_OUT + (j < 0 ? 0 : 5);
+                    const double* ov = xe + PXM * XE_ENDS + px * XE_OUT + (j < 0 ? 0 : 5);
                     if (row == 1) return T(ov[0]);
                     if (row == 0) return T(ov[1]);
                     if (j < 0) return T(o == -1 ? ov[4] : (i == 0 ? ov[2] : ov[3]));
@@ -796,7 +799,7 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
     };
 
     // ================= predictor: W and C from U^n (wFluidFlux.m) ====================
-    const T C = pencil_reduce_max(cell_speeds(q), a.xmajor, TL, PX, tid, px, s_red[0]);
+    const T C = pencil_reduce_max<T, PXM>(cell_speeds(q), a.xmajor, TL, PX, tid, px, s_red[0]);
     viscosity(q, OBJ_PRIM, false);
     const T rC = M::rcp(C);
     const T hfC = T(hf) * C;
@@ -875,7 +878,7 @@ __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long lo
     for (int v = 0; v < 5; ++v) pad_fill(qh[v]);
 
     // ================= corrector: W' and C' from U^(n+1/2) ===========================
-    const T C2 = pencil_reduce_max(cell_speeds(qh), a.xmajor, TL, PX, tid, px, s_red[1]);
+    const T C2 = pencil_reduce_max<T, PXM>(cell_speeds(qh), a.xmajor, TL, PX, tid, px, s_red[1]);
     viscosity(qh, OBJ_FSTORE, a.init_edges != 0);
     const T rC2 = M::rcp(C2);
     const T ffC = T(ff) * C2;

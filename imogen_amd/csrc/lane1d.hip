@@ -14,10 +14,11 @@
 //
 // The bodies are those of the per-launch kernels (fluid_sweep_body, magnet_advect_body,
 // magnet_ct_body, cell_centered_cell), on argument blocks whose tile is one pencil over
-// the workgroup (LANE1D_BS threads of LANE1D_KMAX or LANE1D_KMAX / 2 cells each: the
-// bodies are instantiated for both, and a pencil that the smaller holds runs it), so the
-// results are those of the per-launch plan bit for bit.  (One body per further halving
-// would not fit: each instantiation keeps LDS arrays of its own.)  The time step comes from the batch-wide CFL launch before, the iteration
+// the workgroup (LANE1D_BS threads of LANE1D_KMAX, LANE1D_KMAX / 2 or LANE1D_KMAX / 4
+// cells each: the bodies are instantiated for all three, their per-pencil LDS arrays
+// sized for the one pencil, and a simulation runs the smallest that holds its pencil), so
+// the results are those of the per-launch plan bit for bit.  (Each instantiation keeps LDS
+// arrays of its own: a fourth would not fit.)  The time step comes from the batch-wide CFL launch before, the iteration
 // counters from the batch-wide launch after.  The state stays in the workgroup's L2
 // between the steps.
 #define IMH_FLUID_ESZ 4
@@ -43,7 +44,7 @@ __device__ __forceinline__ void lane1d_run(const Lane1dArgs<T>& a, unsigned char
     // magnetFlux(X=1, I=[3 2]) (order -1).  One call site per body (they are large).
     for (int step = 0; step < 4; ++step) {
         if (step == 0 || step == 3) {
-            fluid_sweep_body<T, LANE1D_BS, KC, true, false>(a.fa, 0);
+            fluid_sweep_body<T, LANE1D_BS, KC, true, false, 1>(a.fa, 0);  // one pencil
         } else {
             const int d1 = step == 1 ? 1 : 2, d2 = 3 - d1;
             for (int j = 0; j < 4; ++j) {
@@ -72,7 +73,9 @@ __global__ void __launch_bounds__(LANE1D_BS) lane1d_kernel(const Lane1dArgs<T>* 
     const Lane1dArgs<T>& a = args[s];
     // cells per thread of this simulation (its fluid tile's K), uniform over the block
     const int kc = __builtin_amdgcn_readfirstlane(a.fa.K);
-    if (kc <= LANE1D_KMAX / 2)
+    if (kc <= LANE1D_KMAX / 4)
+        lane1d_run<T, LANE1D_KMAX / 4>(a, s_mag);
+    else if (kc <= LANE1D_KMAX / 2)
         lane1d_run<T, LANE1D_KMAX / 2>(a, s_mag);
     else
         lane1d_run<T, LANE1D_KMAX>(a, s_mag);
