@@ -222,12 +222,14 @@ static int upload_args_t(imh_batch* b) {
             l.fa = fa[i * 3 + 0];
             l.fa.PX = 1;
             l.fa.TL = LANE1D_BS;
-            // cells per thread: the smallest of the longest pencil's, its half and its
-            // quarter that holds the pencil (the lane kernel has a body for each)
+            // cells per thread: the smallest of a quarter, a half, three quarters and all
+            // of the longest pencil's that holds the pencil (the lane kernel has a body
+            // for each)
             const int N = b->ctx[i]->n[0];
-            l.fa.K = N <= LANE1D_BS * (LANE1D_KMAX / 4)   ? LANE1D_KMAX / 4
-                     : N <= LANE1D_BS * (LANE1D_KMAX / 2) ? LANE1D_KMAX / 2
-                                                           : LANE1D_KMAX;
+            int kc = LANE1D_KMAX;
+            for (int q = 3; q >= 1; --q)
+                if (N <= LANE1D_BS * (q * LANE1D_KMAX / 4)) kc = q * LANE1D_KMAX / 4;
+            l.fa.K = kc;
             l.fa.xmajor = 1;
             for (int dI = 1; dI < 3; ++dI) {
                 l.ma[dI] = ma[i * 9 + 3 * dI + 0];

@@ -14,11 +14,11 @@
 //
 // The bodies are those of the per-launch kernels (fluid_sweep_body, magnet_advect_body,
 // magnet_ct_body, cell_centered_cell), on argument blocks whose tile is one pencil over
-// the workgroup (LANE1D_BS threads of LANE1D_KMAX, LANE1D_KMAX / 2 or LANE1D_KMAX / 4
-// cells each: the bodies are instantiated for all three, their per-pencil LDS arrays
-// sized for the one pencil, and a simulation runs the smallest that holds its pencil), so
-// the results are those of the per-launch plan bit for bit.  (Each instantiation keeps LDS
-// arrays of its own: a fourth would not fit.)  The time step comes from the batch-wide CFL launch before, the iteration
+// the workgroup (LANE1D_BS threads of LANE1D_KMAX, 3/4, 1/2 or 1/4 of it cells each: the
+// bodies are instantiated for all four, their per-pencil LDS arrays sized for the one
+// pencil, and a simulation runs the smallest that holds its pencil), so the results are
+// those of the per-launch plan bit for bit.  (Each instantiation keeps LDS arrays of its
+// own: the four take 152 KB of the 160.)  The time step comes from the batch-wide CFL launch before, the iteration
 // counters from the batch-wide launch after.  The state stays in the workgroup's L2
 // between the steps.
 #define IMH_FLUID_ESZ 4
@@ -77,6 +77,8 @@ __global__ void __launch_bounds__(LANE1D_BS) lane1d_kernel(const Lane1dArgs<T>* 
         lane1d_run<T, LANE1D_KMAX / 4>(a, s_mag);
     else if (kc <= LANE1D_KMAX / 2)
         lane1d_run<T, LANE1D_KMAX / 2>(a, s_mag);
+    else if (kc <= 3 * LANE1D_KMAX / 4)
+        lane1d_run<T, 3 * LANE1D_KMAX / 4>(a, s_mag);
     else
         lane1d_run<T, LANE1D_KMAX>(a, s_mag);
 }
