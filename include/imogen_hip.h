@@ -48,8 +48,9 @@ extern "C" {
 /* shifting/wormhole_shift.m: the lower ghosts of a pencil are the first cells of the pencil
    whose next-dimension coordinate is mirrored (x->y, y->z, z->x), in reverse order and
    negated for the component along the shift; the upper side shifts as const.  3-D grids
-   only (the reference indexes a third dimension); fluid sweeps and the gravity potential
-   (a magnet or artificial viscosity with it is refused). */
+   only (the reference indexes a third dimension); fluid sweeps, the gravity potential and
+   the Neumann-Richtmyer viscosity (a magnet or the Caramana-Shashkov-Whalen viscosity
+   with it is refused). */
 #define IMH_BC_WORMHOLE 8
 
 /* ---- flux limiters (enums/FluxLimiterEnum.m) ---------------------------------------- */
@@ -179,8 +180,9 @@ int imh_set_spacing(imh_ctx *ctx, int dim, const double *line, size_t n);
  * 395-400): type 0 'empty', 1 'neumann_richtmyer' (ArtificialViscosity.m:73-77), 2
  * 'caramana_shashkov_whalen' (ArtificialViscosity.m:80-148, 155-190) with
  * linearViscousStrength and quadraticViscousStrength; its term is added to the pressure
- * of both wFluidFlux calls of every fluid sweep (wFluidFlux.m:24).  Type 2 with fade
- * boundaries returns IMH_ENOTSUP. */
+ * of both wFluidFlux calls of every fluid sweep (wFluidFlux.m:24).  Type 2 at a fade
+ * end takes the multi-launch sweep: 1-D and 2-D grids only (IMH_ENOTSUP on 3-D grids,
+ * with a wormhole boundary, and in a batch). */
 int imh_set_viscosity(imh_ctx *ctx, int type, double linear, double quadratic);
 
 /* ---- time state -------------------------------------------------------------------- */

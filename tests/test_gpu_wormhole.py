@@ -6,8 +6,8 @@ constant shift.  The general sweep holds each such pair in one tile.  Lockstep w
 oracle at <= 1e-12 relative L-inf per iteration (observed 0) on 3-D hydro grids with the
 wormhole on each axis, even and odd partner extents (an odd one pairs its middle pencil
 with itself), the HachisuDisk set-up (z wormhole on both sides, gravity from a fixed
-potential), the exact wave-spread sweep on the other axes, FP32 bulk within 1e-5 of FP64,
-and a batch member equal to the same simulation alone.
+potential), the exact wave-spread sweep on the other axes, Neumann-Richtmyer viscosity,
+FP32 bulk within 1e-5 of FP64, and a batch member equal to the same simulation alone.
 """
 import numpy as np
 import pytest
@@ -40,11 +40,14 @@ def potential(grid):
     return 0.6 * np.sin(2 * np.pi * X) * np.cos(np.pi * Y) + 0.4 * (Z - 0.3) ** 2
 
 
-def lockstep(ini, st, iterations, kind=None, grav=False):
+def lockstep(ini, st, iterations, kind=None, grav=False, visc=None):
     orc = make_oracle(ini, st)
     run = make_gpu(ini, st)
     if kind:
         run.set_fluid_kernel(kind)
+    if visc:
+        orc.set_viscosity(*visc)
+        run.set_viscosity(*visc)
     if grav:
         phi = potential(tuple(ini["grid"]))
         orc.set_gravity(phi, 0.0)
@@ -86,6 +89,17 @@ def test_wormhole_next_to_the_wave_sweep():
     ini, st = box((32, 16, 16), {"x": "wormhole", "y": "circ", "z": "trans"}, seed=5,
                   limiter="superbee")
     lockstep(ini, st, 6, kind="wave")
+
+
+def test_wormhole_with_neumann_richtmyer_viscosity():
+    """The Neumann-Richtmyer term shifts mom(X) by +1 only: the wormhole's upper side
+    (the constant shift).  Caramana-Shashkov-Whalen, which shifts both ways, is refused."""
+    ini, st = box((20, 12, 10), {"x": "wormhole", "y": "circ", "z": "trans"}, seed=21)
+    lockstep(ini, st, 6, visc=("neumann_richtmyer", 0.3, 0.6))
+    run = make_gpu(ini, st)
+    with pytest.raises(Exception, match="wormhole"):
+        run.set_viscosity("caramana_shashkov_whalen", 0.3, 0.6)
+    run.close()
 
 
 def test_wormhole_fp32_close_to_fp64():
