@@ -276,13 +276,13 @@ constexpr int fluid_group() {
 // handles those too, with their code out of line.
 //
 // PXM: the most pencils a tile holds (the size of the per-pencil LDS arrays; the 1-D
-// lane's tiles hold one).
-template <typename T, int BS, int KMAX, bool MHD, bool GEN, int PXM = PXMAX>
+// lane's tiles hold one).  GF > 0: fields per group instead of fluid_group's choice.
+template <typename T, int BS, int KMAX, bool MHD, bool GEN, int PXM = PXMAX, int GF = 0>
 __device__ __forceinline__ void fluid_sweep_body(const FluidArgs<T>& ap, long long tile) {
     const FluidArgs<T> a = ap;  // private copy: stores to the fields cannot alias it
     if (!a.force && !a.sim->active) return;
     using M = FMath<T>;
-    constexpr int G = fluid_group<T, KMAX>();
+    constexpr int G = GF > 0 ? GF : fluid_group<T, KMAX>();
     constexpr int K = KMAX;
     // chunk boundary values of the group: [field*4 + q], q = 0 first FR, 1 last FR,
     // 2 first FL, 3 last FL

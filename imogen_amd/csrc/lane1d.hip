@@ -44,7 +44,8 @@ __device__ __forceinline__ void lane1d_run(const Lane1dArgs<T>& a, unsigned char
     // magnetFlux(X=1, I=[3 2]) (order -1).  One call site per body (they are large).
     for (int step = 0; step < 4; ++step) {
         if (step == 0 || step == 3) {
-            fluid_sweep_body<T, LANE1D_BS, KC, true, false, 1>(a.fa, 0);  // one pencil
+            // one pencil; one field per group for the longer chunks (fewer registers)
+            fluid_sweep_body<T, LANE1D_BS, KC, true, false, 1, (KC > 4 ? 1 : 0)>(a.fa, 0);
         } else {
             const int d1 = step == 1 ? 1 : 2, d2 = 3 - d1;
             for (int j = 0; j < 4; ++j) {
