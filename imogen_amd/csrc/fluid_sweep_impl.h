@@ -33,14 +33,6 @@ template <int V>
 using FieldIdx = std::integral_constant<int, V>;
 
 
-// Block index -> tile index so that consecutive tiles run on the same XCD (blocks are
-// dispatched round-robin over the 8 XCDs; neighbouring pencils then share an L2).
-__device__ inline long long xcd_tile(long long b, long long nblocks) {
-    long long q = nblocks / 8, r = nblocks % 8;
-    long long x = b % 8, t = b / 8;
-    return x * q + (x < r ? x : r) + t;
-}
-
 // Ghost value of one shift call by one cell at a pencil end (the BC rule of that flux
 // object, with its tracker).  Out of line: only pencil-end threads need it, and keeping
 // the BC code out of every call site keeps the kernel inside the instruction cache.

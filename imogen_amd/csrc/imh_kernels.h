@@ -60,12 +60,6 @@ struct BatchTile {
     long long tile;
 };
 
-// Schedule of a persistent launch over a tile list: the list is cut into 8 contiguous
-// ranges, one per XCD (blocks are dispatched round-robin over the XCDs, block b runs on
-// XCD b % 8), and the blocks of an XCD walk its range together, so tiles that neighbour
-// in memory (adjacent pencils of one simulation) are in flight on the same L2.  Returns
-// the tile of round r of this block, or -1 when the block is done.  A grid that is not a
-// multiple of 8 falls back to the grid-stride order.
 // A tile-list entry read so that the compiler knows it is the same in every lane (the
 // block works on one tile): the argument-block index and tile go to scalar registers,
 // and the argument block's fields are then read as uniform values rather than as
@@ -79,6 +73,20 @@ __device__ __forceinline__ void uniform_tile(const BatchTile* tiles, int t, int&
     tile = (long long)(((unsigned long long)hi << 32) | lo);
 }
 
+// Block index -> tile index so that consecutive tiles run on the same XCD (blocks are
+// dispatched round-robin over the 8 XCDs; neighbouring pencils then share an L2).
+__device__ inline long long xcd_tile(long long b, long long nblocks) {
+    long long q = nblocks / 8, r = nblocks % 8;
+    long long x = b % 8, t = b / 8;
+    return x * q + (x < r ? x : r) + t;
+}
+
+// Schedule of a persistent launch over a tile list: the list is cut into 8 contiguous
+// ranges, one per XCD (blocks are dispatched round-robin over the XCDs, block b runs on
+// XCD b % 8), and the blocks of an XCD walk its range together, so tiles that neighbour
+// in memory (adjacent pencils of one simulation) are in flight on the same L2.  Returns
+// the tile of round r of this block, or -1 when the block is done.  A grid that is not a
+// multiple of 8 falls back to the grid-stride order.
 __device__ __forceinline__ int xcd_round_tile(int r, int ntiles) {
     const int G = gridDim.x, b = blockIdx.x;
     if (G % 8 != 0) {
