@@ -1026,7 +1026,7 @@ __global__ void __launch_bounds__(BS) IMH_FLUID_OCCUPANCY fluid_sweep_batch_kern
             uniform_tile(tiles, t, arg, tile);
         } else {
             if (r > 0 || (int)blockIdx.x >= ntiles) break;
-            tile = args[0].xcd_remap ? xcd_tile(blockIdx.x, gridDim.x) : (long long)blockIdx.x;
+            tile = args[0].xcd_remap ? xcd_tile(blockIdx.x, ntiles) : (long long)blockIdx.x;
         }
         fluid_sweep_body<T, BS, KMAX, MHD, GEN>(args[arg], tile);
         __syncthreads();

@@ -734,7 +734,7 @@ fluid_wave_kernel(const FluidArgs<T>* args,
         } else {
             if (r > 0 || (int)blockIdx.x >= ntiles) break;
             // y / z tiles of neighbouring x pencils share 128-byte lines: same XCD, same L2
-            tile = args[0].xcd_remap ? xcd_tile(blockIdx.x, gridDim.x) : (long long)blockIdx.x;
+            tile = args[0].xcd_remap ? xcd_tile(blockIdx.x, ntiles) : (long long)blockIdx.x;
         }
         wv::wave_sweep_body<T, BS, MHD, FAST, D>(args[arg], tile, wv_lds);
         __syncthreads();

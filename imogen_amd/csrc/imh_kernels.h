@@ -74,7 +74,8 @@ __device__ __forceinline__ void uniform_tile(const BatchTile* tiles, int t, int&
 }
 
 // Block index -> tile index so that consecutive tiles run on the same XCD (blocks are
-// dispatched round-robin over the 8 XCDs; neighbouring pencils then share an L2).
+// dispatched round-robin over the 8 XCDs; neighbouring pencils then share an L2).  A
+// bijection of 0..nblocks-1; launches of one block per tile pass their tile count.
 __device__ inline long long xcd_tile(long long b, long long nblocks) {
     long long q = nblocks / 8, r = nblocks % 8;
     long long x = b % 8, t = b / 8;

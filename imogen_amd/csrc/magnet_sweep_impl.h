@@ -701,7 +701,7 @@ __device__ inline bool next_tile(const BatchTile* tiles, int ntiles, int r, int&
     if (!tiles) {
         if (r > 0 || (int)blockIdx.x >= ntiles) return false;
         arg = 0;
-        tile = blockIdx.x;
+        tile = xcd_tile(blockIdx.x, ntiles);  // neighbouring tiles on one XCD
         return true;
     }
     const int t = xcd_round_tile(r, ntiles);
