@@ -303,7 +303,7 @@ __global__ void __launch_bounds__(COL_BS) fluid_column_kernel(const FluidArgs<T>
             uniform_tile(tiles, t, arg, tile);
         } else {
             if (r > 0 || (int)blockIdx.x >= ntiles) break;
-            tile = blockIdx.x;
+            tile = xcd_tile(blockIdx.x, ntiles);  // neighbouring tiles on one XCD
         }
         fluid_column_body<T, MHD>(args[arg], tile);
     }
